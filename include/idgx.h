@@ -1,0 +1,133 @@
+/* idgx — IDG gridding/degridding hot path for AMD Instinct MI355X (gfx950), C ABI.
+ *
+ * Every entry point takes plain pointers and sizes.  Arrays prefixed d_ live in device memory (HBM) and must
+ * stay valid until the work queued on `stream` has finished; h_ arrays are host memory, borrowed for the call.
+ * `stream` is a hipStream_t (NULL = the default stream).  Nothing here calls exit(): every function returns an
+ * idgx_status, and idgx_strerror() names it.
+ *
+ * Array layouts are the reference benchmark's (gridding/inc/types.hpp, gridding/inc/common_gridding.cuh:7-26):
+ *   uvw            float[nr_rows][3]                 metres
+ *   wavenumbers    float[nr_channels]                2*pi*f/c, rad/m
+ *   visibilities   complex64[nr_rows][nr_channels][4]  {XX, XY, YX, YY}; row = time_offset + t
+ *   spheroidal     float[S][S]                       taper
+ *   aterms         complex64[nr_aterm_slots][nr_stations][S][S][4]   2x2 Jones {xx, xy, yx, yy}
+ *   subgrids       complex64[nr_subgrids][4][S][S]
+ *   metadata       idgx_metadata[nr_subgrids]        the reference's 36-byte Metadata record
+ *   channels       idgx_channels[nr_subgrids]        optional per-subgrid channel window (NULL = all)
+ *
+ * Subgrid-local pixel coordinates and phase offsets follow the reference: l = (x + 0.5 - S/2) * image_size / S,
+ * u_offset = (x0 + S/2 - G/2) * 2*pi / image_size (gridding/inc/math.hpp:20-37,
+ * gridding/src/gridding_benchmark.cpp:74-81).
+ */
+#ifndef IDGX_H
+#define IDGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Plan record, layout-identical to the reference's Metadata (gridding/inc/types.hpp:7-25). */
+typedef struct
+{
+    int32_t baseline_offset;
+    int32_t time_offset;
+    int32_t nr_timesteps;
+    int32_t aterm_index;
+    uint32_t station1;
+    uint32_t station2;
+    int32_t x, y, z; /* subgrid corner on the master grid; z = w-layer */
+} idgx_metadata;
+
+/* Channel window of one subgrid: channels [channel_begin, channel_begin + nr_channels). */
+typedef struct
+{
+    int32_t channel_begin;
+    int32_t nr_channels;
+} idgx_channels;
+
+typedef enum
+{
+    IDGX_OK = 0,
+    IDGX_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad size, inconsistent plan */
+    IDGX_ERR_INVALID_METADATA = 2, /* a subgrid reads rows/channels/stations/A-terms outside the arrays */
+    IDGX_ERR_UNSUPPORTED = 3,      /* valid request this build does not implement (e.g. subgrid size) */
+    IDGX_ERR_DEVICE = 4,           /* HIP runtime error (launch, copy, event) */
+    IDGX_ERR_OUT_OF_MEMORY = 5
+} idgx_status;
+
+typedef enum
+{
+    IDGX_PLAIN_FP32 = 0,   /* FP32 throughout, phases rounded exactly as the reference CPU code */
+    IDGX_MIXED_BF16X3 = 1, /* phasor x data contraction on BF16 MFMA, hi/lo split, 3 products, FP32 accumulate */
+    IDGX_MIXED_FP16X3 = 2, /* same on FP16 MFMA */
+    IDGX_MIXED_BF16 = 3,   /* single BF16 product (accuracy reference point) */
+    IDGX_MIXED_FP16 = 4    /* single FP16 product */
+} idgx_precision;
+
+typedef enum
+{
+    IDGX_ATERM_ADJOINT = 0,  /* degridder applies A1 * P * A2^H: the exact adjoint of the gridder */
+    IDGX_ATERM_REFERENCE = 1 /* degridder applies A1^H * P * A2, as the reference CPU degridder does
+                                (gridding/src/degridding_benchmark.cpp:113) */
+} idgx_aterm_convention;
+
+typedef enum
+{
+    /* Plain FP32 path: evaluate sin/cos with the ROCm device library's sincosf (full-range reduction, ~1 ulp)
+     * instead of the default FP32 Cody-Waite reduction to [-pi, pi] followed by v_sin_f32/v_cos_f32. */
+    IDGX_FLAG_ACCURATE_SINCOS = 1
+} idgx_flags;
+
+typedef struct
+{
+    int32_t nr_subgrids;
+    int32_t subgrid_size;      /* S; 8 <= S <= 128, even */
+    int32_t grid_size;         /* G */
+    int32_t nr_channels;       /* channels per visibility row; length of wavenumbers */
+    int32_t nr_stations;
+    int32_t nr_aterm_slots;
+    int64_t nr_rows;           /* rows of uvw / visibilities */
+    int32_t max_nr_timesteps;  /* upper bound of metadata[].nr_timesteps (sizes the launch) */
+    float image_size;          /* radians */
+    float w_step_in_lambda;    /* w-layer spacing; 0 = no w-layers (the reference's value) */
+    int32_t precision;         /* idgx_precision */
+    int32_t aterm_convention;  /* idgx_aterm_convention, degridder only */
+    int32_t flags;             /* bitwise OR of idgx_flags */
+} idgx_plan;
+
+/* Checks a plan and its host-side metadata: every subgrid must read rows, channels, stations and A-term
+ * slots inside the arrays the plan describes.  The GPU entry points do not read device metadata on the host,
+ * so callers validate with this first; the kernels additionally skip (and zero) any subgrid that would read
+ * out of bounds. */
+int idgx_check_plan(const idgx_plan *plan, const idgx_metadata *h_metadata, const idgx_channels *h_channels);
+
+/* Gridder: subgrids[s] = taper * A1^H (sum_{t,c} V[t,c] e^{i phase}) A2, phase = phase_offset - k_c (u l + v m + w n).
+ * Replaces gridding_benchmark_launcher<gpu>::launch's kernel (gridding/src/gridding_benchmark.cu:19-246,326)
+ * and the CPU loop it mirrors (gridding/src/gridding_benchmark.cpp:56-165).  Subgrids are overwritten. */
+int idgx_grid(const idgx_plan *plan, const idgx_metadata *d_metadata, const idgx_channels *d_channels,
+              const float *d_uvw, const float *d_wavenumbers, const void *d_visibilities, const float *d_spheroidal,
+              const void *d_aterms, void *d_subgrids, void *stream);
+
+/* Degridder: V[t,c] = sum_pixels e^{-i phase} A1 (taper * P) A2^H  (the adjoint of idgx_grid; with
+ * IDGX_ATERM_REFERENCE the A-term is A1^H P A2 as in the reference CPU path).  Replaces
+ * degridding_benchmark_launcher<gpu>::launch's kernel (gridding/src/degridding_benchmark.cu:106-203,283) and the
+ * CPU loop (gridding/src/degridding_benchmark.cpp:58-181).  Only the (row, channel) entries the plan covers are
+ * written; the rest of d_visibilities is left untouched. */
+int idgx_degrid(const idgx_plan *plan, const idgx_metadata *d_metadata, const idgx_channels *d_channels,
+                const float *d_uvw, const float *d_wavenumbers, const void *d_subgrids, const float *d_spheroidal,
+                const void *d_aterms, void *d_visibilities, void *stream);
+
+/* Name of a status code. */
+const char *idgx_strerror(int status);
+
+/* Library version string. */
+const char *idgx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IDGX_H */

@@ -1,0 +1,330 @@
+// Plain FP32 gridder and degridder for gfx950.
+//
+// Phases are rounded exactly as in the reference CPU path (gridding/src/gridding_benchmark.cpp:99-121,
+// gridding/src/degridding_benchmark.cpp:157-166): the whole file is compiled with contraction off, so
+// `a * b + c` rounds twice; the complex multiply-accumulates are written as explicit fused multiply-adds.
+//
+// Gridder: one workgroup of 256 threads owns 1024 pixels of one subgrid (4 per thread, 4 correlations each,
+//   held in VGPRs).  Visibilities and uvw of the subgrid stream through LDS in batches of 256 visibilities;
+//   every lane reads the same LDS address per visibility (broadcast, conflict-free).  Taper and A-term are
+//   applied in the epilogue, which overwrites the subgrid.
+// Degridder: one workgroup owns 256 (timestep, channel-chunk) items of one subgrid, each thread one timestep x
+//   up to 4 channels.  Tapered, A-term-corrected pixels plus (l, m, n, phase offset) are staged through LDS in
+//   batches of 512 pixels; each thread sums its visibilities over all pixels in registers, so no cross-lane
+//   reduction is needed.
+#include "idgx_device.hpp"
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace idgx
+{
+namespace
+{
+constexpr int kThreads = 256;
+constexpr int kGridBatchVis = 256; // visibilities per LDS stage
+constexpr int kGridChunk = 64;     // channels per chunk (bounds the per-stage uvw count at 256 timesteps)
+constexpr int kDegridPixBatch = 512;
+
+// Visibility x phasor accumulated into acc[0..7] = {XX.re, XX.im, XY.re, ..., YY.im}.
+__device__ __forceinline__ void cmac4(float *acc, float4 va, float4 vb, float cs, float sn)
+{
+    acc[0] = __builtin_fmaf(va.x, cs, acc[0]);
+    acc[0] = __builtin_fmaf(-va.y, sn, acc[0]);
+    acc[1] = __builtin_fmaf(va.x, sn, acc[1]);
+    acc[1] = __builtin_fmaf(va.y, cs, acc[1]);
+    acc[2] = __builtin_fmaf(va.z, cs, acc[2]);
+    acc[2] = __builtin_fmaf(-va.w, sn, acc[2]);
+    acc[3] = __builtin_fmaf(va.z, sn, acc[3]);
+    acc[3] = __builtin_fmaf(va.w, cs, acc[3]);
+    acc[4] = __builtin_fmaf(vb.x, cs, acc[4]);
+    acc[4] = __builtin_fmaf(-vb.y, sn, acc[4]);
+    acc[5] = __builtin_fmaf(vb.x, sn, acc[5]);
+    acc[5] = __builtin_fmaf(vb.y, cs, acc[5]);
+    acc[6] = __builtin_fmaf(vb.z, cs, acc[6]);
+    acc[6] = __builtin_fmaf(-vb.w, sn, acc[6]);
+    acc[7] = __builtin_fmaf(vb.z, sn, acc[7]);
+    acc[7] = __builtin_fmaf(vb.w, cs, acc[7]);
+}
+
+__device__ __forceinline__ void channel_window(const KernelArgs &a, int s, int &c0, int &nc)
+{
+    if (a.channels)
+    {
+        const idgx_channels w = a.channels[s];
+        c0 = w.channel_begin;
+        nc = w.nr_channels;
+    }
+    else
+    {
+        c0 = 0;
+        nc = a.p.nr_channels;
+    }
+}
+
+template <int NPIX, int SC> __global__ __launch_bounds__(kThreads) void gridder_plain(KernelArgs a)
+{
+    __shared__ float4 s_vis[2 * kGridBatchVis];
+    __shared__ float4 s_uvw[kGridBatchVis];
+    __shared__ float s_wn[kGridChunk];
+
+    const int s = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int S = a.p.S;
+    const int P = S * S;
+    const idgx_metadata m = a.metadata[s];
+    int c0, nc;
+    channel_window(a, s, c0, nc);
+    const long long row0 = first_row(a.metadata, m);
+    const bool ok = subgrid_in_bounds(a.p, m, row0, c0, nc);
+    const long long C = a.p.nr_channels;
+    const int pbase = blockIdx.y * (kThreads * NPIX);
+
+    const Offsets off = gridder_offsets(m, S, a.p.G, a.p.image_size, a.p.w_step);
+    float l[NPIX], mm[NPIX], n[NPIX], poff[NPIX];
+    float acc[NPIX][8];
+#pragma unroll
+    for (int i = 0; i < NPIX; ++i)
+    {
+        const int p = min(pbase + i * kThreads + tid, P - 1);
+        const int x = p % S, y = p / S;
+        l[i] = pixel_lm(x, S, a.p.image_size);
+        mm[i] = pixel_lm(y, S, a.p.image_size);
+        n[i] = pixel_n(l[i], mm[i]);
+        poff[i] = (off.u * l[i] + off.v * mm[i]) + off.w * n[i];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            acc[i][k] = 0.F;
+    }
+
+    if (ok)
+    {
+        const float4 *vis = reinterpret_cast<const float4 *>(a.visibilities);
+        for (int cb = 0; cb < nc; cb += kGridChunk)
+        {
+            const int cc = min(kGridChunk, nc - cb);
+            const int tb_max = kGridBatchVis / cc;
+            for (int t0 = 0; t0 < m.nr_timesteps; t0 += tb_max)
+            {
+                const int tb = min(tb_max, m.nr_timesteps - t0);
+                __syncthreads(); // previous stage fully consumed
+                if (tid < cc)
+                    s_wn[tid] = a.wavenumbers[c0 + cb + tid];
+                for (int q = tid; q < tb; q += kThreads)
+                {
+                    const float *uv = a.uvw + 3 * (row0 + t0 + q);
+                    s_uvw[q] = make_float4(uv[0], uv[1], uv[2], 0.F);
+                }
+                for (int q = tid; q < 2 * tb * cc; q += kThreads)
+                {
+                    const int item = q >> 1;
+                    const int t = item / cc, c = item - t * cc;
+                    s_vis[q] = vis[((row0 + t0 + t) * C + c0 + cb + c) * 2 + (q & 1)];
+                }
+                __syncthreads();
+                for (int t = 0; t < tb; ++t)
+                {
+                    const float4 uv = s_uvw[t];
+                    float pidx[NPIX];
+#pragma unroll
+                    for (int i = 0; i < NPIX; ++i)
+                        pidx[i] = (uv.x * l[i] + uv.y * mm[i]) + uv.z * n[i];
+                    for (int c = 0; c < cc; ++c)
+                    {
+                        const float k = s_wn[c];
+                        const float4 va = s_vis[2 * (t * cc + c)];
+                        const float4 vb = s_vis[2 * (t * cc + c) + 1];
+#pragma unroll
+                        for (int i = 0; i < NPIX; ++i)
+                        {
+                            const float phase = poff[i] - pidx[i] * k;
+                            float sn, cs;
+                            sincos_rad<SC>(phase, sn, cs);
+                            cmac4(acc[i], va, vb, cs, sn);
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // Epilogue: P <- taper * A1^H P A2, overwrite the subgrid (zeros for a subgrid rejected above).
+    float2 *out = reinterpret_cast<float2 *>(a.subgrids_out) + static_cast<size_t>(s) * kCorr * P;
+    const float4 *aterms = reinterpret_cast<const float4 *>(a.aterms);
+#pragma unroll
+    for (int i = 0; i < NPIX; ++i)
+    {
+        const int p = pbase + i * kThreads + tid;
+        if (p >= P)
+            continue;
+        cfloat px[4] = {{acc[i][0], acc[i][1]}, {acc[i][2], acc[i][3]}, {acc[i][4], acc[i][5]}, {acc[i][6], acc[i][7]}};
+        if (ok)
+        {
+            const size_t base = static_cast<size_t>(m.aterm_index) * a.p.nr_stations;
+            cfloat a1[4], a2[4];
+            load_jones(aterms, (base + m.station1) * P + p, a1);
+            load_jones(aterms, (base + m.station2) * P + p, a2);
+            apply_a1h_p_a2(px, a1, a2);
+            const float taper = a.spheroidal[p];
+#pragma unroll
+            for (int pol = 0; pol < 4; ++pol)
+                out[pol * P + p] = make_float2(px[pol].re * taper, px[pol].im * taper);
+        }
+        else
+        {
+#pragma unroll
+            for (int pol = 0; pol < 4; ++pol)
+                out[pol * P + p] = make_float2(0.F, 0.F);
+        }
+    }
+}
+
+template <int NCH, int SC> __global__ __launch_bounds__(kThreads) void degridder_plain(KernelArgs a)
+{
+    __shared__ float4 s_pix[2 * kDegridPixBatch];
+    __shared__ float4 s_geo[kDegridPixBatch]; // l, m, n, phase offset
+
+    const int s = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int S = a.p.S;
+    const int P = S * S;
+    const idgx_metadata m = a.metadata[s];
+    int c0, nc;
+    channel_window(a, s, c0, nc);
+    const long long row0 = first_row(a.metadata, m);
+    if (!subgrid_in_bounds(a.p, m, row0, c0, nc) || nc == 0)
+        return; // uniform: nothing of this subgrid is written
+    const int nchunks = (nc + NCH - 1) / NCH;
+    if (static_cast<long long>(blockIdx.y) * kThreads >= static_cast<long long>(m.nr_timesteps) * nchunks)
+        return; // uniform: no item of this block exists
+
+    const int item = blockIdx.y * kThreads + tid;
+    const int t = item / nchunks;
+    const int cj = (item - t * nchunks) * NCH;
+    const bool active = t < m.nr_timesteps;
+    const long long C = a.p.nr_channels;
+
+    float u = 0.F, v = 0.F, w = 0.F;
+    float k[NCH];
+#pragma unroll
+    for (int j = 0; j < NCH; ++j)
+        k[j] = 0.F;
+    if (active)
+    {
+        const float *uv = a.uvw + 3 * (row0 + t);
+        u = uv[0];
+        v = uv[1];
+        w = uv[2];
+#pragma unroll
+        for (int j = 0; j < NCH; ++j)
+            if (cj + j < nc)
+                k[j] = a.wavenumbers[c0 + cj + j];
+    }
+    float acc[NCH][8];
+#pragma unroll
+    for (int j = 0; j < NCH; ++j)
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            acc[j][q] = 0.F;
+
+    const Offsets off = degridder_offsets(m, S, a.p.G, a.p.image_size, a.p.w_step);
+    const float2 *sub = reinterpret_cast<const float2 *>(a.subgrids) + static_cast<size_t>(s) * kCorr * P;
+    const float4 *aterms = reinterpret_cast<const float4 *>(a.aterms);
+    const size_t abase = static_cast<size_t>(m.aterm_index) * a.p.nr_stations;
+
+    for (int pb = 0; pb < P; pb += kDegridPixBatch)
+    {
+        const int np = min(kDegridPixBatch, P - pb);
+        __syncthreads();
+        for (int q = tid; q < np; q += kThreads)
+        {
+            const int p = pb + q;
+            const int x = p % S, y = p / S;
+            const float taper = a.spheroidal[p];
+            cfloat px[4];
+#pragma unroll
+            for (int pol = 0; pol < 4; ++pol)
+            {
+                const float2 g = sub[pol * P + p];
+                px[pol] = {taper * g.x, taper * g.y};
+            }
+            cfloat a1[4], a2[4];
+            load_jones(aterms, (abase + m.station1) * P + p, a1);
+            load_jones(aterms, (abase + m.station2) * P + p, a2);
+            if (a.p.convention == IDGX_ATERM_REFERENCE)
+                apply_a1h_p_a2(px, a1, a2);
+            else
+                apply_a1_p_a2h(px, a1, a2);
+            s_pix[2 * q] = make_float4(px[0].re, px[0].im, px[1].re, px[1].im);
+            s_pix[2 * q + 1] = make_float4(px[2].re, px[2].im, px[3].re, px[3].im);
+            const float l = pixel_lm(x, S, a.p.image_size);
+            const float mm = pixel_lm(y, S, a.p.image_size);
+            const float n = pixel_n(l, mm);
+            s_geo[q] = make_float4(l, mm, n, (off.u * l + off.v * mm) + off.w * n);
+        }
+        __syncthreads();
+        if (active)
+        {
+            for (int q = 0; q < np; ++q)
+            {
+                const float4 g = s_geo[q];
+                const float4 pa = s_pix[2 * q];
+                const float4 pbv = s_pix[2 * q + 1];
+                const float pidx = (u * g.x + v * g.y) + w * g.z;
+#pragma unroll
+                for (int j = 0; j < NCH; ++j)
+                {
+                    if (cj + j < nc)
+                    {
+                        const float phase = pidx * k[j] - g.w;
+                        float sn, cs;
+                        sincos_rad<SC>(phase, sn, cs);
+                        cmac4(acc[j], pa, pbv, cs, sn);
+                    }
+                }
+            }
+        }
+    }
+    if (active)
+    {
+        float4 *out = reinterpret_cast<float4 *>(a.visibilities_out);
+#pragma unroll
+        for (int j = 0; j < NCH; ++j)
+            if (cj + j < nc)
+            {
+                const size_t o = ((row0 + t) * C + c0 + cj + j) * 2;
+                out[o] = make_float4(acc[j][0], acc[j][1], acc[j][2], acc[j][3]);
+                out[o + 1] = make_float4(acc[j][4], acc[j][5], acc[j][6], acc[j][7]);
+            }
+    }
+}
+} // namespace
+
+constexpr int kGridPix = 4;
+constexpr int kDegridCh = 4;
+
+hipError_t launch_grid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream)
+{
+    const int P = a.p.S * a.p.S;
+    const dim3 grid(a.p.nr_subgrids, (P + kThreads * kGridPix - 1) / (kThreads * kGridPix));
+    if (accurate_sincos)
+        gridder_plain<kGridPix, kSincosOcml><<<grid, kThreads, 0, stream>>>(a);
+    else
+        gridder_plain<kGridPix, kSincosHw><<<grid, kThreads, 0, stream>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_degrid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream)
+{
+    const long long items = static_cast<long long>(a.p.max_nr_timesteps) * ((a.p.nr_channels + kDegridCh - 1) / kDegridCh);
+    const long long blocks = (items + kThreads - 1) / kThreads;
+    if (blocks > 65535)
+        return hipErrorInvalidConfiguration;
+    const dim3 grid(a.p.nr_subgrids, static_cast<unsigned>(blocks > 0 ? blocks : 1));
+    if (accurate_sincos)
+        degridder_plain<kDegridCh, kSincosOcml><<<grid, kThreads, 0, stream>>>(a);
+    else
+        degridder_plain<kDegridCh, kSincosHw><<<grid, kThreads, 0, stream>>>(a);
+    return hipGetLastError();
+}
+} // namespace idgx

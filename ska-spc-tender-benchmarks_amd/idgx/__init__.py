@@ -1,0 +1,175 @@
+"""Python binding of libidgx (the MI355X IDG gridding/degridding library) over its C ABI (include/idgx.h).
+
+Device buffers are passed as raw device pointers; torch tensors on a ROCm device are accepted anywhere a device
+pointer is expected (their data_ptr() is used), torch being only the allocator and stream provider here.  The
+library itself is HIP code: nothing in this module computes gridding results, and loading fails loudly when the
+built library is missing.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "lib", "libidgx.so")
+HEADER = os.path.join(ROOT, "include", "idgx.h")
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_INVALID_METADATA = 2
+ERR_UNSUPPORTED = 3
+ERR_DEVICE = 4
+ERR_OUT_OF_MEMORY = 5
+
+PLAIN_FP32 = 0
+MIXED_BF16X3 = 1
+MIXED_FP16X3 = 2
+MIXED_BF16 = 3
+MIXED_FP16 = 4
+PRECISIONS = {"fp32": PLAIN_FP32, "bf16x3": MIXED_BF16X3, "fp16x3": MIXED_FP16X3, "bf16": MIXED_BF16,
+              "fp16": MIXED_FP16}
+
+ATERM_ADJOINT = 0
+ATERM_REFERENCE = 1
+
+FLAG_ACCURATE_SINCOS = 1
+
+METADATA_DTYPE = np.dtype(
+    [
+        ("baseline_offset", "<i4"),
+        ("time_offset", "<i4"),
+        ("nr_timesteps", "<i4"),
+        ("aterm_index", "<i4"),
+        ("station1", "<u4"),
+        ("station2", "<u4"),
+        ("x", "<i4"),
+        ("y", "<i4"),
+        ("z", "<i4"),
+    ]
+)
+CHANNELS_DTYPE = np.dtype([("channel_begin", "<i4"), ("nr_channels", "<i4")])
+
+
+class Plan(ctypes.Structure):
+    """idgx_plan (include/idgx.h)."""
+
+    _fields_ = [
+        ("nr_subgrids", ctypes.c_int32),
+        ("subgrid_size", ctypes.c_int32),
+        ("grid_size", ctypes.c_int32),
+        ("nr_channels", ctypes.c_int32),
+        ("nr_stations", ctypes.c_int32),
+        ("nr_aterm_slots", ctypes.c_int32),
+        ("nr_rows", ctypes.c_int64),
+        ("max_nr_timesteps", ctypes.c_int32),
+        ("image_size", ctypes.c_float),
+        ("w_step_in_lambda", ctypes.c_float),
+        ("precision", ctypes.c_int32),
+        ("aterm_convention", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+    ]
+
+
+class IdgxError(RuntimeError):
+    def __init__(self, status, what):
+        super().__init__(f"{what}: {strerror(status)} ({status})")
+        self.status = status
+
+
+_lib = None
+
+
+def build():
+    """Compile libidgx for gfx950 in-tree (make -C <package>)."""
+    subprocess.check_call(["make", "-s", "-C", PKG, "-j8"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libidgx is not built ({LIB_PATH} missing): run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.idgx_check_plan.argtypes = [ctypes.POINTER(Plan), vp, vp]
+        L.idgx_grid.argtypes = [ctypes.POINTER(Plan)] + [vp] * 9
+        L.idgx_degrid.argtypes = [ctypes.POINTER(Plan)] + [vp] * 9
+        L.idgx_strerror.restype = ctypes.c_char_p
+        L.idgx_strerror.argtypes = [ctypes.c_int]
+        L.idgx_version.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def strerror(status):
+    return lib().idgx_strerror(int(status)).decode()
+
+
+def exported_functions():
+    """Names of the functions include/idgx.h declares."""
+    import re
+
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(idgx_\w+)\s*\(", text, re.M)))
+
+
+def _dptr(x):
+    """Device pointer of a torch tensor, an int address, or None."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return ctypes.c_void_p(x)
+    return ctypes.c_void_p(x.data_ptr())
+
+
+def _hptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def make_plan(*, nr_subgrids, subgrid_size, grid_size, nr_channels, nr_stations, nr_aterm_slots, nr_rows,
+              max_nr_timesteps, image_size, w_step=0.0, precision=PLAIN_FP32, aterm_convention=ATERM_ADJOINT,
+              flags=0):
+    if isinstance(precision, str):
+        precision = PRECISIONS[precision]
+    return Plan(int(nr_subgrids), int(subgrid_size), int(grid_size), int(nr_channels), int(nr_stations),
+                int(nr_aterm_slots), int(nr_rows), int(max_nr_timesteps), float(image_size), float(w_step),
+                int(precision), int(aterm_convention), int(flags))
+
+
+def check_plan(plan, metadata, channels=None):
+    """Host-side validation of a plan against its metadata (numpy arrays); returns the status code."""
+    assert metadata.dtype == METADATA_DTYPE and metadata.flags.c_contiguous
+    if channels is not None:
+        assert channels.dtype == CHANNELS_DTYPE and channels.flags.c_contiguous
+    return lib().idgx_check_plan(ctypes.byref(plan), _hptr(metadata), _hptr(channels))
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def grid(plan, d_metadata, d_channels, d_uvw, d_wavenumbers, d_visibilities, d_spheroidal, d_aterms, d_subgrids,
+         stream=None):
+    """idgx_grid on device buffers; raises IdgxError on a non-zero status."""
+    rc = lib().idgx_grid(ctypes.byref(plan), _dptr(d_metadata), _dptr(d_channels), _dptr(d_uvw),
+                         _dptr(d_wavenumbers), _dptr(d_visibilities), _dptr(d_spheroidal), _dptr(d_aterms),
+                         _dptr(d_subgrids), _stream(stream))
+    if rc != OK:
+        raise IdgxError(rc, "idgx_grid")
+
+
+def degrid(plan, d_metadata, d_channels, d_uvw, d_wavenumbers, d_subgrids, d_spheroidal, d_aterms, d_visibilities,
+           stream=None):
+    """idgx_degrid on device buffers; raises IdgxError on a non-zero status."""
+    rc = lib().idgx_degrid(ctypes.byref(plan), _dptr(d_metadata), _dptr(d_channels), _dptr(d_uvw),
+                           _dptr(d_wavenumbers), _dptr(d_subgrids), _dptr(d_spheroidal), _dptr(d_aterms),
+                           _dptr(d_visibilities), _stream(stream))
+    if rc != OK:
+        raise IdgxError(rc, "idgx_degrid")

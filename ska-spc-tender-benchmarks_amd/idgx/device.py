@@ -1,0 +1,78 @@
+"""Device residency for a workload: upload the host arrays once, then run the C-ABI gridder/degridder on them.
+
+torch provides the HBM allocations and the stream (plumbing only); every result is computed by libidgx.
+"""
+import numpy as np
+
+from . import METADATA_DTYPE, CHANNELS_DTYPE, ATERM_ADJOINT, PLAIN_FP32, make_plan, check_plan, grid, degrid, \
+    IdgxError, OK
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def upload(a, device="cuda"):
+    """Copy a numpy array to a flat uint8 device tensor (keeps any dtype, including complex and structs)."""
+    torch = _torch()
+    flat = np.ascontiguousarray(a).reshape(-1).view(np.uint8)
+    return torch.from_numpy(flat.copy()).to(device)
+
+
+def download(t, dtype, shape):
+    return t.cpu().numpy().view(dtype).reshape(shape)
+
+
+class DeviceWorkload:
+    """Inputs of one workload resident in HBM, plus output buffers for both directions."""
+
+    def __init__(self, w, device="cuda", visibilities_on_device=None):
+        torch = _torch()
+        self.w = w
+        self.device = device
+        self.S = int(w["subgrid_size"])
+        self.n = int(w["nr_subgrids"])
+        self.C = int(w["wavenumbers"].shape[0])
+        self.nr_rows = int(w["uvw"].shape[0])
+        self.d_metadata = upload(w["metadata"], device)
+        self.d_channels = None if w.get("channels") is None else upload(w["channels"], device)
+        self.d_uvw = upload(w["uvw"], device)
+        self.d_wavenumbers = upload(w["wavenumbers"], device)
+        self.d_spheroidal = upload(w["spheroidal"], device)
+        self.d_aterms = upload(w["aterms"], device)
+        if visibilities_on_device is not None:
+            self.d_visibilities = visibilities_on_device
+        else:
+            self.d_visibilities = upload(w["visibilities"], device)
+        self.d_subgrids = torch.zeros(max(self.n, 1) * 4 * self.S * self.S * 8, dtype=torch.uint8, device=device)
+        self.d_vis_out = torch.zeros(self.nr_rows * self.C * 4 * 8, dtype=torch.uint8, device=device)
+
+    def plan(self, precision=PLAIN_FP32, aterm_convention=ATERM_ADJOINT, flags=0):
+        w = self.w
+        p = make_plan(nr_subgrids=self.n, subgrid_size=self.S, grid_size=w["grid_size"], nr_channels=self.C,
+                      nr_stations=w["nr_stations"], nr_aterm_slots=w["nr_aterm_slots"], nr_rows=self.nr_rows,
+                      max_nr_timesteps=w["max_nr_timesteps"], image_size=w["image_size"], w_step=w["w_step"],
+                      precision=precision, aterm_convention=aterm_convention, flags=flags)
+        rc = check_plan(p, w["metadata"], w.get("channels"))
+        if rc != OK:
+            raise IdgxError(rc, "idgx_check_plan")
+        return p
+
+    def grid(self, plan, stream=None, subgrids_in=None):
+        """Run the gridder; the subgrids land in self.d_subgrids."""
+        grid(plan, self.d_metadata, self.d_channels, self.d_uvw, self.d_wavenumbers, self.d_visibilities,
+             self.d_spheroidal, self.d_aterms, self.d_subgrids, stream)
+
+    def degrid(self, plan, d_subgrids=None, stream=None):
+        """Run the degridder on d_subgrids (default: self.d_subgrids); visibilities land in self.d_vis_out."""
+        degrid(plan, self.d_metadata, self.d_channels, self.d_uvw, self.d_wavenumbers,
+               self.d_subgrids if d_subgrids is None else d_subgrids, self.d_spheroidal, self.d_aterms,
+               self.d_vis_out, stream)
+
+    def subgrids(self):
+        return download(self.d_subgrids, np.complex64, (max(self.n, 1), 4, self.S, self.S))[: self.n]
+
+    def visibilities_out(self):
+        return download(self.d_vis_out, np.complex64, (self.nr_rows, self.C, 4))
