@@ -1,0 +1,113 @@
+"""Host-side checks (CPU): the product's reference-workload generator against the golden vectors, the C ABI's
+exported symbols, and plan validation error codes."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import idgx
+from helpers import golden_names, load_golden
+from idgx import observation, workload
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_reference_workload_bit_exact(name):
+    w = load_golden(name)
+    sh = w["shape"]
+    g = workload.reference_workload(sh["nstations"], sh["ntimeslots"], sh["ntimesteps_per_subgrid"], sh["nchannels"],
+                                    sh["subgrid_size"], sh["grid_size"])
+    for key in ("uvw", "frequencies", "wavenumbers", "visibilities", "spheroidal", "aterms", "metadata", "baselines"):
+        a = np.ascontiguousarray(g[key])
+        b = np.ascontiguousarray(w[key]).reshape(a.shape) if key != "metadata" else w[key]
+        assert a.tobytes() == b.tobytes(), key
+    ramp = workload.reference_subgrids(w["nr_subgrids"], sh["subgrid_size"])
+    assert ramp.tobytes() == w["ref_degrid_subgrids"].tobytes()
+
+
+def test_c_abi_exports_every_declared_symbol():
+    names = idgx.exported_functions()
+    assert {"idgx_grid", "idgx_degrid", "idgx_check_plan", "idgx_strerror"} <= set(names)
+    lib = idgx.lib()
+    for n in names:
+        assert hasattr(lib, n), n
+    assert idgx.lib().idgx_version().startswith(b"idgx")
+
+
+def test_metadata_layout_is_36_bytes():
+    assert idgx.METADATA_DTYPE.itemsize == 36
+    assert ctypes.sizeof(idgx.Plan) == 56
+
+
+def _plan(w, **kw):
+    args = dict(nr_subgrids=w["nr_subgrids"], subgrid_size=w["subgrid_size"], grid_size=w["grid_size"],
+                nr_channels=w["wavenumbers"].shape[0], nr_stations=w["nr_stations"],
+                nr_aterm_slots=w["nr_aterm_slots"], nr_rows=w["uvw"].shape[0],
+                max_nr_timesteps=w["max_nr_timesteps"], image_size=w["image_size"])
+    args.update(kw)
+    return idgx.make_plan(**args)
+
+
+def test_check_plan_accepts_valid_and_rejects_out_of_bounds_metadata():
+    w = observation.make_observation("small", visibilities=False)
+    md, ch = w["metadata"], w["channels"]
+    assert idgx.check_plan(_plan(w), md, ch) == idgx.OK
+    bad_cases = {
+        "station": ("station2", w["nr_stations"]),
+        "rows": ("time_offset", w["uvw"].shape[0]),
+        "negative rows": ("time_offset", -1),
+        "aterm slot": ("aterm_index", 1),
+        "timesteps": ("nr_timesteps", w["max_nr_timesteps"] + 1),
+    }
+    for what, (field, value) in bad_cases.items():
+        m = md.copy()
+        m[field][3] = value
+        assert idgx.check_plan(_plan(w), m, ch) == idgx.ERR_INVALID_METADATA, what
+    c = ch.copy()
+    c["channel_begin"][2] = 10
+    assert idgx.check_plan(_plan(w), md, c) == idgx.ERR_INVALID_METADATA
+
+
+def test_check_plan_rejects_bad_plans():
+    w = observation.make_observation("small", visibilities=False)
+    md, ch = w["metadata"], w["channels"]
+    assert idgx.check_plan(_plan(w, subgrid_size=7), md, ch) == idgx.ERR_UNSUPPORTED
+    assert idgx.check_plan(_plan(w, subgrid_size=256), md, ch) == idgx.ERR_UNSUPPORTED
+    assert idgx.check_plan(_plan(w, nr_channels=0), md, ch) == idgx.ERR_INVALID_ARGUMENT
+    assert idgx.check_plan(_plan(w, image_size=0.0), md, ch) == idgx.ERR_INVALID_ARGUMENT
+    assert idgx.check_plan(_plan(w, precision=9), md, ch) == idgx.ERR_INVALID_ARGUMENT
+    assert idgx.check_plan(_plan(w, aterm_convention=5), md, ch) == idgx.ERR_INVALID_ARGUMENT
+    assert idgx.check_plan(_plan(w, flags=8), md, ch) == idgx.ERR_INVALID_ARGUMENT
+
+
+def test_grid_entry_points_reject_null_arguments_without_touching_the_device():
+    w = observation.make_observation("small", visibilities=False)
+    p = _plan(w)
+    assert idgx.lib().idgx_grid(ctypes.byref(p), *([None] * 9)) == idgx.ERR_INVALID_ARGUMENT
+    assert idgx.lib().idgx_degrid(ctypes.byref(p), *([None] * 9)) == idgx.ERR_INVALID_ARGUMENT
+    assert idgx.lib().idgx_grid(None, *([None] * 9)) == idgx.ERR_INVALID_ARGUMENT
+    assert idgx.strerror(idgx.ERR_INVALID_METADATA).startswith("subgrid metadata")
+
+
+def test_planner_covers_observation_and_keeps_subgrids_on_grid():
+    w = observation.make_observation("ska_low", visibilities=False)
+    md, ch = w["metadata"], w["channels"]
+    S, G = w["subgrid_size"], w["grid_size"]
+    assert (md["x"] >= 0).all() and (md["x"] + S <= G).all() and (md["y"] >= 0).all() and (md["y"] + S <= G).all()
+    # Every (row, channel) sample is covered at most once, and almost all are covered.
+    C = w["wavenumbers"].shape[0]
+    cover = np.zeros((w["nr_rows"], C), np.int8)
+    for s in range(0, md.size, 37):
+        r0 = md["time_offset"][s]
+        cover[r0:r0 + md["nr_timesteps"][s], ch["channel_begin"][s]:ch["channel_begin"][s] + ch["nr_channels"][s]] += 1
+    assert cover.max() <= 1
+    assert w["nr_visibilities"] > 0.99 * w["nr_rows"] * C
+    # Samples of each subgrid lie within the usable footprint (S - support cells) around its centre.
+    freqs = w["frequencies"].astype(np.float64)
+    rows = md["time_offset"][:, None] + np.arange(128)[None, :]
+    for s in range(0, md.size, 997):
+        uvw = w["uvw"][rows[s, :md["nr_timesteps"][s]]].astype(np.float64)
+        f = freqs[ch["channel_begin"][s]:ch["channel_begin"][s] + ch["nr_channels"][s]]
+        gu = uvw[:, 0:1] * f[None, :] / observation.SPEED_OF_LIGHT * float(w["image_size"]) + G / 2
+        gv = uvw[:, 1:2] * f[None, :] / observation.SPEED_OF_LIGHT * float(w["image_size"]) + G / 2
+        assert gu.min() >= md["x"][s] + 3.5 and gu.max() <= md["x"][s] + S - 3.5
+        assert gv.min() >= md["y"][s] + 3.5 and gv.max() <= md["y"][s] + S - 3.5
