@@ -65,15 +65,76 @@ def cpu_baseline(w, every, nthreads):
     return dict(grid_s=t1 - t0, degrid_s=t2 - t1, nvis=nvis, nsub=idx.size)
 
 
+def time_precision(dw, precision, steps, warmup, stream):
+    """Average gridder and degridder kernel time (s) over `steps` round trips, HIP events on `stream`."""
+    import torch
+
+    plan = dw.plan(precision=precision)
+    for _ in range(warmup):
+        dw.grid(plan, stream=stream)
+        dw.degrid(plan, stream=stream)
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ev[k][0].record(stream)
+        dw.grid(plan, stream=stream)
+        ev[k][1].record(stream)
+        dw.degrid(plan, stream=stream)
+        ev[k][2].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_grid = sum(e[0].elapsed_time(e[1]) for e in ev) / steps / 1e3
+    t_degrid = sum(e[1].elapsed_time(e[2]) for e in ev) / steps / 1e3
+    return t_grid, t_degrid, wall / steps
+
+
+def accuracy_vs_fp64(dw, w, host_vis, idx):
+    """Relative L2 error vs the FP64 oracle of the gridder output and the degridder output (degridding the same
+    subgrids) on the sample of subgrids idx; returns (whole-sample, worst subgrid) per direction."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from idgx import observation
+    from helpers import oracle_degrid, oracle_grid, per_subgrid_rel_l2, rel_l2
+
+    import torch
+
+    sw = observation.subset(dict(w, visibilities=host_vis), idx)
+    S, C = int(w["subgrid_size"]), int(w["wavenumbers"].shape[0])
+    dev = dw.d_subgrids.device
+    subs = dw.d_subgrids.view(-1, 4 * S * S * 8)[torch.from_numpy(idx).to(dev)].cpu().numpy()
+    subs = subs.view(np.complex64).reshape(idx.size, 4, S, S)
+    rows = torch.from_numpy(sw["source_rows"]).to(dev)
+    vis = dw.d_vis_out.view(-1, C * 4 * 8)[rows].cpu().numpy().view(np.complex64).reshape(-1, C, 4)
+    g64 = oracle_grid(sw, fp64=True)
+    v64 = oracle_degrid(sw, subs, fp64=True)
+    md, ch = sw["metadata"], sw["channels"]
+    worst_v = 0.0
+    for s in range(md.size):
+        r0, nt = int(md["time_offset"][s]), int(md["nr_timesteps"][s])
+        c0, nc = int(ch["channel_begin"][s]), int(ch["nr_channels"][s])
+        worst_v = max(worst_v, rel_l2(vis[r0:r0 + nt, c0:c0 + nc], v64[r0:r0 + nt, c0:c0 + nc]))
+    mask = np.zeros(vis.shape[:2], bool)
+    for s in range(md.size):
+        r0, nt = int(md["time_offset"][s]), int(md["nr_timesteps"][s])
+        mask[r0:r0 + nt, int(ch["channel_begin"][s]):int(ch["channel_begin"][s]) + int(ch["nr_channels"][s])] = True
+    return {
+        "gridder": rel_l2(subs, g64), "gridder_worst_subgrid": float(per_subgrid_rel_l2(subs, g64).max()),
+        "degridder": rel_l2(vis[mask], v64[mask]), "degridder_worst_subgrid": worst_v,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="ska_low")
-    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--precision", default="bf16x3", help="headline form (the plain fp32 form is timed beside it)")
     ap.add_argument("--cpu-every", type=int, default=0, help="CPU baseline on every k-th subgrid (0 = auto)")
+    ap.add_argument("--check-every", type=int, default=0, help="FP64 accuracy sample: every k-th subgrid (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the hot path is per-GPU; see the sharding design); "
@@ -82,7 +143,6 @@ def main():
 
     import torch
 
-    import idgx
     from idgx import observation
     from idgx.device import DeviceWorkload
 
@@ -96,45 +156,42 @@ def main():
     # Synthetic complex64 visibilities, real/imag ~ N(0,1), generated in place in HBM.
     d_vis = torch.randn(n_rows * C * 8, generator=gen, device=dev, dtype=torch.float32)
     dw = DeviceWorkload(w, device=dev, visibilities_on_device=d_vis)
-    plan = dw.plan(precision=args.precision)
     stream = torch.cuda.current_stream()
-    print(f"bench: {args.config}: {w['nr_subgrids']} subgrids, {w['nr_visibilities']} visibilities, "
-          f"setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
-
-    for _ in range(args.warmup):
-        dw.grid(plan, stream=stream)
-        dw.degrid(plan, stream=stream)
-    torch.cuda.synchronize()
-
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ev[k][0].record(stream)
-        dw.grid(plan, stream=stream)
-        ev[k][1].record(stream)
-        dw.degrid(plan, stream=stream)
-        ev[k][2].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    t_grid = sum(e[0].elapsed_time(e[1]) for e in ev) / args.steps / 1e3
-    t_degrid = sum(e[1].elapsed_time(e[2]) for e in ev) / args.steps / 1e3
-
     md, ch, S = w["metadata"], w["channels"], int(w["subgrid_size"])
     flops, sincos, nvis, _ = flop_model(md, ch, S, C)
-    step_s = wall / args.steps
-    value = nvis / (t_grid + t_degrid) / 1e6
-    dominant, t_dom = ("degridder", t_degrid) if t_degrid > t_grid else ("gridder", t_grid)
+    print(f"bench: {args.config}: {w['nr_subgrids']} subgrids, {nvis} visibilities, "
+          f"setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+
+    need_host = not (args.no_cpu and args.no_check)
+    host_vis = d_vis.view(torch.uint8).cpu().numpy().view(np.complex64).reshape(n_rows, C, 4) if need_host else None
+    check_every = args.check_every or max(1, w["nr_subgrids"] // 160)
+    check_idx = np.arange(check_every // 2, w["nr_subgrids"], check_every)
+
+    forms = []
+    for prec in dict.fromkeys(["fp32", args.precision]):
+        tg, td, step = time_precision(dw, prec, args.steps, args.warmup, stream)
+        rec = {"precision": prec, "gridder_ms": tg * 1e3, "degridder_ms": td * 1e3, "step_ms": step * 1e3,
+               "round_trip_mvis_s": nvis / (tg + td) / 1e6, "gridder_mvis_s": nvis / tg / 1e6,
+               "degridder_mvis_s": nvis / td / 1e6,
+               "gridder_tflops": flops / tg / 1e12, "degridder_tflops": flops / td / 1e12}
+        if not args.no_check:
+            rec["rel_l2_vs_fp64"] = accuracy_vs_fp64(dw, w, host_vis, check_idx)
+        print(f"bench: {prec}: gridder {tg * 1e3:.2f} ms, degridder {td * 1e3:.2f} ms, "
+              f"{rec['round_trip_mvis_s']:.1f} Mvis/s", file=sys.stderr, flush=True)
+        forms.append(rec)
+    head = forms[-1]
+    plain = forms[0]
+    tg, td = head["gridder_ms"] / 1e3, head["degridder_ms"] / 1e3
+    dominant, t_dom = ("degridder", td) if td > tg else ("gridder", tg)
     achieved = flops / t_dom / 1e12
     bytes_dom = algorithmic_bytes(md, ch, S, C, int(w["nr_aterm_slots"]), int(w["nr_stations"]), dominant)
+    mfma_flops = 0 if head["precision"] == "fp32" else 128 * sincos * (1 if head["precision"].endswith("x3") else 0.5)
 
     cpu = None
     if not args.no_cpu:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         every = args.cpu_every or max(1, w["nr_subgrids"] // 256)
-        host_vis = d_vis.view(torch.uint8).cpu().numpy().view(np.complex64).reshape(n_rows, C, 4)
-        w["visibilities"] = host_vis
-        r = cpu_baseline(w, every, nthreads)
+        r = cpu_baseline(dict(w, visibilities=host_vis), every, nthreads)
         cpu = {
             "value": r["nvis"] / (r["grid_s"] + r["degrid_s"]) / 1e6,
             "unit": "Mvis/s",
@@ -143,35 +200,38 @@ def main():
             "sample": f"every {every}th subgrid of the plan ({r['nsub']} subgrids, {r['nvis']} visibilities); "
                       f"gridder {r['nvis'] / r['grid_s'] / 1e6:.3f} Mvis/s, degridder "
                       f"{r['nvis'] / r['degrid_s'] / 1e6:.3f} Mvis/s; oracle FP32 restatement of the reference "
-                      f"OpenMP CPU path, -O3 -ffp-contract=off",
+                      f"OpenMP CPU path (same loop nest and OpenMP structure), -O3 -ffp-contract=off",
         }
 
     out = {
         "metric": "Mvis/s gridder+degridder, 1 MI355X: FP32 vs MFMA mixed at rel-err<=5e-5",
-        "value": value,
+        "value": head["round_trip_mvis_s"],
         "unit": "Mvis/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": step_s * 1e3,
+        "ms_per_step": head["step_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.precision == "fp32" else args.precision,
+        "dtype": "f32" if head["precision"] == "fp32" else
+                 {"bf16x3": "bf16x3-mfma+f32acc", "fp16x3": "fp16x3-mfma+f32acc"}.get(head["precision"],
+                                                                                 head["precision"] + "-mfma"),
         "data": "synthetic (seeded layout/uvw/plan; N(0,1) complex64 visibilities generated in HBM)",
         "config": {
             "workload": f"{args.config}: {w['config']['n_stations']} stations, {w['config']['n_times']} x 1 s, "
-                        f"{C} channels, S={S}, G={w['grid_size']}, image {float(w['image_size'])} rad",
+                        f"{C} channels {w['config']['f_lo'] / 1e6:.0f}-{w['config']['f_hi'] / 1e6:.0f} MHz, "
+                        f"S={S}, {w['config']['timesteps_per_subgrid']} timesteps/subgrid, G={w['grid_size']}, "
+                        f"image {float(w['image_size']):.3g} rad",
             "subgrids": int(w["nr_subgrids"]),
             "visibilities": nvis,
-            "precision": args.precision,
+            "precision": head["precision"],
         },
-        "gridder_mvis_s": nvis / t_grid / 1e6,
-        "degridder_mvis_s": nvis / t_degrid / 1e6,
-        "gridder_ms": t_grid * 1e3,
-        "degridder_ms": t_degrid * 1e3,
+        "speedup_vs_plain_fp32": head["round_trip_mvis_s"] / plain["round_trip_mvis_s"],
+        "forms": forms,
         "gflops_per_pass": flops / 1e9,
         "gsincos_per_pass": sincos / 1e9,
+        "accuracy_sample": f"every {check_every}th subgrid ({check_idx.size} subgrids) vs the FP64 oracle",
         "roofline": {
             "bound": "mfma",
             "kernel": dominant,
@@ -180,6 +240,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": None,
+            "flop_model": "P*(5*T + 34*V) + 134*P per subgrid (FP32-equivalent algorithmic flops), sincos separate",
+            "mfma_utilisation": (mfma_flops / t_dom / 1e12) / 2500.0 if mfma_flops else None,
             "algorithmic_bytes": bytes_dom,
             "hbm_gbs_algorithmic": bytes_dom / t_dom / 1e9,
         },

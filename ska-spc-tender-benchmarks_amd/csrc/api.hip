@@ -112,7 +112,7 @@ extern "C" int idgx_grid(const idgx_plan *plan, const idgx_metadata *d_metadata,
     case IDGX_PLAIN_FP32:
         return to_status(idgx::launch_grid_plain(a, accurate_sincos(plan), st));
     default:
-        return IDGX_ERR_UNSUPPORTED;
+        return to_status(idgx::launch_grid_mixed(a, plan->precision, st));
     }
 }
 
@@ -143,7 +143,7 @@ extern "C" int idgx_degrid(const idgx_plan *plan, const idgx_metadata *d_metadat
     case IDGX_PLAIN_FP32:
         return to_status(idgx::launch_degrid_plain(a, accurate_sincos(plan), st));
     default:
-        return IDGX_ERR_UNSUPPORTED;
+        return to_status(idgx::launch_degrid_mixed(a, plan->precision, st));
     }
 }
 

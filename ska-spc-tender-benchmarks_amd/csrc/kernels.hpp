@@ -26,4 +26,6 @@ struct KernelArgs
 
 hipError_t launch_grid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream);
 hipError_t launch_degrid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream);
+hipError_t launch_grid_mixed(const KernelArgs &a, int precision, hipStream_t stream);
+hipError_t launch_degrid_mixed(const KernelArgs &a, int precision, hipStream_t stream);
 } // namespace idgx

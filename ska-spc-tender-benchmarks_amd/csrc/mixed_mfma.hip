@@ -1,0 +1,573 @@
+// Mixed-precision gridder and degridder for gfx950: FP64 phase-argument reduction, FP32 phasors, and the
+// phasor x data contraction on v_mfma_f32_16x16x32_{bf16,f16} with FP32 accumulation.
+//
+// Phase reduction.  With l = (x' + 1/2) delta, x' = x - S/2 (an integer), delta = image_size / S, and
+//     du = k_c u_t / 2pi - u_off,  dv = ...,  dw = k_c w_t / 2pi - w_off      (wavelengths, formed in FP64,
+// u_off = (x0 + S/2 - G/2) / image_size the subgrid's phase centre), the phase in revolutions is
+//     du l + dv m + dw n = qu x' + qv y' + dw n + (qu + qv) / 2,     qu = du delta, qv = dv delta.
+// Because x', y' are integers, qu and qv can be reduced mod 1 in FP64 without changing the phasor, and the
+// constant (qu + qv)/2 mod 1 becomes one rotation per visibility (folded into the gridder's B operand, or
+// applied to the degridder's output).  The per-pair phase is then bounded by ~S/2 + |dw n| revolutions for any
+// input, so three FP32 operations evaluate it to ~1e-6 revolutions and v_sin_f32/v_cos_f32 (domain +-256
+// revolutions) take it directly.  This replaces the reference's FP32 phase_offset - k (u l + v m + w n)
+// (gridding/src/gridding_benchmark.cpp:112-121), whose two large terms cancel.
+//
+// Contraction as a real GEMM.  pixel[p][corr] = sum_v phasor[p][v] * vis[v][corr] (gridder) is
+//     D[p][n] = sum_k A[p][k] B[k][n],   A[p][2v] = cos, A[p][2v+1] = sin,
+//     B[2v][re] = a, B[2v+1][re] = -b, B[2v][im] = b, B[2v+1][im] = a   (vis = a + ib, 4 correlations x re/im)
+// which fills 8 of the 16 MFMA columns; the other 8 carry the low half of B, so one MFMA with A_hi gives
+// A_hi*B_hi + A_hi*B_lo and one with A_lo gives A_lo*B_hi (+ A_lo*B_lo): the three-product split in two
+// instructions.  Columns n and n+8 are summed at the end.  The degridder is the same with pixels and
+// visibilities exchanging roles (vis[v][corr] = sum_p phasor[v][p] * pixel[p][corr]).
+//
+// Operand layout of v_mfma_f32_16x16x32 (one wave): lane l holds A[l%16][8*(l/16) .. +7], B[8*(l/16) .. +7][l%16]
+// and D[4*(l/16) .. +3][l%16].  So a lane evaluates phasors of one row (pixel, or visibility) against four
+// k-pairs, packs (cos, sin) pairs straight into A registers, and reads its 16-byte B fragment from LDS.
+#include "idgx_device.hpp"
+#include "kernels.hpp"
+
+namespace idgx
+{
+namespace
+{
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+enum DType
+{
+    kBF16 = 0,
+    kFP16 = 1
+};
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kVisBatch = 256;  // gridder: visibilities staged per LDS batch
+constexpr int kPixBatch = 256;  // degridder: pixels staged per LDS batch
+
+// Round a pair of floats to two 16-bit values (RNE), x0 in the low half.
+template <int DT> __device__ __forceinline__ uint32_t pack2(float x0, float x1)
+{
+    if constexpr (DT == kBF16)
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){x0, x1}, bf16x2));
+    else
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){x0, x1}, f16x2));
+}
+
+template <int DT> __device__ __forceinline__ void unpack2(uint32_t p, float &y0, float &y1)
+{
+    if constexpr (DT == kBF16)
+    {
+        y0 = __builtin_bit_cast(float, p << 16);
+        y1 = __builtin_bit_cast(float, p & 0xffff0000u);
+    }
+    else
+    {
+        const f16x2 h = __builtin_bit_cast(f16x2, p);
+        y0 = static_cast<float>(h[0]);
+        y1 = static_cast<float>(h[1]);
+    }
+}
+
+// hi = round(x), lo = round(x - hi), both packed as pairs.
+template <int DT> __device__ __forceinline__ void split2(float x0, float x1, uint32_t &hi, uint32_t &lo)
+{
+    hi = pack2<DT>(x0, x1);
+    float h0, h1;
+    unpack2<DT>(hi, h0, h1);
+    lo = pack2<DT>(x0 - h0, x1 - h1);
+}
+
+__device__ __forceinline__ f32x4 mfma(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, u32x4 b, f32x4 c,
+                                      std::integral_constant<int, kBF16>)
+{
+    const u32x4 a = {a0, a1, a2, a3};
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+
+__device__ __forceinline__ f32x4 mfma(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, u32x4 b, f32x4 c,
+                                      std::integral_constant<int, kFP16>)
+{
+    const u32x4 a = {a0, a1, a2, a3};
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0,
+                                                  0, 0);
+}
+
+// Writes the B-operand words of one k-pair item (a visibility for the gridder, a pixel for the degridder) with
+// complex values v[0..3] into a 1 KiB LDS block laid out as [lane][4 words]: item q of the block occupies word
+// (q % 4) of lanes (q / 4) * 16 + n, n = column.  Columns 0-7 carry the high parts, 8-15 the low parts.
+template <int DT, bool SPLIT>
+__device__ __forceinline__ void write_b_item(uint32_t *blk, int q, const cfloat *v)
+{
+    const int kg = q >> 2, slot = q & 3;
+    uint32_t *base = blk + kg * 64 + slot;
+#pragma unroll
+    for (int pol = 0; pol < 4; ++pol)
+    {
+        uint32_t hi, lo;
+        split2<DT>(v[pol].re, v[pol].im, hi, lo);
+        if (!SPLIT)
+            lo = 0;
+        // hi = (re | im << 16).  Column re: (cos row re, sin row -im); column im: (cos row im, sin row re).
+        const uint32_t re_col_hi = (hi & 0xffffu) | ((hi ^ 0x80000000u) & 0xffff0000u);
+        const uint32_t im_col_hi = (hi >> 16) | (hi << 16);
+        const uint32_t re_col_lo = (lo & 0xffffu) | ((lo ^ 0x80000000u) & 0xffff0000u);
+        const uint32_t im_col_lo = (lo >> 16) | (lo << 16);
+        base[(2 * pol) * 4] = re_col_hi;
+        base[(2 * pol + 1) * 4] = im_col_hi;
+        base[(8 + 2 * pol) * 4] = SPLIT ? re_col_lo : 0u;
+        base[(8 + 2 * pol + 1) * 4] = SPLIT ? im_col_lo : 0u;
+    }
+}
+
+__device__ __forceinline__ void channel_window(const KernelArgs &a, int s, int &c0, int &nc)
+{
+    if (a.channels)
+    {
+        const idgx_channels w = a.channels[s];
+        c0 = w.channel_begin;
+        nc = w.nr_channels;
+    }
+    else
+    {
+        c0 = 0;
+        nc = a.p.nr_channels;
+    }
+}
+
+struct CentreOffsets
+{
+    double u, v, w; // wavelengths
+};
+
+__device__ __forceinline__ CentreOffsets centre_offsets(const PlanDev &p, const idgx_metadata &m)
+{
+    const double inv_image = 1.0 / static_cast<double>(p.image_size);
+    CentreOffsets o;
+    o.u = (m.x + p.S * 0.5 - p.G * 0.5) * inv_image;
+    o.v = (m.y + p.S * 0.5 - p.G * 0.5) * inv_image;
+    o.w = static_cast<double>(p.w_step) * (m.z + 0.5);
+    return o;
+}
+
+constexpr double kInv2Pi = 0.15915494309189533577;
+
+// Reduced phase coordinates of one visibility (revolutions), signed by sgn: per-pixel-step slopes qu, qv in
+// [-1/2, 1/2], the w slope dw per unit n, and the constant rot = (qu + qv)/2 mod 1.
+struct Reduced
+{
+    float qu, qv, dw, rot;
+};
+
+__device__ __forceinline__ Reduced reduce_visibility(const float *uvw, float k, const CentreOffsets &o, double delta,
+                                                     float sgn)
+{
+    const double kk = static_cast<double>(k) * kInv2Pi;
+    const double qu = __builtin_fma(kk, static_cast<double>(uvw[0]), -o.u) * delta;
+    const double qv = __builtin_fma(kk, static_cast<double>(uvw[1]), -o.v) * delta;
+    const double dw = __builtin_fma(kk, static_cast<double>(uvw[2]), -o.w);
+    double rot = 0.5 * (qu + qv);
+    rot -= __builtin_rint(rot);
+    Reduced r;
+    r.qu = sgn * static_cast<float>(qu - __builtin_rint(qu));
+    r.qv = sgn * static_cast<float>(qv - __builtin_rint(qv));
+    r.dw = sgn * static_cast<float>(dw);
+    r.rot = sgn * static_cast<float>(rot);
+    return r;
+}
+
+// c * e^{2 pi i rot} for the four correlations.
+__device__ __forceinline__ void rotate4(cfloat *v, float rot)
+{
+    const float sn = __builtin_amdgcn_sinf(rot), cs = __builtin_amdgcn_cosf(rot);
+#pragma unroll
+    for (int pol = 0; pol < 4; ++pol)
+        v[pol] = {v[pol].re * cs - v[pol].im * sn, v[pol].re * sn + v[pol].im * cs};
+}
+
+__device__ __forceinline__ float pixel_lm_f(int x, int S, float image_size)
+{
+    return (static_cast<float>(x) + 0.5F - 0.5F * static_cast<float>(S)) * (image_size / static_cast<float>(S));
+}
+
+// x' = x - S/2, exact in float.
+__device__ __forceinline__ float pixel_offset(int x, int S)
+{
+    return static_cast<float>(x - S / 2);
+}
+
+__device__ __forceinline__ float pixel_n_f(float l, float m)
+{
+    const float r2 = l * l + m * m;
+    return r2 > 1.F ? 1.F : r2 / (1.F + __builtin_sqrtf(1.F - r2));
+}
+
+// Phasors of one row against four k-pairs, packed into A registers (hi, and lo when split).
+template <int DT, bool SPLIT>
+__device__ __forceinline__ void phasors4(f32x4 du, f32x4 dv, f32x4 dw, float l, float m, float n, uint32_t *hi,
+                                         uint32_t *lo)
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+    {
+        const float ph = __builtin_fmaf(du[j], l, __builtin_fmaf(dv[j], m, dw[j] * n));
+        const float s = __builtin_amdgcn_sinf(ph);
+        const float c = __builtin_amdgcn_cosf(ph);
+        if constexpr (SPLIT)
+            split2<DT>(c, s, hi[j], lo[j]);
+        else
+            hi[j] = pack2<DT>(c, s);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Gridder: one workgroup = 4 waves x PT tiles x 16 pixels of one subgrid.
+// ---------------------------------------------------------------------------------------------------------------
+template <int DT, bool SPLIT, int PT> __global__ __launch_bounds__(kThreads) void gridder_mfma(KernelArgs a)
+{
+    constexpr int kBlocks = kVisBatch / 16;
+    constexpr int kPix = kWaves * PT * 16;
+    __shared__ u32x4 s_b[kBlocks * 64];
+    __shared__ f32x4 s_red[kBlocks * 4 * 3]; // [block][k-group][du, dv, dw] over the group's 4 visibilities
+    __shared__ float s_pix[kPix * 8];
+
+    const int s = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int S = a.p.S, P = S * S;
+    const idgx_metadata m = a.metadata[s];
+    int c0, nc;
+    channel_window(a, s, c0, nc);
+    const long long row0 = first_row(a.metadata, m);
+    const bool ok = subgrid_in_bounds(a.p, m, row0, c0, nc);
+    const int pbase = blockIdx.y * kPix;
+    const long long C = a.p.nr_channels;
+
+    float xp[PT], yp[PT], n[PT];
+#pragma unroll
+    for (int t = 0; t < PT; ++t)
+    {
+        const int p = min(pbase + (wave * PT + t) * 16 + (lane & 15), P - 1);
+        xp[t] = pixel_offset(p % S, S);
+        yp[t] = pixel_offset(p / S, S);
+        n[t] = pixel_n_f(pixel_lm_f(p % S, S, a.p.image_size), pixel_lm_f(p / S, S, a.p.image_size));
+    }
+    f32x4 acc[PT];
+#pragma unroll
+    for (int t = 0; t < PT; ++t)
+        acc[t] = (f32x4){0.F, 0.F, 0.F, 0.F};
+
+    const int V = ok ? m.nr_timesteps * nc : 0;
+    const CentreOffsets off = centre_offsets(a.p, m);
+    const double delta = static_cast<double>(a.p.image_size) / S;
+    const float4 *vis = reinterpret_cast<const float4 *>(a.visibilities);
+    uint32_t *s_bw = reinterpret_cast<uint32_t *>(s_b);
+    float *s_rw = reinterpret_cast<float *>(s_red);
+
+    for (int v0 = 0; v0 < V; v0 += kVisBatch)
+    {
+        __syncthreads();
+        {
+            // Stage one visibility per thread: reduced phase coordinates and its B-operand words.
+            const int v = v0 + tid;
+            cfloat val[4] = {{0.F, 0.F}, {0.F, 0.F}, {0.F, 0.F}, {0.F, 0.F}};
+            float du = 0.F, dv = 0.F, dw = 0.F;
+            if (v < V)
+            {
+                const int t = v / nc, c = v - t * nc;
+                const long long r = row0 + t;
+                const Reduced rd = reduce_visibility(a.uvw + 3 * r, a.wavenumbers[c0 + c], off, delta, -1.F);
+                du = rd.qu;
+                dv = rd.qv;
+                dw = rd.dw;
+                const float4 x0 = vis[(r * C + c0 + c) * 2], x1 = vis[(r * C + c0 + c) * 2 + 1];
+                val[0] = {x0.x, x0.y};
+                val[1] = {x0.z, x0.w};
+                val[2] = {x1.x, x1.y};
+                val[3] = {x1.z, x1.w};
+                rotate4(val, rd.rot);
+            }
+            const int blk = tid >> 4, q = tid & 15;
+            write_b_item<DT, SPLIT>(s_bw + blk * 256, q, val);
+            float *r = s_rw + (blk * 4 + (q >> 2)) * 12 + (q & 3);
+            r[0] = du;
+            r[4] = dv;
+            r[8] = dw;
+        }
+        __syncthreads();
+        const int nblk = min(kBlocks, (V - v0 + 15) >> 4);
+        for (int b = 0; b < nblk; ++b)
+        {
+            const f32x4 *red = s_red + (b * 4 + (lane >> 4)) * 3;
+            const f32x4 du = red[0], dv = red[1], dw = red[2];
+            const u32x4 bf = s_b[b * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < PT; ++t)
+            {
+                uint32_t hi[4], lo[4];
+                phasors4<DT, SPLIT>(du, dv, dw, xp[t], yp[t], n[t], hi, lo);
+                acc[t] = mfma(hi[0], hi[1], hi[2], hi[3], bf, acc[t], std::integral_constant<int, DT>());
+                if constexpr (SPLIT)
+                    acc[t] = mfma(lo[0], lo[1], lo[2], lo[3], bf, acc[t], std::integral_constant<int, DT>());
+            }
+        }
+    }
+
+    // Sum the high and low column halves, gather each pixel's 8 values in LDS.
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < PT; ++t)
+    {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            const float v = acc[t][j] + __shfl_xor(acc[t][j], 8);
+            const int col = lane & 15;
+            if (col < 8)
+                s_pix[((wave * PT + t) * 16 + (lane >> 4) * 4 + j) * 8 + col] = v;
+        }
+    }
+    __syncthreads();
+    float2 *out = reinterpret_cast<float2 *>(a.subgrids_out) + static_cast<size_t>(s) * kCorr * P;
+    const float4 *aterms = reinterpret_cast<const float4 *>(a.aterms);
+    for (int q = tid; q < kPix; q += kThreads)
+    {
+        const int p = pbase + q;
+        if (p >= P)
+            break;
+        const float *v = s_pix + q * 8;
+        cfloat px[4] = {{v[0], v[1]}, {v[2], v[3]}, {v[4], v[5]}, {v[6], v[7]}};
+        if (ok)
+        {
+            const size_t base = static_cast<size_t>(m.aterm_index) * a.p.nr_stations;
+            cfloat a1[4], a2[4];
+            load_jones(aterms, (base + m.station1) * P + p, a1);
+            load_jones(aterms, (base + m.station2) * P + p, a2);
+            apply_a1h_p_a2(px, a1, a2);
+            const float taper = a.spheroidal[p];
+#pragma unroll
+            for (int pol = 0; pol < 4; ++pol)
+                out[pol * P + p] = make_float2(px[pol].re * taper, px[pol].im * taper);
+        }
+        else
+        {
+#pragma unroll
+            for (int pol = 0; pol < 4; ++pol)
+                out[pol * P + p] = make_float2(0.F, 0.F);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Degridder: one workgroup = one chunk of 4 waves x VT tiles x 16 visibilities of one subgrid.  The subgrid's
+// pixels stream through LDS in batches of 256 (taper and A-term applied, split into B fragments); the
+// accumulators stay in registers across batches, so each visibility is written once.
+// ---------------------------------------------------------------------------------------------------------------
+template <int DT, bool SPLIT, int VT> __global__ __launch_bounds__(kThreads) void degridder_mfma(KernelArgs a)
+{
+    constexpr int kBlocks = kPixBatch / 16;
+    constexpr int kChunk = kWaves * VT * 16;
+    __shared__ u32x4 s_b[kBlocks * 64];
+    __shared__ f32x4 s_lmn[kBlocks * 4 * 3]; // [block][k-group][x', y', n] over the group's 4 pixels
+    __shared__ float s_vis[kChunk * 8];
+
+    const int s = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int S = a.p.S, P = S * S;
+    const idgx_metadata m = a.metadata[s];
+    int c0, nc;
+    channel_window(a, s, c0, nc);
+    const long long row0 = first_row(a.metadata, m);
+    if (!subgrid_in_bounds(a.p, m, row0, c0, nc) || nc == 0)
+        return; // uniform: nothing of this subgrid is written
+    const int V = m.nr_timesteps * nc;
+    const int v0 = blockIdx.y * kChunk;
+    if (v0 >= V)
+        return; // uniform
+    const long long C = a.p.nr_channels;
+    const CentreOffsets off = centre_offsets(a.p, m);
+    const double delta = static_cast<double>(a.p.image_size) / S;
+    const float2 *sub = reinterpret_cast<const float2 *>(a.subgrids) + static_cast<size_t>(s) * kCorr * P;
+    const float4 *aterms = reinterpret_cast<const float4 *>(a.aterms);
+    const size_t abase = static_cast<size_t>(m.aterm_index) * a.p.nr_stations;
+    uint32_t *s_bw = reinterpret_cast<uint32_t *>(s_b);
+    float *s_lw = reinterpret_cast<float *>(s_lmn);
+
+    // Reduced phase coordinates of this lane's visibility in each tile (row lane % 16).
+    float du[VT], dv[VT], dw[VT];
+#pragma unroll
+    for (int t = 0; t < VT; ++t)
+    {
+        const int v = v0 + (wave * VT + t) * 16 + (lane & 15);
+        du[t] = dv[t] = dw[t] = 0.F;
+        if (v < V)
+        {
+            const int tt = v / nc, c = v - tt * nc;
+            const Reduced rd = reduce_visibility(a.uvw + 3 * (row0 + tt), a.wavenumbers[c0 + c], off, delta, 1.F);
+            du[t] = rd.qu;
+            dv[t] = rd.qv;
+            dw[t] = rd.dw;
+        }
+    }
+    f32x4 acc[VT];
+#pragma unroll
+    for (int t = 0; t < VT; ++t)
+        acc[t] = (f32x4){0.F, 0.F, 0.F, 0.F};
+
+    for (int pb = 0; pb < P; pb += kPixBatch)
+    {
+        const int np = min(kPixBatch, P - pb);
+        __syncthreads();
+        for (int q = tid; q < kPixBatch; q += kThreads)
+        {
+            cfloat px[4] = {{0.F, 0.F}, {0.F, 0.F}, {0.F, 0.F}, {0.F, 0.F}};
+            float l = 0.F, mm = 0.F, n = 0.F;
+            if (q < np)
+            {
+                const int p = pb + q;
+                const float taper = a.spheroidal[p];
+#pragma unroll
+                for (int pol = 0; pol < 4; ++pol)
+                {
+                    const float2 g = sub[pol * P + p];
+                    px[pol] = {taper * g.x, taper * g.y};
+                }
+                cfloat a1[4], a2[4];
+                load_jones(aterms, (abase + m.station1) * P + p, a1);
+                load_jones(aterms, (abase + m.station2) * P + p, a2);
+                if (a.p.convention == IDGX_ATERM_REFERENCE)
+                    apply_a1h_p_a2(px, a1, a2);
+                else
+                    apply_a1_p_a2h(px, a1, a2);
+                l = pixel_offset(p % S, S);
+                mm = pixel_offset(p / S, S);
+                n = pixel_n_f(pixel_lm_f(p % S, S, a.p.image_size), pixel_lm_f(p / S, S, a.p.image_size));
+            }
+            const int blk = q >> 4, qq = q & 15;
+            write_b_item<DT, SPLIT>(s_bw + blk * 256, qq, px);
+            float *r = s_lw + (blk * 4 + (qq >> 2)) * 12 + (qq & 3);
+            r[0] = l;
+            r[4] = mm;
+            r[8] = n;
+        }
+        __syncthreads();
+        const int nblk = (np + 15) >> 4;
+        for (int b = 0; b < nblk; ++b)
+        {
+            const f32x4 *g = s_lmn + (b * 4 + (lane >> 4)) * 3;
+            const f32x4 l4 = g[0], m4 = g[1], n4 = g[2];
+            const u32x4 bf = s_b[b * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < VT; ++t)
+            {
+                uint32_t hi[4], lo[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                {
+                    const float ph = __builtin_fmaf(du[t], l4[j], __builtin_fmaf(dv[t], m4[j], dw[t] * n4[j]));
+                    const float sn = __builtin_amdgcn_sinf(ph);
+                    const float cs = __builtin_amdgcn_cosf(ph);
+                    if constexpr (SPLIT)
+                        split2<DT>(cs, sn, hi[j], lo[j]);
+                    else
+                        hi[j] = pack2<DT>(cs, sn);
+                }
+                acc[t] = mfma(hi[0], hi[1], hi[2], hi[3], bf, acc[t], std::integral_constant<int, DT>());
+                if constexpr (SPLIT)
+                    acc[t] = mfma(lo[0], lo[1], lo[2], lo[3], bf, acc[t], std::integral_constant<int, DT>());
+            }
+        }
+    }
+
+    // Combine column halves, gather 8 values per visibility in LDS, write 32 B per visibility.
+#pragma unroll
+    for (int t = 0; t < VT; ++t)
+    {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            const float v = acc[t][j] + __shfl_xor(acc[t][j], 8);
+            const int col = lane & 15;
+            if (col < 8)
+                s_vis[((wave * VT + t) * 16 + (lane >> 4) * 4 + j) * 8 + col] = v;
+        }
+    }
+    __syncthreads();
+    float4 *vout = reinterpret_cast<float4 *>(a.visibilities_out);
+    for (int q = tid; q < kChunk; q += kThreads)
+    {
+        const int v = v0 + q;
+        if (v >= V)
+            break;
+        const int tt = v / nc, c = v - tt * nc;
+        const size_t o = ((row0 + tt) * C + c0 + c) * 2;
+        const float *x = s_vis + q * 8;
+        cfloat val[4] = {{x[0], x[1]}, {x[2], x[3]}, {x[4], x[5]}, {x[6], x[7]}};
+        rotate4(val, reduce_visibility(a.uvw + 3 * (row0 + tt), a.wavenumbers[c0 + c], off, delta, 1.F).rot);
+        vout[o] = make_float4(val[0].re, val[0].im, val[1].re, val[1].im);
+        vout[o + 1] = make_float4(val[2].re, val[2].im, val[3].re, val[3].im);
+    }
+}
+
+template <int DT, bool SPLIT> hipError_t grid_mfma(const KernelArgs &a, hipStream_t st)
+{
+    constexpr int PT = 4;
+    const int P = a.p.S * a.p.S;
+    const int per = kWaves * PT * 16;
+    gridder_mfma<DT, SPLIT, PT><<<dim3(a.p.nr_subgrids, (P + per - 1) / per), kThreads, 0, st>>>(a);
+    return hipGetLastError();
+}
+
+template <int DT, bool SPLIT> hipError_t degrid_mfma(const KernelArgs &a, hipStream_t st)
+{
+    constexpr int VT = 4;
+    constexpr int chunk = kWaves * VT * 16;
+    const long long vmax = static_cast<long long>(a.p.max_nr_timesteps) * a.p.nr_channels;
+    const long long blocks = (vmax + chunk - 1) / chunk;
+    if (blocks > 65535)
+        return hipErrorInvalidConfiguration;
+    degridder_mfma<DT, SPLIT, VT><<<dim3(a.p.nr_subgrids, blocks > 0 ? blocks : 1), kThreads, 0, st>>>(a);
+    return hipGetLastError();
+}
+} // namespace
+
+hipError_t launch_grid_mixed(const KernelArgs &a, int precision, hipStream_t st)
+{
+    switch (precision)
+    {
+    case IDGX_MIXED_BF16X3:
+        return grid_mfma<kBF16, true>(a, st);
+    case IDGX_MIXED_FP16X3:
+        return grid_mfma<kFP16, true>(a, st);
+    case IDGX_MIXED_BF16:
+        return grid_mfma<kBF16, false>(a, st);
+    case IDGX_MIXED_FP16:
+        return grid_mfma<kFP16, false>(a, st);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_degrid_mixed(const KernelArgs &a, int precision, hipStream_t st)
+{
+    switch (precision)
+    {
+    case IDGX_MIXED_BF16X3:
+        return degrid_mfma<kBF16, true>(a, st);
+    case IDGX_MIXED_FP16X3:
+        return degrid_mfma<kFP16, true>(a, st);
+    case IDGX_MIXED_BF16:
+        return degrid_mfma<kBF16, false>(a, st);
+    case IDGX_MIXED_FP16:
+        return degrid_mfma<kFP16, false>(a, st);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+} // namespace idgx
