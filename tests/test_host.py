@@ -111,3 +111,21 @@ def test_planner_covers_observation_and_keeps_subgrids_on_grid():
         gv = uvw[:, 1:2] * f[None, :] / observation.SPEED_OF_LIGHT * float(w["image_size"]) + G / 2
         assert gu.min() >= md["x"][s] + 3.5 and gu.max() <= md["x"][s] + S - 3.5
         assert gv.min() >= md["y"][s] + 3.5 and gv.max() <= md["y"][s] + S - 3.5
+
+
+REFERENCE_INC = "/root/reference/gridding/inc"
+
+
+@pytest.mark.skipif(not __import__("os").path.isdir(REFERENCE_INC), reason="reference sources not present here")
+def test_gpu_launcher_is_source_compatible_with_reference_headers(tmp_path):
+    """The drop-in TU compiles against the reference's own gridding_benchmark.h / types.hpp (build check only)."""
+    import os
+    import subprocess
+
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ska-spc-tender-benchmarks_amd")
+    root = os.path.dirname(pkg)
+    cmd = ["g++", "-std=gnu++17", "-c", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", f"-I{REFERENCE_INC}",
+           "-I/root/reference/benchmarks-common/inc", f"-I{root}/include",
+           os.path.join(pkg, "host", "src", "launcher_hip.cpp"), "-o", str(tmp_path / "l.o")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
