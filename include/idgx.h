@@ -120,6 +120,49 @@ int idgx_degrid(const idgx_plan *plan, const idgx_metadata *d_metadata, const id
                 const float *d_uvw, const float *d_wavenumbers, const void *d_subgrids, const float *d_spheroidal,
                 const void *d_aterms, void *d_visibilities, void *stream);
 
+/* ---- Stages after the gridder / before the degridder (no reference counterpart: the reference benchmark stops
+ * at image-domain subgrids, gridding/src/gridding_benchmark.cpp:155-163) ------------------------------------- */
+
+typedef enum
+{
+    IDGX_FFT_FORWARD = 0, /* image-domain subgrid -> uv subgrid (after the gridder) */
+    IDGX_FFT_BACKWARD = 1 /* uv subgrid -> image-domain subgrid (before the degridder); adjoint of FORWARD */
+} idgx_fft_direction;
+
+/* In-place batched 2-D transform of d_subgrids (complex64 [nr_subgrids][4][S][S]) that maps the gridder's pixel
+ * x, at l = (x + 1/2 - S/2) image_size / S, onto uv cell k at u = u_centre + (k - S/2) / image_size:
+ *   FORWARD  G[k] = sum_x I[x] exp(+2 pi i (k - S/2)(x - S/2 + 1/2) / S)   (in y and x)
+ *   BACKWARD I[x] = sum_k G[k] exp(-2 pi i (k - S/2)(x - S/2 + 1/2) / S)
+ * unnormalised, so BACKWARD(FORWARD(I)) = S^2 I.  S must be 2^a 3^b <= 64 (8, 12, 16, 24, 32, 48, 64). */
+int idgx_subgrid_fft(const idgx_plan *plan, void *d_subgrids, int direction, void *stream);
+
+/* Conflict-free order of the adder: tiles of tile_size x tile_size grid cells (tile_size = S), coloured by
+ * tile-coordinate parity; tiles [color_begin[c], color_begin[c+1]) have colour c, and tile t adds subgrids
+ * order[tile_start[t] .. tile_start[t+1]) in that order. */
+typedef struct
+{
+    int32_t tile_size;
+    int32_t nr_tiles;
+    int32_t color_begin[5];
+} idgx_adder_plan;
+
+/* Builds the adder order on the host from the plan's metadata.  h_order has nr_subgrids entries, h_tile_start
+ * nr_subgrids + 1 (only nr_tiles + 1 are used). */
+int idgx_adder_plan_build(const idgx_plan *plan, const idgx_metadata *h_metadata, idgx_adder_plan *adder_plan,
+                          int32_t *h_order, int32_t *h_tile_start);
+
+/* Adder: d_grid[pol][y0 + y][x0 + x] += d_subgrids[s][pol][y][x] for every subgrid (cells outside the grid are
+ * skipped).  d_grid is complex64 [4][G][G]; d_order / d_tile_start are the device copies of the arrays built by
+ * idgx_adder_plan_build.  The summation order per cell is fixed, so results are bit-reproducible. */
+int idgx_add_subgrids(const idgx_plan *plan, const idgx_adder_plan *adder_plan, const idgx_metadata *d_metadata,
+                      const int32_t *d_order, const int32_t *d_tile_start, const void *d_subgrids, void *d_grid,
+                      void *stream);
+
+/* Splitter: d_subgrids[s][pol][y][x] = d_grid[pol][y0 + y][x0 + x] (zero outside the grid); the adjoint of the
+ * adder. */
+int idgx_split_subgrids(const idgx_plan *plan, const idgx_metadata *d_metadata, const void *d_grid, void *d_subgrids,
+                        void *stream);
+
 /* Name of a status code. */
 const char *idgx_strerror(int status);
 

@@ -1,4 +1,8 @@
 // C ABI of libidgx: argument validation and dispatch to the gfx950 kernels.
+#include <algorithm>
+#include <utility>
+#include <vector>
+
 #include "idgx.h"
 #include "kernels.hpp"
 
@@ -145,6 +149,109 @@ extern "C" int idgx_degrid(const idgx_plan *plan, const idgx_metadata *d_metadat
     default:
         return to_status(idgx::launch_degrid_mixed(a, plan->precision, st));
     }
+}
+
+extern "C" int idgx_subgrid_fft(const idgx_plan *plan, void *d_subgrids, int direction, void *stream)
+{
+    const int rc = check_plan_struct(plan);
+    if (rc != IDGX_OK)
+        return rc;
+    if (direction != IDGX_FFT_FORWARD && direction != IDGX_FFT_BACKWARD)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    if (!idgx::subgrid_fft_supported(plan->subgrid_size))
+        return IDGX_ERR_UNSUPPORTED;
+    if (plan->nr_subgrids == 0)
+        return IDGX_OK;
+    if (!d_subgrids)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    return to_status(idgx::launch_subgrid_fft(d_subgrids, plan->nr_subgrids, plan->subgrid_size,
+                                              direction == IDGX_FFT_FORWARD, static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int idgx_adder_plan_build(const idgx_plan *plan, const idgx_metadata *md, idgx_adder_plan *ap,
+                                     int32_t *order, int32_t *tile_start)
+{
+    const int rc = check_plan_struct(plan);
+    if (rc != IDGX_OK)
+        return rc;
+    if (!ap || (plan->nr_subgrids > 0 && (!md || !order || !tile_start)))
+        return IDGX_ERR_INVALID_ARGUMENT;
+    const int n = plan->nr_subgrids;
+    const int T = plan->subgrid_size;
+    auto floordiv = [](long long a, long long b) { return a >= 0 ? a / b : -((-a + b - 1) / b); };
+    std::vector<std::pair<long long, int>> keys(n);
+    const long long ntiles_axis = floordiv(plan->grid_size, T) + 3; // tiles from -1 .. G/T + 1
+    for (int s = 0; s < n; ++s)
+    {
+        const long long tx = floordiv(md[s].x, T) + 1, ty = floordiv(md[s].y, T) + 1;
+        const long long color = ((tx - 1) & 1) + 2 * ((ty - 1) & 1);
+        const long long tx_c = std::min(std::max(tx, 0LL), ntiles_axis - 1);
+        const long long ty_c = std::min(std::max(ty, 0LL), ntiles_axis - 1);
+        keys[s] = {(color * ntiles_axis + ty_c) * ntiles_axis + tx_c, s};
+    }
+    std::stable_sort(keys.begin(), keys.end(),
+                     [](const std::pair<long long, int> &a, const std::pair<long long, int> &b) {
+                         return a.first < b.first;
+                     });
+    ap->tile_size = T;
+    int tiles = 0;
+    for (int c = 0; c <= 4; ++c)
+        ap->color_begin[c] = 0;
+    long long prev = -1;
+    for (int i = 0; i < n; ++i)
+    {
+        order[i] = keys[i].second;
+        if (keys[i].first != prev)
+        {
+            const int color = static_cast<int>(keys[i].first / (ntiles_axis * ntiles_axis));
+            tile_start[tiles] = i;
+            for (int c = color + 1; c <= 4; ++c)
+                ap->color_begin[c] = tiles + 1;
+            ++tiles;
+            prev = keys[i].first;
+        }
+    }
+    tile_start[tiles] = n;
+    ap->nr_tiles = tiles;
+    for (int c = 1; c <= 4; ++c)
+        ap->color_begin[c] = std::max(ap->color_begin[c], ap->color_begin[c - 1]);
+    ap->color_begin[4] = tiles;
+    return IDGX_OK;
+}
+
+extern "C" int idgx_add_subgrids(const idgx_plan *plan, const idgx_adder_plan *ap, const idgx_metadata *d_metadata,
+                                 const int32_t *d_order, const int32_t *d_tile_start, const void *d_subgrids,
+                                 void *d_grid, void *stream)
+{
+    const int rc = check_plan_struct(plan);
+    if (rc != IDGX_OK)
+        return rc;
+    if (!ap || ap->tile_size < plan->subgrid_size || ap->nr_tiles < 0 || ap->color_begin[0] != 0 ||
+        ap->color_begin[4] != ap->nr_tiles)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    for (int c = 0; c < 4; ++c)
+        if (ap->color_begin[c] > ap->color_begin[c + 1])
+            return IDGX_ERR_INVALID_ARGUMENT;
+    if (plan->nr_subgrids == 0 || ap->nr_tiles == 0)
+        return IDGX_OK;
+    if (!d_metadata || !d_order || !d_tile_start || !d_subgrids || !d_grid)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    return to_status(idgx::launch_adder(d_metadata, d_order, d_tile_start, ap->color_begin, plan->subgrid_size,
+                                        plan->grid_size, d_subgrids, d_grid, static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int idgx_split_subgrids(const idgx_plan *plan, const idgx_metadata *d_metadata, const void *d_grid,
+                                   void *d_subgrids, void *stream)
+{
+    const int rc = check_plan_struct(plan);
+    if (rc != IDGX_OK)
+        return rc;
+    if (plan->nr_subgrids == 0)
+        return IDGX_OK;
+    if (!d_metadata || !d_grid || !d_subgrids)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    return to_status(idgx::launch_splitter(d_metadata, plan->nr_subgrids, plan->subgrid_size, plan->grid_size, d_grid,
+                                           d_subgrids, static_cast<hipStream_t>(stream)));
 }
 
 extern "C" const char *idgx_strerror(int status)

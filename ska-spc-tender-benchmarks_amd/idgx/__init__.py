@@ -73,6 +73,16 @@ class Plan(ctypes.Structure):
     ]
 
 
+FFT_FORWARD = 0
+FFT_BACKWARD = 1
+
+
+class AdderPlan(ctypes.Structure):
+    """idgx_adder_plan (include/idgx.h)."""
+
+    _fields_ = [("tile_size", ctypes.c_int32), ("nr_tiles", ctypes.c_int32), ("color_begin", ctypes.c_int32 * 5)]
+
+
 class IdgxError(RuntimeError):
     def __init__(self, status, what):
         super().__init__(f"{what}: {strerror(status)} ({status})")
@@ -97,6 +107,10 @@ def lib():
         L.idgx_check_plan.argtypes = [ctypes.POINTER(Plan), vp, vp]
         L.idgx_grid.argtypes = [ctypes.POINTER(Plan)] + [vp] * 9
         L.idgx_degrid.argtypes = [ctypes.POINTER(Plan)] + [vp] * 9
+        L.idgx_subgrid_fft.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, vp]
+        L.idgx_adder_plan_build.argtypes = [ctypes.POINTER(Plan), vp, ctypes.POINTER(AdderPlan), vp, vp]
+        L.idgx_add_subgrids.argtypes = [ctypes.POINTER(Plan), ctypes.POINTER(AdderPlan)] + [vp] * 6
+        L.idgx_split_subgrids.argtypes = [ctypes.POINTER(Plan)] + [vp] * 4
         L.idgx_strerror.restype = ctypes.c_char_p
         L.idgx_strerror.argtypes = [ctypes.c_int]
         L.idgx_version.restype = ctypes.c_char_p
@@ -173,3 +187,37 @@ def degrid(plan, d_metadata, d_channels, d_uvw, d_wavenumbers, d_subgrids, d_sph
                            _dptr(d_visibilities), _stream(stream))
     if rc != OK:
         raise IdgxError(rc, "idgx_degrid")
+
+
+def subgrid_fft(plan, d_subgrids, direction, stream=None):
+    """idgx_subgrid_fft: in-place forward (image -> uv) or backward transform of every subgrid."""
+    rc = lib().idgx_subgrid_fft(ctypes.byref(plan), _dptr(d_subgrids), int(direction), _stream(stream))
+    if rc != OK:
+        raise IdgxError(rc, "idgx_subgrid_fft")
+
+
+def adder_plan_build(plan, metadata):
+    """Host-side conflict-free adder order: returns (AdderPlan, order int32[n], tile_start int32[n + 1])."""
+    n = int(plan.nr_subgrids)
+    ap = AdderPlan()
+    order = np.zeros(max(n, 1), np.int32)
+    tile_start = np.zeros(n + 1, np.int32)
+    rc = lib().idgx_adder_plan_build(ctypes.byref(plan), _hptr(np.ascontiguousarray(metadata)), ctypes.byref(ap),
+                                     _hptr(order), _hptr(tile_start))
+    if rc != OK:
+        raise IdgxError(rc, "idgx_adder_plan_build")
+    return ap, order[:n], tile_start[:ap.nr_tiles + 1]
+
+
+def add_subgrids(plan, adder_plan, d_metadata, d_order, d_tile_start, d_subgrids, d_grid, stream=None):
+    rc = lib().idgx_add_subgrids(ctypes.byref(plan), ctypes.byref(adder_plan), _dptr(d_metadata), _dptr(d_order),
+                                 _dptr(d_tile_start), _dptr(d_subgrids), _dptr(d_grid), _stream(stream))
+    if rc != OK:
+        raise IdgxError(rc, "idgx_add_subgrids")
+
+
+def split_subgrids(plan, d_metadata, d_grid, d_subgrids, stream=None):
+    rc = lib().idgx_split_subgrids(ctypes.byref(plan), _dptr(d_metadata), _dptr(d_grid), _dptr(d_subgrids),
+                                   _stream(stream))
+    if rc != OK:
+        raise IdgxError(rc, "idgx_split_subgrids")

@@ -5,7 +5,7 @@ torch provides the HBM allocations and the stream (plumbing only); every result 
 import numpy as np
 
 from . import METADATA_DTYPE, CHANNELS_DTYPE, ATERM_ADJOINT, PLAIN_FP32, make_plan, check_plan, grid, degrid, \
-    IdgxError, OK
+    IdgxError, OK, FFT_FORWARD, FFT_BACKWARD, subgrid_fft, adder_plan_build, add_subgrids, split_subgrids
 
 
 def _torch():
@@ -76,3 +76,29 @@ class DeviceWorkload:
 
     def visibilities_out(self):
         return download(self.d_vis_out, np.complex64, (self.nr_rows, self.C, 4))
+
+    # ---- stages around the gridder / degridder -------------------------------------------------------------
+    def fft(self, plan, direction=FFT_FORWARD, d_subgrids=None, stream=None):
+        subgrid_fft(plan, self.d_subgrids if d_subgrids is None else d_subgrids, direction, stream)
+
+    def grid_buffer(self):
+        """A zeroed complex64 [4][G][G] master grid in HBM (flat bytes)."""
+        torch = _torch()
+        G = int(self.w["grid_size"])
+        return torch.zeros(4 * G * G * 8, dtype=torch.uint8, device=self.device)
+
+    def add(self, plan, d_grid, d_subgrids=None, stream=None):
+        """Adds the (uv) subgrids onto d_grid in the conflict-free tile order."""
+        if not hasattr(self, "_adder"):
+            ap, order, tile_start = adder_plan_build(plan, self.w["metadata"])
+            self._adder = (ap, upload(order, self.device), upload(tile_start, self.device))
+        ap, d_order, d_tile_start = self._adder
+        add_subgrids(plan, ap, self.d_metadata, d_order, d_tile_start,
+                     self.d_subgrids if d_subgrids is None else d_subgrids, d_grid, stream)
+
+    def split(self, plan, d_grid, d_subgrids=None, stream=None):
+        split_subgrids(plan, self.d_metadata, d_grid, self.d_subgrids if d_subgrids is None else d_subgrids, stream)
+
+    def grid_array(self, d_grid):
+        G = int(self.w["grid_size"])
+        return download(d_grid, np.complex64, (4, G, G))

@@ -129,3 +129,27 @@ def test_gpu_launcher_is_source_compatible_with_reference_headers(tmp_path):
            os.path.join(pkg, "host", "src", "launcher_hip.cpp"), "-o", str(tmp_path / "l.o")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_adder_plan_is_conflict_free_and_complete():
+    """Every subgrid is listed once; within a colour, subgrids of different tiles never touch the same cell."""
+    w = observation.make_observation("ska_low", visibilities=False)
+    p = _plan(w)
+    ap, order, tile_start = idgx.adder_plan_build(p, w["metadata"])
+    md, S = w["metadata"], w["subgrid_size"]
+    assert sorted(order.tolist()) == list(range(md.size))
+    assert tile_start[0] == 0 and tile_start[-1] == md.size and np.all(np.diff(tile_start) > 0)
+    cb = list(ap.color_begin)
+    assert cb[0] == 0 and cb[4] == ap.nr_tiles and all(a <= b for a, b in zip(cb, cb[1:]))
+    for c in range(4):
+        # footprint bounding box of each tile of colour c; boxes of distinct tiles must not intersect
+        boxes = []
+        for t in range(cb[c], cb[c + 1]):
+            s = order[tile_start[t]:tile_start[t + 1]]
+            boxes.append((md["x"][s].min(), md["x"][s].max() + S, md["y"][s].min(), md["y"][s].max() + S))
+        boxes = np.array(boxes)
+        for i in range(len(boxes)):
+            x0, x1, y0, y1 = boxes[i]
+            others = np.delete(boxes, i, axis=0)
+            overlap = (others[:, 0] < x1) & (x0 < others[:, 1]) & (others[:, 2] < y1) & (y0 < others[:, 3])
+            assert not overlap.any()
