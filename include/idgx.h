@@ -78,7 +78,10 @@ typedef enum
 {
     /* Plain FP32 path: evaluate sin/cos with the ROCm device library's sincosf (full-range reduction, ~1 ulp)
      * instead of the default FP32 Cody-Waite reduction to [-pi, pi] followed by v_sin_f32/v_cos_f32. */
-    IDGX_FLAG_ACCURATE_SINCOS = 1
+    IDGX_FLAG_ACCURATE_SINCOS = 1,
+    /* Mixed forms: evaluate the per-pair phase qu x' + qv y' + dw n as an FP32 MFMA (v_mfma_f32_16x16x4_f32)
+     * instead of three VALU fused multiply-adds.  Same accuracy; a tuning choice. */
+    IDGX_FLAG_PHASE_MFMA = 2
 } idgx_flags;
 
 typedef struct
@@ -162,6 +165,81 @@ int idgx_add_subgrids(const idgx_plan *plan, const idgx_adder_plan *adder_plan, 
  * adder. */
 int idgx_split_subgrids(const idgx_plan *plan, const idgx_metadata *d_metadata, const void *d_grid, void *d_subgrids,
                         void *stream);
+
+/* ---- Batched execution: one persistent launch per direction over a whole plan ---------------------------------
+ *
+ * A batch holds up to 4 subgrid classes (one subgrid size each: mixed sizes run in the same launch).  Its
+ * work queue of (class, subgrid, block) items is built on the host at creation, sorted heaviest first, and
+ * consumed by persistent workgroups through one device atomic counter.  Optional per-class visibility selections
+ * (flagging, ragged timesteps) restrict each subgrid to the listed entries: subgrid s uses
+ * vis_index[vis_offset[s] .. vis_offset[s+1]), each entry (t << 16) | c relative to the subgrid's first row and
+ * channel window.  Creation allocates the queue memory; the execution entry points allocate nothing, and
+ * idgx_batch_capture records them into a hipGraph (no memset nodes) for idgx_batch_launch.  Batches run the
+ * mixed-precision forms (precision != IDGX_PLAIN_FP32).  A batch is used from one host thread at a time. */
+
+typedef struct idgx_batch idgx_batch;
+
+enum
+{
+    IDGX_STAGE_GRID = 1,
+    IDGX_STAGE_DEGRID = 2
+};
+
+typedef struct
+{
+    idgx_plan plan;                  /* all classes share precision, channels, rows, stations, grid and image size */
+    const idgx_metadata *h_metadata; /* host copy of the class's metadata */
+    const idgx_channels *h_channels; /* host copy of the channel windows, or NULL */
+    const int64_t *h_vis_offset;     /* host copy of the selection offsets (nr_subgrids + 1), or NULL (dense) */
+    int64_t nr_vis_index;            /* length of the class's selection list */
+} idgx_batch_class;
+
+typedef struct
+{
+    const idgx_metadata *d_metadata;
+    const idgx_channels *d_channels;
+    const uint32_t *d_vis_index; /* NULL together with d_vis_offset = dense */
+    const int64_t *d_vis_offset;
+    const float *d_spheroidal; /* [S][S] of this class */
+    const void *d_aterms;      /* [slots][stations][S][S][4] of this class */
+    void *d_subgrids;          /* [n][4][S][S]: gridder output, degridder input */
+} idgx_batch_class_data;
+
+typedef struct
+{
+    const float *d_uvw;
+    const float *d_wavenumbers;
+    const void *d_visibilities; /* gridder input */
+    void *d_visibilities_out;   /* degridder output (may be the same buffer) */
+    idgx_batch_class_data cls[4];
+} idgx_batch_data;
+
+int idgx_batch_create(const idgx_batch_class *classes, int nr_classes, idgx_batch **batch);
+void idgx_batch_destroy(idgx_batch *batch);
+
+/* Tile shapes: shape i = (tiles of 16 pixels / visibilities per wave, LDS staging depth). */
+int idgx_batch_shape_count(void);
+int idgx_batch_shape_info(int shape, int *tiles_per_wave, int *lds_batch);
+int idgx_batch_set_shape(idgx_batch *batch, int gridder_shape, int degridder_shape);
+int idgx_batch_get_shape(const idgx_batch *batch, int *gridder_shape, int *degridder_shape);
+
+/* Times every shape `repeats` times per direction on the given data (overwriting its outputs), keeps the fastest
+ * and reports its average time (ms). */
+int idgx_batch_autotune(idgx_batch *batch, const idgx_batch_data *data, int repeats, void *stream,
+                        float *best_gridder_ms, float *best_degridder_ms);
+
+/* One persistent gridder / degridder launch over every subgrid of every class. */
+int idgx_batch_grid(idgx_batch *batch, const idgx_batch_data *data, void *stream);
+int idgx_batch_degrid(idgx_batch *batch, const idgx_batch_data *data, void *stream);
+
+/* Records the selected stages (IDGX_STAGE_*) for `data` into a hipGraph (stream must not be NULL);
+ * idgx_batch_launch replays it.  Changing the shape drops the graph. */
+int idgx_batch_capture(idgx_batch *batch, const idgx_batch_data *data, int stages, void *stream);
+int idgx_batch_launch(idgx_batch *batch, void *stream);
+
+/* Queue sizes and resident workgroups of the current shapes. */
+int idgx_batch_items(const idgx_batch *batch, int *gridder_items, int *degridder_items, int *gridder_workgroups,
+                     int *degridder_workgroups);
 
 /* Name of a status code. */
 const char *idgx_strerror(int status);

@@ -27,7 +27,7 @@ int check_plan_struct(const idgx_plan *p)
         return IDGX_ERR_INVALID_ARGUMENT;
     if (p->precision < IDGX_PLAIN_FP32 || p->precision > IDGX_MIXED_FP16)
         return IDGX_ERR_INVALID_ARGUMENT;
-    if (p->flags & ~IDGX_FLAG_ACCURATE_SINCOS)
+    if (p->flags & ~(IDGX_FLAG_ACCURATE_SINCOS | IDGX_FLAG_PHASE_MFMA))
         return IDGX_ERR_INVALID_ARGUMENT;
     return IDGX_OK;
 }
@@ -116,7 +116,7 @@ extern "C" int idgx_grid(const idgx_plan *plan, const idgx_metadata *d_metadata,
     case IDGX_PLAIN_FP32:
         return to_status(idgx::launch_grid_plain(a, accurate_sincos(plan), st));
     default:
-        return to_status(idgx::launch_grid_mixed(a, plan->precision, st));
+        return to_status(idgx::launch_grid_mixed(a, plan->precision, (plan->flags & IDGX_FLAG_PHASE_MFMA) != 0, st));
     }
 }
 
@@ -147,7 +147,7 @@ extern "C" int idgx_degrid(const idgx_plan *plan, const idgx_metadata *d_metadat
     case IDGX_PLAIN_FP32:
         return to_status(idgx::launch_degrid_plain(a, accurate_sincos(plan), st));
     default:
-        return to_status(idgx::launch_degrid_mixed(a, plan->precision, st));
+        return to_status(idgx::launch_degrid_mixed(a, plan->precision, (plan->flags & IDGX_FLAG_PHASE_MFMA) != 0, st));
     }
 }
 

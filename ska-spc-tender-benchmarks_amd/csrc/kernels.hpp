@@ -22,12 +22,46 @@ struct KernelArgs
     const void *aterms;
     const void *subgrids; // degridder input
     void *subgrids_out;   // gridder output
+    // Optional visibility selection (flagging, ragged timesteps): subgrid s grids / degrids the entries
+    // vis_index[vis_offset[s] .. vis_offset[s+1]), each (t << 16) | c relative to the subgrid's first row and
+    // channel window.  Null = every (t, c) of the subgrid's window.
+    const uint32_t *vis_index;
+    const long long *vis_offset;
+};
+
+// Persistent-kernel work: up to kMaxClasses subgrid classes (one subgrid size each) share one launch; the queue
+// holds (class << 28 | subgrid, block) items, heaviest first; workgroups take items with an atomic counter.
+constexpr int kMaxClasses = 4;
+struct QueueArgs
+{
+    KernelArgs cls[kMaxClasses];
+    const uint2 *items;
+    int nr_items;
+    int *counter; // zeroed by a reset kernel before every launch
+};
+
+// Tile shape of the mixed kernels: gridder = tiles of 16 pixels per wave (PT) and visibilities staged per LDS
+// batch (VB); degridder = tiles of 16 visibilities per wave (VT) and pixels staged per LDS batch (PB).
+struct MixedShape
+{
+    int tiles;
+    int batch;
 };
 
 hipError_t launch_grid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream);
 hipError_t launch_degrid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream);
-hipError_t launch_grid_mixed(const KernelArgs &a, int precision, hipStream_t stream);
-hipError_t launch_degrid_mixed(const KernelArgs &a, int precision, hipStream_t stream);
+hipError_t launch_grid_mixed(const KernelArgs &a, int precision, bool phase_mfma, hipStream_t stream);
+hipError_t launch_degrid_mixed(const KernelArgs &a, int precision, bool phase_mfma, hipStream_t stream);
+
+// Persistent work-queue forms (the queue items are built on the host, see batch.cpp).
+int mixed_shape_count();
+MixedShape mixed_shape(int i);
+int grid_pixels_per_item(int shape);      // pixels one gridder item covers
+int degrid_visibilities_per_item(int shape); // visibilities one degridder item covers
+hipError_t launch_grid_mixed_queue(const QueueArgs &q, int precision, int shape, int workgroups, hipStream_t st);
+hipError_t launch_degrid_mixed_queue(const QueueArgs &q, int precision, int shape, int workgroups, hipStream_t st);
+hipError_t launch_reset_counter(int *counter, hipStream_t st);
+int queue_workgroups(int precision, bool gridder, int shape); // resident workgroups for a persistent launch
 
 bool subgrid_fft_supported(int S);
 hipError_t launch_subgrid_fft(void *subgrids, long long nr_subgrids, int S, bool forward, hipStream_t stream);

@@ -23,6 +23,16 @@
 // Operand layout of v_mfma_f32_16x16x32 (one wave): lane l holds A[l%16][8*(l/16) .. +7], B[8*(l/16) .. +7][l%16]
 // and D[4*(l/16) .. +3][l%16].  So a lane evaluates phasors of one row (pixel, or visibility) against four
 // k-pairs, packs (cos, sin) pairs straight into A registers, and reads its 16-byte B fragment from LDS.
+//
+// The phases themselves come from the matrix core too: phi = qu x' + qv y' + dw n is a rank-3 product, one
+// v_mfma_f32_16x16x4_f32 per 16 x 16 tile (FP32 operands, FP32 result).  Its D layout, lane l holding
+// D[4*(l/16)+j][l%16], is exactly the A layout above when D's rows are the k-pairs and its columns the rows of
+// the contraction, so the four phases a lane needs arrive in its own registers.  Per pair the VALU then only
+// evaluates v_sin/v_cos and the hi/lo split.
+//
+// Workgroup numbering spreads the workgroups of one subgrid over the XCDs (block b = sub_block * n + s): subgrid
+// costs are uneven, and grouping a subgrid's workgroups on one XCD (for L2 reuse) measured 1.2-2x slower because
+// the round-robin dispatch then waits on the most loaded XCD.
 #include "idgx_device.hpp"
 #include "kernels.hpp"
 
@@ -46,8 +56,6 @@ enum DType
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
-constexpr int kVisBatch = 256;  // gridder: visibilities staged per LDS batch
-constexpr int kPixBatch = 256;  // degridder: pixels staged per LDS batch
 
 // Round a pair of floats to two 16-bit values (RNE), x0 in the low half.
 template <int DT> __device__ __forceinline__ uint32_t pack2(float x0, float x1)
@@ -81,6 +89,13 @@ template <int DT> __device__ __forceinline__ void split2(float x0, float x1, uin
     unpack2<DT>(hi, h0, h1);
     lo = pack2<DT>(x0 - h0, x1 - h1);
 }
+
+// Phases of a 16 x 16 tile: D[i][j] = sum_k A[i][k] B[k][j], K = 4 (fourth component zero).
+__device__ __forceinline__ f32x4 phase_mfma(float a, float b)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, (f32x4){0.F, 0.F, 0.F, 0.F}, 0, 0, 0);
+}
+
 
 __device__ __forceinline__ f32x4 mfma(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, u32x4 b, f32x4 c,
                                       std::integral_constant<int, kBF16>)
@@ -207,36 +222,68 @@ __device__ __forceinline__ float pixel_n_f(float l, float m)
     return r2 > 1.F ? 1.F : r2 / (1.F + __builtin_sqrtf(1.F - r2));
 }
 
-// Phasors of one row against four k-pairs, packed into A registers (hi, and lo when split).
-template <int DT, bool SPLIT>
-__device__ __forceinline__ void phasors4(f32x4 du, f32x4 dv, f32x4 dw, float l, float m, float n, uint32_t *hi,
-                                         uint32_t *lo)
+// The visibility v of subgrid s: (t, c) relative to its first row / channel window, and the subgrid's count.
+struct Selection
 {
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
+    const uint32_t *index; // null: dense (t, c) = (v / nc, v % nc)
+    int nc;
+    __device__ __forceinline__ void at(int v, int &t, int &c) const
     {
-        const float ph = __builtin_fmaf(du[j], l, __builtin_fmaf(dv[j], m, dw[j] * n));
-        const float s = __builtin_amdgcn_sinf(ph);
-        const float c = __builtin_amdgcn_cosf(ph);
-        if constexpr (SPLIT)
-            split2<DT>(c, s, hi[j], lo[j]);
+        if (index)
+        {
+            const uint32_t e = index[v];
+            t = static_cast<int>(e >> 16);
+            c = static_cast<int>(e & 0xffffu);
+        }
         else
-            hi[j] = pack2<DT>(c, s);
+        {
+            t = v / nc;
+            c = v - t * nc;
+        }
     }
+};
+
+__device__ __forceinline__ int selection_of(const KernelArgs &a, int s, int nr_timesteps, int nc, Selection &sel)
+{
+    sel.nc = nc;
+    if (a.vis_index)
+    {
+        const long long o0 = a.vis_offset[s], o1 = a.vis_offset[s + 1];
+        sel.index = a.vis_index + o0;
+        return static_cast<int>(o1 - o0);
+    }
+    sel.index = nullptr;
+    return nr_timesteps * nc;
 }
 
-// ---------------------------------------------------------------------------------------------------------------
-// Gridder: one workgroup = 4 waves x PT tiles x 16 pixels of one subgrid.
-// ---------------------------------------------------------------------------------------------------------------
-template <int DT, bool SPLIT, int PT> __global__ __launch_bounds__(kThreads) void gridder_mfma(KernelArgs a)
+// Selected entries must lie inside the subgrid's window (entries outside are treated as empty visibilities).
+__device__ __forceinline__ bool entry_ok(int t, int c, int nr_timesteps, int nc)
 {
-    constexpr int kBlocks = kVisBatch / 16;
-    constexpr int kPix = kWaves * PT * 16;
-    __shared__ u32x4 s_b[kBlocks * 64];
-    __shared__ f32x4 s_red[kBlocks * 4 * 3]; // [block][k-group][du, dv, dw] over the group's 4 visibilities
-    __shared__ float s_pix[kPix * 8];
+    return t >= 0 && t < nr_timesteps && c >= 0 && c < nc;
+}
 
-    const int s = blockIdx.x;
+template <int PT, int VB> struct GridderShared
+{
+    u32x4 b[(VB / 16) * 64];
+    float coef[(VB / 16) * 64];
+    float pix[kWaves * PT * 16 * 8];
+};
+
+template <int VT, int PB> struct DegridderShared
+{
+    u32x4 b[(PB / 16) * 64];
+    float coord[(PB / 16) * 64];
+    float vis[kWaves * VT * 16 * 8];
+};
+
+// ---------------------------------------------------------------------------------------------------------------
+// Gridder item: 4 waves x PT tiles x 16 pixels (block pblk) of subgrid s; visibilities staged VB at a time.
+// ---------------------------------------------------------------------------------------------------------------
+template <int DT, bool SPLIT, int PT, int VB, bool PM>
+__device__ __forceinline__ void gridder_item(const KernelArgs &a, int s, int pblk, GridderShared<PT, VB> &sm)
+{
+    constexpr int kBlocks = VB / 16;
+    constexpr int kPix = kWaves * PT * 16;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int S = a.p.S, P = S * S;
@@ -245,46 +292,50 @@ template <int DT, bool SPLIT, int PT> __global__ __launch_bounds__(kThreads) voi
     channel_window(a, s, c0, nc);
     const long long row0 = first_row(a.metadata, m);
     const bool ok = subgrid_in_bounds(a.p, m, row0, c0, nc);
-    const int pbase = blockIdx.y * kPix;
+    const int pbase = pblk * kPix;
     const long long C = a.p.nr_channels;
 
-    float xp[PT], yp[PT], n[PT];
+    // Pixel coordinates of row lane%16 of each tile: (x', y', n); for PM only component lane/16 of them (the B
+    // operand of the phase MFMA).
+    float coord[PT], xp[PT], yp[PT], nn[PT];
 #pragma unroll
     for (int t = 0; t < PT; ++t)
     {
         const int p = min(pbase + (wave * PT + t) * 16 + (lane & 15), P - 1);
+        const int kk = lane >> 4;
         xp[t] = pixel_offset(p % S, S);
         yp[t] = pixel_offset(p / S, S);
-        n[t] = pixel_n_f(pixel_lm_f(p % S, S, a.p.image_size), pixel_lm_f(p / S, S, a.p.image_size));
+        nn[t] = pixel_n_f(pixel_lm_f(p % S, S, a.p.image_size), pixel_lm_f(p / S, S, a.p.image_size));
+        coord[t] = kk == 0 ? xp[t] : kk == 1 ? yp[t] : kk == 2 ? nn[t] : 0.F;
     }
     f32x4 acc[PT];
 #pragma unroll
     for (int t = 0; t < PT; ++t)
         acc[t] = (f32x4){0.F, 0.F, 0.F, 0.F};
 
-    const int V = ok ? m.nr_timesteps * nc : 0;
+    Selection sel;
+    const int V = ok ? selection_of(a, s, m.nr_timesteps, nc, sel) : 0;
     const CentreOffsets off = centre_offsets(a.p, m);
     const double delta = static_cast<double>(a.p.image_size) / S;
     const float4 *vis = reinterpret_cast<const float4 *>(a.visibilities);
-    uint32_t *s_bw = reinterpret_cast<uint32_t *>(s_b);
-    float *s_rw = reinterpret_cast<float *>(s_red);
+    uint32_t *s_bw = reinterpret_cast<uint32_t *>(sm.b);
 
-    for (int v0 = 0; v0 < V; v0 += kVisBatch)
+    for (int v0 = 0; v0 < V; v0 += VB)
     {
         __syncthreads();
+        for (int q = tid; q < VB; q += kThreads)
         {
-            // Stage one visibility per thread: reduced phase coordinates and its B-operand words.
-            const int v = v0 + tid;
+            // Stage one visibility: reduced phase coefficients and its B-operand words.
+            const int v = v0 + q;
             cfloat val[4] = {{0.F, 0.F}, {0.F, 0.F}, {0.F, 0.F}, {0.F, 0.F}};
-            float du = 0.F, dv = 0.F, dw = 0.F;
+            Reduced rd = {0.F, 0.F, 0.F, 0.F};
+            int t = 0, c = 0;
             if (v < V)
+                sel.at(v, t, c);
+            if (v < V && entry_ok(t, c, m.nr_timesteps, nc))
             {
-                const int t = v / nc, c = v - t * nc;
                 const long long r = row0 + t;
-                const Reduced rd = reduce_visibility(a.uvw + 3 * r, a.wavenumbers[c0 + c], off, delta, -1.F);
-                du = rd.qu;
-                dv = rd.qv;
-                dw = rd.dw;
+                rd = reduce_visibility(a.uvw + 3 * r, a.wavenumbers[c0 + c], off, delta, -1.F);
                 const float4 x0 = vis[(r * C + c0 + c) * 2], x1 = vis[(r * C + c0 + c) * 2 + 1];
                 val[0] = {x0.x, x0.y};
                 val[1] = {x0.z, x0.w};
@@ -292,25 +343,61 @@ template <int DT, bool SPLIT, int PT> __global__ __launch_bounds__(kThreads) voi
                 val[3] = {x1.z, x1.w};
                 rotate4(val, rd.rot);
             }
-            const int blk = tid >> 4, q = tid & 15;
-            write_b_item<DT, SPLIT>(s_bw + blk * 256, q, val);
-            float *r = s_rw + (blk * 4 + (q >> 2)) * 12 + (q & 3);
-            r[0] = du;
-            r[4] = dv;
-            r[8] = dw;
+            const int blk = q >> 4, qq = q & 15;
+            write_b_item<DT, SPLIT>(s_bw + blk * 256, qq, val);
+            if constexpr (PM)
+            {
+                float *cf = sm.coef + blk * 64 + qq;
+                cf[0] = rd.qu;
+                cf[16] = rd.qv;
+                cf[32] = rd.dw;
+                cf[48] = 0.F;
+            }
+            else
+            {
+                float *cf = sm.coef + blk * 64 + (qq >> 2) * 16 + (qq & 3);
+                cf[0] = rd.qu;
+                cf[4] = rd.qv;
+                cf[8] = rd.dw;
+            }
         }
         __syncthreads();
         const int nblk = min(kBlocks, (V - v0 + 15) >> 4);
         for (int b = 0; b < nblk; ++b)
         {
-            const f32x4 *red = s_red + (b * 4 + (lane >> 4)) * 3;
-            const f32x4 du = red[0], dv = red[1], dw = red[2];
-            const u32x4 bf = s_b[b * 64 + lane];
+            const u32x4 bf = sm.b[b * 64 + lane];
+            f32x4 phs[PT];
+            if constexpr (PM)
+            {
+                const float coef = sm.coef[b * 64 + lane];
+#pragma unroll
+                for (int t = 0; t < PT; ++t)
+                    phs[t] = phase_mfma(coef, coord[t]); // phi(visibility 4*(lane/16)+j, pixel lane%16)
+            }
+            else
+            {
+                const f32x4 *cf = reinterpret_cast<const f32x4 *>(sm.coef + b * 64 + (lane >> 4) * 16);
+                const f32x4 du = cf[0], dv = cf[1], dw = cf[2];
+#pragma unroll
+                for (int t = 0; t < PT; ++t)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        phs[t][j] = __builtin_fmaf(du[j], xp[t], __builtin_fmaf(dv[j], yp[t], dw[j] * nn[t]));
+            }
 #pragma unroll
             for (int t = 0; t < PT; ++t)
             {
                 uint32_t hi[4], lo[4];
-                phasors4<DT, SPLIT>(du, dv, dw, xp[t], yp[t], n[t], hi, lo);
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                {
+                    const float sn = __builtin_amdgcn_sinf(phs[t][j]);
+                    const float cs = __builtin_amdgcn_cosf(phs[t][j]);
+                    if constexpr (SPLIT)
+                        split2<DT>(cs, sn, hi[j], lo[j]);
+                    else
+                        hi[j] = pack2<DT>(cs, sn);
+                }
                 acc[t] = mfma(hi[0], hi[1], hi[2], hi[3], bf, acc[t], std::integral_constant<int, DT>());
                 if constexpr (SPLIT)
                     acc[t] = mfma(lo[0], lo[1], lo[2], lo[3], bf, acc[t], std::integral_constant<int, DT>());
@@ -318,7 +405,7 @@ template <int DT, bool SPLIT, int PT> __global__ __launch_bounds__(kThreads) voi
         }
     }
 
-    // Sum the high and low column halves, gather each pixel's 8 values in LDS.
+    // Sum the high and low column halves, gather each pixel's 8 values in LDS, apply A-term and taper.
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < PT; ++t)
@@ -329,7 +416,7 @@ template <int DT, bool SPLIT, int PT> __global__ __launch_bounds__(kThreads) voi
             const float v = acc[t][j] + __shfl_xor(acc[t][j], 8);
             const int col = lane & 15;
             if (col < 8)
-                s_pix[((wave * PT + t) * 16 + (lane >> 4) * 4 + j) * 8 + col] = v;
+                sm.pix[((wave * PT + t) * 16 + (lane >> 4) * 4 + j) * 8 + col] = v;
         }
     }
     __syncthreads();
@@ -340,7 +427,7 @@ template <int DT, bool SPLIT, int PT> __global__ __launch_bounds__(kThreads) voi
         const int p = pbase + q;
         if (p >= P)
             break;
-        const float *v = s_pix + q * 8;
+        const float *v = sm.pix + q * 8;
         cfloat px[4] = {{v[0], v[1]}, {v[2], v[3]}, {v[4], v[5]}, {v[6], v[7]}};
         if (ok)
         {
@@ -364,19 +451,14 @@ template <int DT, bool SPLIT, int PT> __global__ __launch_bounds__(kThreads) voi
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Degridder: one workgroup = one chunk of 4 waves x VT tiles x 16 visibilities of one subgrid.  The subgrid's
-// pixels stream through LDS in batches of 256 (taper and A-term applied, split into B fragments); the
-// accumulators stay in registers across batches, so each visibility is written once.
+// Degridder item: chunk vblk of 4 waves x VT tiles x 16 visibilities of subgrid s.  The subgrid's pixels stream
+// through LDS PB at a time (taper and A-term applied, split into B fragments); the accumulators stay in
+// registers across batches, so each visibility is written once.
 // ---------------------------------------------------------------------------------------------------------------
-template <int DT, bool SPLIT, int VT> __global__ __launch_bounds__(kThreads) void degridder_mfma(KernelArgs a)
+template <int DT, bool SPLIT, int VT, int PB, bool PM>
+__device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int vblk, DegridderShared<VT, PB> &sm)
 {
-    constexpr int kBlocks = kPixBatch / 16;
     constexpr int kChunk = kWaves * VT * 16;
-    __shared__ u32x4 s_b[kBlocks * 64];
-    __shared__ f32x4 s_lmn[kBlocks * 4 * 3]; // [block][k-group][x', y', n] over the group's 4 pixels
-    __shared__ float s_vis[kChunk * 8];
-
-    const int s = blockIdx.x;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int S = a.p.S, P = S * S;
@@ -386,8 +468,9 @@ template <int DT, bool SPLIT, int VT> __global__ __launch_bounds__(kThreads) voi
     const long long row0 = first_row(a.metadata, m);
     if (!subgrid_in_bounds(a.p, m, row0, c0, nc) || nc == 0)
         return; // uniform: nothing of this subgrid is written
-    const int V = m.nr_timesteps * nc;
-    const int v0 = blockIdx.y * kChunk;
+    Selection sel;
+    const int V = selection_of(a, s, m.nr_timesteps, nc, sel);
+    const int v0 = vblk * kChunk;
     if (v0 >= V)
         return; // uniform
     const long long C = a.p.nr_channels;
@@ -396,23 +479,27 @@ template <int DT, bool SPLIT, int VT> __global__ __launch_bounds__(kThreads) voi
     const float2 *sub = reinterpret_cast<const float2 *>(a.subgrids) + static_cast<size_t>(s) * kCorr * P;
     const float4 *aterms = reinterpret_cast<const float4 *>(a.aterms);
     const size_t abase = static_cast<size_t>(m.aterm_index) * a.p.nr_stations;
-    uint32_t *s_bw = reinterpret_cast<uint32_t *>(s_b);
-    float *s_lw = reinterpret_cast<float *>(s_lmn);
+    uint32_t *s_bw = reinterpret_cast<uint32_t *>(sm.b);
 
-    // Reduced phase coordinates of this lane's visibility in each tile (row lane % 16).
-    float du[VT], dv[VT], dw[VT];
+    // Reduced phase coefficients of visibility lane%16 of each tile; for PM only component lane/16 of them (the B
+    // operand of the phase MFMA).
+    float coef[VT], qu[VT], qv[VT], qw[VT];
 #pragma unroll
     for (int t = 0; t < VT; ++t)
     {
         const int v = v0 + (wave * VT + t) * 16 + (lane & 15);
-        du[t] = dv[t] = dw[t] = 0.F;
+        coef[t] = qu[t] = qv[t] = qw[t] = 0.F;
+        int tt = 0, c = 0;
         if (v < V)
+            sel.at(v, tt, c);
+        if (v < V && entry_ok(tt, c, m.nr_timesteps, nc))
         {
-            const int tt = v / nc, c = v - tt * nc;
             const Reduced rd = reduce_visibility(a.uvw + 3 * (row0 + tt), a.wavenumbers[c0 + c], off, delta, 1.F);
-            du[t] = rd.qu;
-            dv[t] = rd.qv;
-            dw[t] = rd.dw;
+            const int kk = lane >> 4;
+            qu[t] = rd.qu;
+            qv[t] = rd.qv;
+            qw[t] = rd.dw;
+            coef[t] = kk == 0 ? rd.qu : kk == 1 ? rd.qv : kk == 2 ? rd.dw : 0.F;
         }
     }
     f32x4 acc[VT];
@@ -420,11 +507,11 @@ template <int DT, bool SPLIT, int VT> __global__ __launch_bounds__(kThreads) voi
     for (int t = 0; t < VT; ++t)
         acc[t] = (f32x4){0.F, 0.F, 0.F, 0.F};
 
-    for (int pb = 0; pb < P; pb += kPixBatch)
+    for (int pb = 0; pb < P; pb += PB)
     {
-        const int np = min(kPixBatch, P - pb);
+        const int np = min(PB, P - pb);
         __syncthreads();
-        for (int q = tid; q < kPixBatch; q += kThreads)
+        for (int q = tid; q < PB; q += kThreads)
         {
             cfloat px[4] = {{0.F, 0.F}, {0.F, 0.F}, {0.F, 0.F}, {0.F, 0.F}};
             float l = 0.F, mm = 0.F, n = 0.F;
@@ -451,18 +538,45 @@ template <int DT, bool SPLIT, int VT> __global__ __launch_bounds__(kThreads) voi
             }
             const int blk = q >> 4, qq = q & 15;
             write_b_item<DT, SPLIT>(s_bw + blk * 256, qq, px);
-            float *r = s_lw + (blk * 4 + (qq >> 2)) * 12 + (qq & 3);
-            r[0] = l;
-            r[4] = mm;
-            r[8] = n;
+            if constexpr (PM)
+            {
+                float *r = sm.coord + blk * 64 + qq;
+                r[0] = l;
+                r[16] = mm;
+                r[32] = n;
+                r[48] = 0.F;
+            }
+            else
+            {
+                float *r = sm.coord + blk * 64 + (qq >> 2) * 16 + (qq & 3);
+                r[0] = l;
+                r[4] = mm;
+                r[8] = n;
+            }
         }
         __syncthreads();
         const int nblk = (np + 15) >> 4;
         for (int b = 0; b < nblk; ++b)
         {
-            const f32x4 *g = s_lmn + (b * 4 + (lane >> 4)) * 3;
-            const f32x4 l4 = g[0], m4 = g[1], n4 = g[2];
-            const u32x4 bf = s_b[b * 64 + lane];
+            const u32x4 bf = sm.b[b * 64 + lane];
+            f32x4 phs[VT];
+            if constexpr (PM)
+            {
+                const float cd = sm.coord[b * 64 + lane];
+#pragma unroll
+                for (int t = 0; t < VT; ++t)
+                    phs[t] = phase_mfma(cd, coef[t]); // phi(pixel 4*(lane/16)+j, visibility lane%16)
+            }
+            else
+            {
+                const f32x4 *g = reinterpret_cast<const f32x4 *>(sm.coord + b * 64 + (lane >> 4) * 16);
+                const f32x4 x4 = g[0], y4 = g[1], n4 = g[2];
+#pragma unroll
+                for (int t = 0; t < VT; ++t)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        phs[t][j] = __builtin_fmaf(qu[t], x4[j], __builtin_fmaf(qv[t], y4[j], qw[t] * n4[j]));
+            }
 #pragma unroll
             for (int t = 0; t < VT; ++t)
             {
@@ -470,9 +584,8 @@ template <int DT, bool SPLIT, int VT> __global__ __launch_bounds__(kThreads) voi
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                 {
-                    const float ph = __builtin_fmaf(du[t], l4[j], __builtin_fmaf(dv[t], m4[j], dw[t] * n4[j]));
-                    const float sn = __builtin_amdgcn_sinf(ph);
-                    const float cs = __builtin_amdgcn_cosf(ph);
+                    const float sn = __builtin_amdgcn_sinf(phs[t][j]);
+                    const float cs = __builtin_amdgcn_cosf(phs[t][j]);
                     if constexpr (SPLIT)
                         split2<DT>(cs, sn, hi[j], lo[j]);
                     else
@@ -495,7 +608,7 @@ template <int DT, bool SPLIT, int VT> __global__ __launch_bounds__(kThreads) voi
             const float v = acc[t][j] + __shfl_xor(acc[t][j], 8);
             const int col = lane & 15;
             if (col < 8)
-                s_vis[((wave * VT + t) * 16 + (lane >> 4) * 4 + j) * 8 + col] = v;
+                sm.vis[((wave * VT + t) * 16 + (lane >> 4) * 4 + j) * 8 + col] = v;
         }
     }
     __syncthreads();
@@ -505,69 +618,257 @@ template <int DT, bool SPLIT, int VT> __global__ __launch_bounds__(kThreads) voi
         const int v = v0 + q;
         if (v >= V)
             break;
-        const int tt = v / nc, c = v - tt * nc;
+        int tt, c;
+        sel.at(v, tt, c);
+        if (!entry_ok(tt, c, m.nr_timesteps, nc))
+            continue;
         const size_t o = ((row0 + tt) * C + c0 + c) * 2;
-        const float *x = s_vis + q * 8;
+        const float *x = sm.vis + q * 8;
         cfloat val[4] = {{x[0], x[1]}, {x[2], x[3]}, {x[4], x[5]}, {x[6], x[7]}};
         rotate4(val, reduce_visibility(a.uvw + 3 * (row0 + tt), a.wavenumbers[c0 + c], off, delta, 1.F).rot);
         vout[o] = make_float4(val[0].re, val[0].im, val[1].re, val[1].im);
         vout[o + 1] = make_float4(val[2].re, val[2].im, val[3].re, val[3].im);
     }
+    __syncthreads(); // LDS is reused by the next item of a persistent workgroup
 }
 
-template <int DT, bool SPLIT> hipError_t grid_mfma(const KernelArgs &a, hipStream_t st)
+// ---------------------------------------------------------------------------------------------------------------
+// Kernels: one-shot grids (workgroup b = item (b % n, b / n), so a subgrid's workgroups spread over the XCDs)
+// and persistent workgroups fed from the device work queue.
+// ---------------------------------------------------------------------------------------------------------------
+template <int DT, bool SPLIT, int PT, int VB, bool PM>
+__global__ __launch_bounds__(kThreads) void gridder_mfma(KernelArgs a)
 {
-    constexpr int PT = 4;
+    __shared__ GridderShared<PT, VB> sm;
+    const int pblk = blockIdx.x / a.p.nr_subgrids;
+    gridder_item<DT, SPLIT, PT, VB, PM>(a, blockIdx.x - pblk * a.p.nr_subgrids, pblk, sm);
+}
+
+template <int DT, bool SPLIT, int VT, int PB, bool PM>
+__global__ __launch_bounds__(kThreads) void degridder_mfma(KernelArgs a)
+{
+    __shared__ DegridderShared<VT, PB> sm;
+    const int vblk = blockIdx.x / a.p.nr_subgrids;
+    degridder_item<DT, SPLIT, VT, PB, PM>(a, blockIdx.x - vblk * a.p.nr_subgrids, vblk, sm);
+}
+
+// Takes the next queue item (all threads of the workgroup see the same one); returns false when the queue is
+// exhausted.  The counter only grows, so every workgroup leaves after at most nr_items + 1 attempts in total.
+__device__ __forceinline__ bool next_item(const QueueArgs &q, int &cls, int &s, int &blk)
+{
+    __shared__ int s_item;
+    if (threadIdx.x == 0)
+        s_item = atomicAdd(q.counter, 1);
+    __syncthreads();
+    const int i = s_item;
+    __syncthreads();
+    if (i >= q.nr_items)
+        return false;
+    const uint2 it = q.items[i];
+    cls = static_cast<int>(it.x >> 28);
+    s = static_cast<int>(it.x & 0x0fffffffu);
+    blk = static_cast<int>(it.y);
+    return true;
+}
+
+template <int DT, bool SPLIT, int PT, int VB>
+__global__ __launch_bounds__(kThreads) void gridder_mfma_queue(QueueArgs q)
+{
+    __shared__ GridderShared<PT, VB> sm;
+    int cls, s, blk;
+    while (next_item(q, cls, s, blk))
+    {
+        gridder_item<DT, SPLIT, PT, VB, false>(q.cls[cls], s, blk, sm);
+        __syncthreads();
+    }
+}
+
+template <int DT, bool SPLIT, int VT, int PB>
+__global__ __launch_bounds__(kThreads) void degridder_mfma_queue(QueueArgs q)
+{
+    __shared__ DegridderShared<VT, PB> sm;
+    int cls, s, blk;
+    while (next_item(q, cls, s, blk))
+        degridder_item<DT, SPLIT, VT, PB, false>(q.cls[cls], s, blk, sm);
+}
+
+__global__ void reset_counter_kernel(int *counter)
+{
+    *counter = 0;
+}
+
+// ---- launchers ------------------------------------------------------------------------------------------------
+
+constexpr int kDefaultPT = 4, kDefaultVB = 256, kDefaultVT = 4, kDefaultPB = 256;
+
+template <int DT, bool SPLIT> hipError_t grid_mfma(const KernelArgs &a, bool pm, hipStream_t st)
+{
     const int P = a.p.S * a.p.S;
-    const int per = kWaves * PT * 16;
-    gridder_mfma<DT, SPLIT, PT><<<dim3(a.p.nr_subgrids, (P + per - 1) / per), kThreads, 0, st>>>(a);
+    const int pix = kWaves * kDefaultPT * 16;
+    const long long blocks = static_cast<long long>(a.p.nr_subgrids) * ((P + pix - 1) / pix);
+    if (blocks > 0x7fffffffLL)
+        return hipErrorInvalidConfiguration;
+    if (pm)
+        gridder_mfma<DT, SPLIT, kDefaultPT, kDefaultVB, true><<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(a);
+    else
+        gridder_mfma<DT, SPLIT, kDefaultPT, kDefaultVB, false><<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(a);
     return hipGetLastError();
 }
 
-template <int DT, bool SPLIT> hipError_t degrid_mfma(const KernelArgs &a, hipStream_t st)
+template <int DT, bool SPLIT> hipError_t degrid_mfma(const KernelArgs &a, bool pm, hipStream_t st)
 {
-    constexpr int VT = 4;
-    constexpr int chunk = kWaves * VT * 16;
+    constexpr int chunk = kWaves * kDefaultVT * 16;
     const long long vmax = static_cast<long long>(a.p.max_nr_timesteps) * a.p.nr_channels;
-    const long long blocks = (vmax + chunk - 1) / chunk;
-    if (blocks > 65535)
+    const long long per = vmax > 0 ? (vmax + chunk - 1) / chunk : 1;
+    const long long blocks = static_cast<long long>(a.p.nr_subgrids) * per;
+    if (blocks > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
-    degridder_mfma<DT, SPLIT, VT><<<dim3(a.p.nr_subgrids, blocks > 0 ? blocks : 1), kThreads, 0, st>>>(a);
+    if (pm)
+        degridder_mfma<DT, SPLIT, kDefaultVT, kDefaultPB, true><<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(a);
+    else
+        degridder_mfma<DT, SPLIT, kDefaultVT, kDefaultPB, false><<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(a);
     return hipGetLastError();
+}
+
+// Tile shapes the persistent kernels are built for (the autotuner picks among them): {tiles per wave, LDS batch}.
+constexpr MixedShape kShapes[] = {{4, 256}, {2, 256}, {8, 256}, {4, 128}, {4, 512}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+template <int DT, bool SPLIT, int I> void *queue_fn(bool gridder)
+{
+    if (gridder)
+        return reinterpret_cast<void *>(&gridder_mfma_queue<DT, SPLIT, kShapes[I].tiles, kShapes[I].batch>);
+    return reinterpret_cast<void *>(&degridder_mfma_queue<DT, SPLIT, kShapes[I].tiles, kShapes[I].batch>);
+}
+
+template <int DT, bool SPLIT> void *queue_kernel(bool gridder, int shape)
+{
+    switch (shape)
+    {
+    case 0:
+        return queue_fn<DT, SPLIT, 0>(gridder);
+    case 1:
+        return queue_fn<DT, SPLIT, 1>(gridder);
+    case 2:
+        return queue_fn<DT, SPLIT, 2>(gridder);
+    case 3:
+        return queue_fn<DT, SPLIT, 3>(gridder);
+    case 4:
+        return queue_fn<DT, SPLIT, 4>(gridder);
+    default:
+        return nullptr;
+    }
+}
+
+void *queue_kernel_for(int precision, bool gridder, int shape)
+{
+    switch (precision)
+    {
+    case IDGX_MIXED_BF16X3:
+        return queue_kernel<kBF16, true>(gridder, shape);
+    case IDGX_MIXED_FP16X3:
+        return queue_kernel<kFP16, true>(gridder, shape);
+    case IDGX_MIXED_BF16:
+        return queue_kernel<kBF16, false>(gridder, shape);
+    case IDGX_MIXED_FP16:
+        return queue_kernel<kFP16, false>(gridder, shape);
+    default:
+        return nullptr;
+    }
+}
+
+hipError_t launch_queue(const QueueArgs &q, int precision, bool gridder, int shape, int workgroups, hipStream_t st)
+{
+    void *fn = queue_kernel_for(precision, gridder, shape);
+    if (!fn || workgroups <= 0)
+        return hipErrorInvalidValue;
+    QueueArgs arg = q;
+    void *args[] = {&arg};
+    return hipLaunchKernel(fn, dim3(workgroups), dim3(kThreads), args, 0, st);
 }
 } // namespace
 
-hipError_t launch_grid_mixed(const KernelArgs &a, int precision, hipStream_t st)
+hipError_t launch_grid_mixed(const KernelArgs &a, int precision, bool pm, hipStream_t st)
 {
     switch (precision)
     {
     case IDGX_MIXED_BF16X3:
-        return grid_mfma<kBF16, true>(a, st);
+        return grid_mfma<kBF16, true>(a, pm, st);
     case IDGX_MIXED_FP16X3:
-        return grid_mfma<kFP16, true>(a, st);
+        return grid_mfma<kFP16, true>(a, pm, st);
     case IDGX_MIXED_BF16:
-        return grid_mfma<kBF16, false>(a, st);
+        return grid_mfma<kBF16, false>(a, pm, st);
     case IDGX_MIXED_FP16:
-        return grid_mfma<kFP16, false>(a, st);
+        return grid_mfma<kFP16, false>(a, pm, st);
     default:
         return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_degrid_mixed(const KernelArgs &a, int precision, hipStream_t st)
+hipError_t launch_degrid_mixed(const KernelArgs &a, int precision, bool pm, hipStream_t st)
 {
     switch (precision)
     {
     case IDGX_MIXED_BF16X3:
-        return degrid_mfma<kBF16, true>(a, st);
+        return degrid_mfma<kBF16, true>(a, pm, st);
     case IDGX_MIXED_FP16X3:
-        return degrid_mfma<kFP16, true>(a, st);
+        return degrid_mfma<kFP16, true>(a, pm, st);
     case IDGX_MIXED_BF16:
-        return degrid_mfma<kBF16, false>(a, st);
+        return degrid_mfma<kBF16, false>(a, pm, st);
     case IDGX_MIXED_FP16:
-        return degrid_mfma<kFP16, false>(a, st);
+        return degrid_mfma<kFP16, false>(a, pm, st);
     default:
         return hipErrorInvalidValue;
     }
+}
+
+int mixed_shape_count()
+{
+    return kNumShapes;
+}
+
+MixedShape mixed_shape(int i)
+{
+    return kShapes[i];
+}
+
+int grid_pixels_per_item(int shape)
+{
+    return kWaves * kShapes[shape].tiles * 16;
+}
+
+int degrid_visibilities_per_item(int shape)
+{
+    return kWaves * kShapes[shape].tiles * 16;
+}
+
+hipError_t launch_grid_mixed_queue(const QueueArgs &q, int precision, int shape, int workgroups, hipStream_t st)
+{
+    return launch_queue(q, precision, true, shape, workgroups, st);
+}
+
+hipError_t launch_degrid_mixed_queue(const QueueArgs &q, int precision, int shape, int workgroups, hipStream_t st)
+{
+    return launch_queue(q, precision, false, shape, workgroups, st);
+}
+
+hipError_t launch_reset_counter(int *counter, hipStream_t st)
+{
+    reset_counter_kernel<<<1, 1, 0, st>>>(counter);
+    return hipGetLastError();
+}
+
+int queue_workgroups(int precision, bool gridder, int shape)
+{
+    void *fn = queue_kernel_for(precision, gridder, shape);
+    if (!fn)
+        return 0;
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
+    return cus * per_cu;
 }
 } // namespace idgx

@@ -179,12 +179,16 @@ template <int NPIX, int SC> __global__ __launch_bounds__(kThreads) void gridder_
     }
 }
 
-template <int NCH, int SC> __global__ __launch_bounds__(kThreads) void degridder_plain(KernelArgs a)
+template <int NCH, int SC> __global__ __launch_bounds__(kThreads) void degridder_plain(KernelArgs a, int per)
 {
     __shared__ float4 s_pix[2 * kDegridPixBatch];
     __shared__ float4 s_geo[kDegridPixBatch]; // l, m, n, phase offset
 
-    const int s = blockIdx.x;
+    // Workgroup b = vblock * n + s: a subgrid's workgroups are spread over the XCDs (balances uneven subgrids).
+    const int vblock = blockIdx.x / a.p.nr_subgrids;
+    const int s = blockIdx.x - vblock * a.p.nr_subgrids;
+    if (vblock >= per)
+        return; // uniform
     const int tid = threadIdx.x;
     const int S = a.p.S;
     const int P = S * S;
@@ -195,10 +199,10 @@ template <int NCH, int SC> __global__ __launch_bounds__(kThreads) void degridder
     if (!subgrid_in_bounds(a.p, m, row0, c0, nc) || nc == 0)
         return; // uniform: nothing of this subgrid is written
     const int nchunks = (nc + NCH - 1) / NCH;
-    if (static_cast<long long>(blockIdx.y) * kThreads >= static_cast<long long>(m.nr_timesteps) * nchunks)
+    if (static_cast<long long>(vblock) * kThreads >= static_cast<long long>(m.nr_timesteps) * nchunks)
         return; // uniform: no item of this block exists
 
-    const int item = blockIdx.y * kThreads + tid;
+    const int item = vblock * kThreads + tid;
     const int t = item / nchunks;
     const int cj = (item - t * nchunks) * NCH;
     const bool active = t < m.nr_timesteps;
@@ -317,14 +321,15 @@ hipError_t launch_grid_plain(const KernelArgs &a, bool accurate_sincos, hipStrea
 hipError_t launch_degrid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream)
 {
     const long long items = static_cast<long long>(a.p.max_nr_timesteps) * ((a.p.nr_channels + kDegridCh - 1) / kDegridCh);
-    const long long blocks = (items + kThreads - 1) / kThreads;
-    if (blocks > 65535)
+    const long long per = items > 0 ? (items + kThreads - 1) / kThreads : 1;
+    const long long blocks = static_cast<long long>(a.p.nr_subgrids) * per;
+    if (blocks > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
-    const dim3 grid(a.p.nr_subgrids, static_cast<unsigned>(blocks > 0 ? blocks : 1));
+    const dim3 grid(static_cast<unsigned>(blocks));
     if (accurate_sincos)
-        degridder_plain<kDegridCh, kSincosOcml><<<grid, kThreads, 0, stream>>>(a);
+        degridder_plain<kDegridCh, kSincosOcml><<<grid, kThreads, 0, stream>>>(a, static_cast<int>(per));
     else
-        degridder_plain<kDegridCh, kSincosHw><<<grid, kThreads, 0, stream>>>(a);
+        degridder_plain<kDegridCh, kSincosHw><<<grid, kThreads, 0, stream>>>(a, static_cast<int>(per));
     return hipGetLastError();
 }
 } // namespace idgx

@@ -36,6 +36,7 @@ ATERM_ADJOINT = 0
 ATERM_REFERENCE = 1
 
 FLAG_ACCURATE_SINCOS = 1
+FLAG_PHASE_MFMA = 2
 
 METADATA_DTYPE = np.dtype(
     [
@@ -83,6 +84,28 @@ class AdderPlan(ctypes.Structure):
     _fields_ = [("tile_size", ctypes.c_int32), ("nr_tiles", ctypes.c_int32), ("color_begin", ctypes.c_int32 * 5)]
 
 
+STAGE_GRID = 1
+STAGE_DEGRID = 2
+
+
+class BatchClass(ctypes.Structure):
+    """idgx_batch_class (include/idgx.h)."""
+
+    _fields_ = [("plan", Plan), ("h_metadata", ctypes.c_void_p), ("h_channels", ctypes.c_void_p),
+                ("h_vis_offset", ctypes.c_void_p), ("nr_vis_index", ctypes.c_int64)]
+
+
+class BatchClassData(ctypes.Structure):
+    _fields_ = [("d_metadata", ctypes.c_void_p), ("d_channels", ctypes.c_void_p), ("d_vis_index", ctypes.c_void_p),
+                ("d_vis_offset", ctypes.c_void_p), ("d_spheroidal", ctypes.c_void_p), ("d_aterms", ctypes.c_void_p),
+                ("d_subgrids", ctypes.c_void_p)]
+
+
+class BatchData(ctypes.Structure):
+    _fields_ = [("d_uvw", ctypes.c_void_p), ("d_wavenumbers", ctypes.c_void_p), ("d_visibilities", ctypes.c_void_p),
+                ("d_visibilities_out", ctypes.c_void_p), ("cls", BatchClassData * 4)]
+
+
 class IdgxError(RuntimeError):
     def __init__(self, status, what):
         super().__init__(f"{what}: {strerror(status)} ({status})")
@@ -111,6 +134,19 @@ def lib():
         L.idgx_adder_plan_build.argtypes = [ctypes.POINTER(Plan), vp, ctypes.POINTER(AdderPlan), vp, vp]
         L.idgx_add_subgrids.argtypes = [ctypes.POINTER(Plan), ctypes.POINTER(AdderPlan)] + [vp] * 6
         L.idgx_split_subgrids.argtypes = [ctypes.POINTER(Plan)] + [vp] * 4
+        L.idgx_batch_create.argtypes = [ctypes.POINTER(BatchClass), ctypes.c_int, ctypes.POINTER(vp)]
+        L.idgx_batch_destroy.argtypes = [vp]
+        L.idgx_batch_destroy.restype = None
+        L.idgx_batch_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.idgx_batch_get_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.idgx_batch_shape_info.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.idgx_batch_autotune.argtypes = [vp, ctypes.POINTER(BatchData), ctypes.c_int, vp,
+                                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+        L.idgx_batch_grid.argtypes = [vp, ctypes.POINTER(BatchData), vp]
+        L.idgx_batch_degrid.argtypes = [vp, ctypes.POINTER(BatchData), vp]
+        L.idgx_batch_capture.argtypes = [vp, ctypes.POINTER(BatchData), ctypes.c_int, vp]
+        L.idgx_batch_launch.argtypes = [vp, vp]
+        L.idgx_batch_items.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 4
         L.idgx_strerror.restype = ctypes.c_char_p
         L.idgx_strerror.argtypes = [ctypes.c_int]
         L.idgx_version.restype = ctypes.c_char_p
@@ -221,3 +257,94 @@ def split_subgrids(plan, d_metadata, d_grid, d_subgrids, stream=None):
                                    _stream(stream))
     if rc != OK:
         raise IdgxError(rc, "idgx_split_subgrids")
+
+
+def _check(rc, what):
+    if rc != OK:
+        raise IdgxError(rc, what)
+
+
+class Batch:
+    """A plan's persistent work-queue execution (idgx_batch_*).  `classes` is a list of (plan, metadata, channels,
+    vis_offset) with host numpy arrays (channels / vis_offset may be None)."""
+
+    def __init__(self, classes):
+        arr = (BatchClass * len(classes))()
+        self._keep = []
+        for k, (plan, md, ch, off) in enumerate(classes):
+            md = np.ascontiguousarray(md)
+            ch = None if ch is None else np.ascontiguousarray(ch)
+            off = None if off is None else np.ascontiguousarray(off, dtype=np.int64)
+            self._keep += [md, ch, off]
+            arr[k].plan = plan
+            arr[k].h_metadata = md.ctypes.data
+            arr[k].h_channels = None if ch is None else ch.ctypes.data
+            arr[k].h_vis_offset = None if off is None else off.ctypes.data
+            arr[k].nr_vis_index = 0 if off is None else int(off[-1])
+        h = ctypes.c_void_p()
+        _check(lib().idgx_batch_create(arr, len(classes), ctypes.byref(h)), "idgx_batch_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().idgx_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def data(d_uvw, d_wavenumbers, d_visibilities, d_visibilities_out, classes):
+        """classes: list of dicts with d_metadata, d_channels, d_vis_index, d_vis_offset, d_spheroidal, d_aterms,
+        d_subgrids (device tensors or None)."""
+        bd = BatchData()
+        bd.d_uvw, bd.d_wavenumbers = _dptr(d_uvw), _dptr(d_wavenumbers)
+        bd.d_visibilities, bd.d_visibilities_out = _dptr(d_visibilities), _dptr(d_visibilities_out)
+        for k, c in enumerate(classes):
+            for f in ("d_metadata", "d_channels", "d_vis_index", "d_vis_offset", "d_spheroidal", "d_aterms",
+                      "d_subgrids"):
+                setattr(bd.cls[k], f, _dptr(c.get(f)))
+        return bd
+
+    def grid(self, data, stream=None):
+        _check(lib().idgx_batch_grid(self.handle, ctypes.byref(data), _stream(stream)), "idgx_batch_grid")
+
+    def degrid(self, data, stream=None):
+        _check(lib().idgx_batch_degrid(self.handle, ctypes.byref(data), _stream(stream)), "idgx_batch_degrid")
+
+    def set_shape(self, gridder_shape, degridder_shape):
+        _check(lib().idgx_batch_set_shape(self.handle, gridder_shape, degridder_shape), "idgx_batch_set_shape")
+
+    def shape(self):
+        g, d = ctypes.c_int(), ctypes.c_int()
+        _check(lib().idgx_batch_get_shape(self.handle, ctypes.byref(g), ctypes.byref(d)), "idgx_batch_get_shape")
+        return g.value, d.value
+
+    @staticmethod
+    def shapes():
+        out = []
+        for i in range(lib().idgx_batch_shape_count()):
+            t, b = ctypes.c_int(), ctypes.c_int()
+            _check(lib().idgx_batch_shape_info(i, ctypes.byref(t), ctypes.byref(b)), "idgx_batch_shape_info")
+            out.append((t.value, b.value))
+        return out
+
+    def autotune(self, data, repeats=2, stream=None):
+        g, d = ctypes.c_float(), ctypes.c_float()
+        _check(lib().idgx_batch_autotune(self.handle, ctypes.byref(data), repeats, _stream(stream), ctypes.byref(g),
+                                         ctypes.byref(d)), "idgx_batch_autotune")
+        return g.value, d.value
+
+    def capture(self, data, stages, stream):
+        _check(lib().idgx_batch_capture(self.handle, ctypes.byref(data), stages, _stream(stream)), "idgx_batch_capture")
+
+    def launch(self, stream):
+        _check(lib().idgx_batch_launch(self.handle, _stream(stream)), "idgx_batch_launch")
+
+    def items(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(lib().idgx_batch_items(self.handle, *[ctypes.byref(x) for x in v]), "idgx_batch_items")
+        return tuple(x.value for x in v)

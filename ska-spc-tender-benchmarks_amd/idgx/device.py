@@ -102,3 +102,74 @@ class DeviceWorkload:
     def grid_array(self, d_grid):
         G = int(self.w["grid_size"])
         return download(d_grid, np.complex64, (4, G, G))
+
+
+def selection_lists(w, flags):
+    """Visibility selection of every subgrid of workload w from a boolean flag mask [rows][C] (True = flagged):
+    returns (vis_index uint32 [(t << 16) | c], vis_offset int64 [n + 1]) relative to each subgrid's window."""
+    md, ch = w["metadata"], w.get("channels")
+    C = w["wavenumbers"].shape[0]
+    index, offset = [], [0]
+    for s in range(md.size):
+        r0 = int(md["time_offset"][s]) + int(md["baseline_offset"][s]) - int(md["baseline_offset"][0])
+        T = int(md["nr_timesteps"][s])
+        c0, nc = (0, C) if ch is None else (int(ch["channel_begin"][s]), int(ch["nr_channels"][s]))
+        keep = ~flags[r0:r0 + T, c0:c0 + nc]
+        t, c = np.nonzero(keep)
+        index.append(((t.astype(np.uint32) << 16) | c.astype(np.uint32)))
+        offset.append(offset[-1] + t.size)
+    vi = np.concatenate(index) if index else np.zeros(0, np.uint32)
+    return np.ascontiguousarray(vi.astype(np.uint32)), np.asarray(offset, np.int64)
+
+
+class DeviceBatch:
+    """One or more subgrid classes of an observation resident in HBM, run through the batch API (persistent
+    work-queue kernels).  `classes` are workload dicts sharing uvw / wavenumbers / visibilities rows; each may
+    carry 'vis_index' / 'vis_offset' selection lists."""
+
+    def __init__(self, classes, precision="bf16x3", device="cuda", visibilities_on_device=None):
+        from . import Batch
+
+        torch = _torch()
+        self.device = device
+        w0 = classes[0]
+        self.C = int(w0["wavenumbers"].shape[0])
+        self.nr_rows = int(w0["uvw"].shape[0])
+        self.d_uvw = upload(w0["uvw"], device)
+        self.d_wavenumbers = upload(w0["wavenumbers"], device)
+        self.d_visibilities = visibilities_on_device if visibilities_on_device is not None else \
+            upload(w0["visibilities"], device)
+        self.d_vis_out = torch.zeros(self.nr_rows * self.C * 4 * 8, dtype=torch.uint8, device=device)
+        self.classes = []
+        specs, cls_data = [], []
+        for w in classes:
+            S, n = int(w["subgrid_size"]), int(w["nr_subgrids"])
+            c = dict(w=w, S=S, n=n,
+                     d_metadata=upload(w["metadata"], device),
+                     d_channels=None if w.get("channels") is None else upload(w["channels"], device),
+                     d_vis_index=None if w.get("vis_index") is None else upload(w["vis_index"], device),
+                     d_vis_offset=None if w.get("vis_offset") is None else upload(w["vis_offset"], device),
+                     d_spheroidal=upload(w["spheroidal"], device), d_aterms=upload(w["aterms"], device),
+                     d_subgrids=torch.zeros(max(n, 1) * 4 * S * S * 8, dtype=torch.uint8, device=device))
+            plan = make_plan(nr_subgrids=n, subgrid_size=S, grid_size=w["grid_size"], nr_channels=self.C,
+                             nr_stations=w["nr_stations"], nr_aterm_slots=w["nr_aterm_slots"], nr_rows=self.nr_rows,
+                             max_nr_timesteps=w["max_nr_timesteps"], image_size=w["image_size"], w_step=w["w_step"],
+                             precision=precision)
+            c["plan"] = plan
+            self.classes.append(c)
+            specs.append((plan, w["metadata"], w.get("channels"), w.get("vis_offset")))
+        self.batch = Batch(specs)
+        self.data = Batch.data(self.d_uvw, self.d_wavenumbers, self.d_visibilities, self.d_vis_out, self.classes)
+
+    def grid(self, stream=None):
+        self.batch.grid(self.data, stream)
+
+    def degrid(self, stream=None):
+        self.batch.degrid(self.data, stream)
+
+    def subgrids(self, k=0):
+        c = self.classes[k]
+        return download(c["d_subgrids"], np.complex64, (max(c["n"], 1), 4, c["S"], c["S"]))[: c["n"]]
+
+    def visibilities_out(self):
+        return download(self.d_vis_out, np.complex64, (self.nr_rows, self.C, 4))

@@ -1,0 +1,87 @@
+"""Turn rocprofv3 --pmc CSV output into per-kernel HBM traffic and instruction-mix figures.
+
+    python -m idgx.pmc_summary --cal-fetch DIR --cal-write DIR --fetch DIR --write DIR [--sq DIR] \
+        --cal-mib 1024 --out profiles/hbm_traffic_ska_low
+
+FETCH_SIZE / WRITE_SIZE are read in KiB per dispatch.  The calibration runs are a plain device-to-device copy of
+--cal-mib MiB (idgx.calibrate); scale factor = true bytes / counter bytes of that copy, applied to the kernels'
+counters.  Writes <out>.md (table, commands) and <out>.json (bytes per launch by kernel), both plain text.
+"""
+import argparse
+import collections
+import csv
+import glob
+import json
+import os
+import sys
+
+KERNELS = ["gridder_plain", "degridder_plain", "gridder_mfma", "degridder_mfma", "subgrid_fft_kernel",
+           "adder_kernel", "splitter_kernel"]
+
+
+def short_name(name):
+    for k in KERNELS:
+        if k in name and not (k.startswith("gridder") and "degridder" in name):
+            return k
+    if "copyBuffer" in name:
+        return "copy"
+    return None
+
+
+def load(d):
+    out = collections.defaultdict(lambda: collections.defaultdict(list))
+    for p in glob.glob(os.path.join(d, "*counter_collection.csv")):
+        for r in csv.DictReader(open(p)):
+            k = short_name(r["Kernel_Name"])
+            if k:
+                out[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return out
+
+
+def mean(v):
+    return sum(v) / len(v) if v else float("nan")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    for k in ("cal-fetch", "cal-write", "fetch", "write"):
+        ap.add_argument("--" + k, required=True)
+    ap.add_argument("--sq")
+    ap.add_argument("--cal-mib", type=float, default=1024)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--command", default="")
+    a = ap.parse_args()
+    true_bytes = a.cal_mib * 1024 * 1024
+    f_fetch = true_bytes / (mean(load(a.cal_fetch)["copy"]["FETCH_SIZE"]) * 1024)
+    f_write = true_bytes / (mean(load(a.cal_write)["copy"]["WRITE_SIZE"]) * 1024)
+    fetch, write = load(a.fetch), load(a.write)
+    sq = load(a.sq) if a.sq else {}
+    res = {"calibration": {"copy_mib": a.cal_mib, "fetch_factor": f_fetch, "write_factor": f_write}, "kernels": {}}
+    lines = ["# HBM traffic and instruction mix per kernel (rocprofv3 --pmc)", "",
+             f"Calibration: device-to-device copy of {a.cal_mib:.0f} MiB (idgx.calibrate); FETCH_SIZE factor "
+             f"{f_fetch:.3f}, WRITE_SIZE factor {f_write:.3f} (true bytes / counter bytes).", ""]
+    if a.command:
+        lines += ["Commands:", "", "```", a.command, "```", ""]
+    lines += ["| kernel | HBM read GB/launch | HBM write GB/launch | VALU instr | TRANS_F32 | FMA_F32 | MFMA MOPs BF16 |",
+              "|---|---|---|---|---|---|---|"]
+    for k in KERNELS:
+        if k not in fetch and k not in write:
+            continue
+        rd = mean(fetch[k]["FETCH_SIZE"]) * 1024 * f_fetch if k in fetch else float("nan")
+        wr = mean(write[k]["WRITE_SIZE"]) * 1024 * f_write if k in write else float("nan")
+        s = sq.get(k, {})
+        res["kernels"][k] = {"read_bytes": rd, "write_bytes": wr, "traffic_bytes": rd + wr,
+                             **{c: mean(v) for c, v in s.items()}}
+        lines.append(f"| `{k}` | {rd / 1e9:.2f} | {wr / 1e9:.2f} | {mean(s.get('SQ_INSTS_VALU', [])):.3e} | "
+                     f"{mean(s.get('SQ_INSTS_VALU_TRANS_F32', [])):.3e} | {mean(s.get('SQ_INSTS_VALU_FMA_F32', [])):.3e} | "
+                     f"{mean(s.get('SQ_INSTS_VALU_MFMA_MOPS_BF16', [])):.3e} |")
+    with open(a.out + ".md", "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(a.out + ".json", "w") as f:
+        json.dump(res, f, indent=1)
+    print("\n".join(lines))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

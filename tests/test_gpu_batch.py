@@ -1,0 +1,137 @@
+"""Batched execution (idgx_batch_*): persistent work-queue kernels over whole plans, visibility selections
+(flagging), mixed subgrid sizes in one launch, hipGraph capture and the autotuner."""
+import numpy as np
+import pytest
+
+from helpers import oracle_degrid, oracle_grid, per_subgrid_rel_l2, rel_l2
+from idgx import STAGE_DEGRID, STAGE_GRID, Batch, observation
+from idgx.device import DeviceBatch, DeviceWorkload, selection_lists, upload
+
+pytestmark = pytest.mark.gpu
+
+
+def _oneshot(w, precision):
+    dw = DeviceWorkload(w)
+    plan = dw.plan(precision=precision)
+    dw.grid(plan)
+    sub = dw.subgrids()
+    dw.degrid(plan)
+    return sub, dw.visibilities_out()
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
+def test_batch_equals_oneshot_kernels_bitwise(precision):
+    """Same item code and default tile shape as the one-shot kernels: identical bits, in any item order."""
+    full = observation.make_observation("ska_low", visibilities=False)
+    w = observation.subset(full, np.arange(5, full["nr_subgrids"], 311))
+    w["visibilities"] = observation.random_visibilities(w["nr_rows"], w["wavenumbers"].shape[0], seed=3)
+    sub1, vis1 = _oneshot(w, precision)
+    db = DeviceBatch([w], precision=precision)
+    assert db.batch.shape() == (0, 0)
+    db.grid()
+    db.degrid()
+    assert db.subgrids().tobytes() == sub1.tobytes()
+    assert db.visibilities_out().tobytes() == vis1.tobytes()
+
+
+def test_batch_with_flag_selection_vs_fp64():
+    w = observation.make_observation("small")
+    rng = np.random.default_rng(6)
+    frac = rng.uniform(0.0, 0.6, w["nr_rows"] // 128)  # per baseline
+    flags = rng.random((w["nr_rows"], w["wavenumbers"].shape[0])) < np.repeat(frac, 128)[:, None]
+    vi, vo = selection_lists(w, flags)
+    w["vis_index"], w["vis_offset"] = vi, vo
+    db = DeviceBatch([w], precision="bf16x3")
+    db.grid()
+    got = db.subgrids()
+    w0 = dict(w, visibilities=np.where(flags[..., None], 0, w["visibilities"]).astype(np.complex64))
+    g64 = oracle_grid(w0, fp64=True)
+    e = per_subgrid_rel_l2(got, g64)
+    print(f"flagged ({flags.mean():.2f} of visibilities): gridder worst subgrid vs FP64 {e.max():.3e}")
+    assert e.max() <= 5e-5
+    db.degrid()
+    vis = db.visibilities_out()
+    v64 = oracle_degrid(w0, got, fp64=True)
+    assert not np.any(vis[flags]), "flagged visibilities must not be written"
+    ev = rel_l2(vis[~flags], v64[~flags])
+    print(f"flagged degridder vs FP64 {ev:.3e}")
+    assert ev <= 5e-5
+
+
+def test_mixed_subgrid_sizes_in_one_launch():
+    w = observation.make_observation("small")
+    classes = []
+    for k, S in enumerate((24, 32, 48)):
+        idx = np.arange(k, w["nr_subgrids"], 3)
+        c = observation.subset(w, idx)
+        c["visibilities"] = w["visibilities"][c["source_rows"]]
+        c.update(subgrid_size=S, spheroidal=observation.prolate_spheroidal_taper(S),
+                 aterms=observation.identity_aterms(1, w["nr_stations"], S))
+        classes.append(c)
+    # Classes share the visibility rows: give every class all rows, with its own time offsets.
+    rows = np.concatenate([c["source_rows"] for c in classes])
+    base = 0
+    shared = []
+    for c in classes:
+        md = c["metadata"].copy()
+        md["time_offset"] += base
+        base += c["nr_rows"]
+        shared.append(dict(c, metadata=md))
+    uvw, vis = w["uvw"][rows], w["visibilities"][rows]
+    for c in shared:
+        c.update(uvw=uvw, visibilities=vis, nr_rows=rows.size)
+    db = DeviceBatch(shared, precision="bf16x3")
+    db.grid()
+    for k, c in enumerate(shared):
+        ref, _ = _oneshot(c, "bf16x3")
+        assert db.subgrids(k).tobytes() == ref.tobytes(), f"class S={c['subgrid_size']}"
+    db.degrid()
+    vis_b = db.visibilities_out()
+    for k, c in enumerate(shared):
+        dw = DeviceWorkload(c)
+        dw.d_subgrids.copy_(upload(db.subgrids(k)))
+        dw.degrid(dw.plan(precision="bf16x3"))
+        r0 = int(c["metadata"]["time_offset"][0])
+        r1 = r0 + int(c["metadata"]["nr_timesteps"].sum())
+        assert dw.visibilities_out()[r0:r1].tobytes() == vis_b[r0:r1].tobytes()
+
+
+def test_graph_replays_match_eager():
+    import torch
+
+    w = observation.make_observation("small")
+    db = DeviceBatch([w], precision="bf16x3")
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        db.grid(stream)
+        db.degrid(stream)
+    stream.synchronize()
+    sub0, vis0 = db.subgrids(), db.visibilities_out()
+    db.batch.capture(db.data, STAGE_GRID | STAGE_DEGRID, stream)
+    for r in range(3):
+        db.classes[0]["d_subgrids"].zero_()
+        db.d_vis_out.zero_()
+        torch.cuda.synchronize()
+        db.batch.launch(stream)
+        stream.synchronize()
+        assert db.subgrids().tobytes() == sub0.tobytes(), f"replay {r}"
+        assert db.visibilities_out().tobytes() == vis0.tobytes(), f"replay {r}"
+        x = torch.ones(1000, device="cuda").sum()  # eager work between replays
+        assert float(x) == 1000.0
+
+
+def test_autotuner_picks_a_shape_and_keeps_results():
+    w = observation.make_observation("small")
+    db = DeviceBatch([w], precision="bf16x3")
+    db.grid()
+    db.degrid()
+    sub0, vis0 = db.subgrids(), db.visibilities_out()
+    tg, td = db.batch.autotune(db.data, repeats=2)
+    gs, ds = db.batch.shape()
+    shapes = Batch.shapes()
+    print(f"autotune: gridder shape {shapes[gs]} {tg:.3f} ms, degridder shape {shapes[ds]} {td:.3f} ms")
+    assert 0 <= gs < len(shapes) and 0 <= ds < len(shapes) and tg > 0 and td > 0
+    db.grid()
+    db.degrid()
+    assert rel_l2(db.subgrids(), sub0) < 1e-6
+    assert rel_l2(db.visibilities_out(), vis0) < 1e-6
