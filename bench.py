@@ -65,22 +65,67 @@ def cpu_baseline(w, every, nthreads):
     return dict(grid_s=t1 - t0, degrid_s=t2 - t1, nvis=nvis, nsub=idx.size)
 
 
-def time_precision(dw, precision, steps, warmup, stream):
+class OneShot:
+    """One launch per direction over the whole plan (idgx_grid / idgx_degrid)."""
+
+    def __init__(self, dw, precision):
+        self.dw, self.precision, self.path = dw, precision, "one-shot"
+        self.plan = dw.plan(precision=precision)
+
+    def grid(self, stream):
+        self.dw.grid(self.plan, stream=stream)
+
+    def degrid(self, stream):
+        self.dw.degrid(self.plan, stream=stream)
+
+    def buffers(self):
+        return self.dw.d_subgrids, self.dw.d_vis_out
+
+
+class Batched:
+    """Persistent work-queue kernels over the whole plan (idgx_batch_*), tile shape chosen by the autotuner."""
+
+    def __init__(self, w, d_vis, precision, stream):
+        from idgx import Batch
+        from idgx.device import DeviceBatch
+
+        self.precision, self.path = precision, "batched"
+        self.db = DeviceBatch([w], precision=precision, visibilities_on_device=d_vis)
+        tg, td = self.db.batch.autotune(self.db.data, repeats=2, stream=stream)
+        gs, ds = self.db.batch.shape()
+        shapes = Batch.shapes()
+        items = self.db.batch.items()
+        self.info = {"gridder_shape": {"tiles_per_wave": shapes[gs][0], "lds_visibilities": shapes[gs][1]},
+                     "degridder_shape": {"tiles_per_wave": shapes[ds][0], "lds_pixels": shapes[ds][1]},
+                     "queue_items": {"gridder": items[0], "degridder": items[1]},
+                     "persistent_workgroups": {"gridder": items[2], "degridder": items[3]},
+                     "autotune_ms": {"gridder": tg, "degridder": td}}
+
+    def grid(self, stream):
+        self.db.grid(stream)
+
+    def degrid(self, stream):
+        self.db.degrid(stream)
+
+    def buffers(self):
+        return self.db.classes[0]["d_subgrids"], self.db.d_vis_out
+
+
+def time_form(runner, steps, warmup, stream):
     """Average gridder and degridder kernel time (s) over `steps` round trips, HIP events on `stream`."""
     import torch
 
-    plan = dw.plan(precision=precision)
     for _ in range(warmup):
-        dw.grid(plan, stream=stream)
-        dw.degrid(plan, stream=stream)
+        runner.grid(stream)
+        runner.degrid(stream)
     torch.cuda.synchronize()
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
     t0 = time.perf_counter()
     for k in range(steps):
         ev[k][0].record(stream)
-        dw.grid(plan, stream=stream)
+        runner.grid(stream)
         ev[k][1].record(stream)
-        dw.degrid(plan, stream=stream)
+        runner.degrid(stream)
         ev[k][2].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -89,7 +134,7 @@ def time_precision(dw, precision, steps, warmup, stream):
     return t_grid, t_degrid, wall / steps
 
 
-def accuracy_vs_fp64(dw, w, host_vis, idx):
+def accuracy_vs_fp64(runner, w, host_vis, idx):
     """Relative L2 error vs the FP64 oracle of the gridder output and the degridder output (degridding the same
     subgrids) on the sample of subgrids idx; returns (whole-sample, worst subgrid) per direction."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -101,11 +146,12 @@ def accuracy_vs_fp64(dw, w, host_vis, idx):
 
     sw = observation.subset(dict(w, visibilities=host_vis), idx)
     S, C = int(w["subgrid_size"]), int(w["wavenumbers"].shape[0])
-    dev = dw.d_subgrids.device
-    subs = dw.d_subgrids.view(-1, 4 * S * S * 8)[torch.from_numpy(idx).to(dev)].cpu().numpy()
+    d_sub, d_vis_out = runner.buffers()
+    dev = d_sub.device
+    subs = d_sub.view(-1, 4 * S * S * 8)[torch.from_numpy(idx).to(dev)].cpu().numpy()
     subs = subs.view(np.complex64).reshape(idx.size, 4, S, S)
     rows = torch.from_numpy(sw["source_rows"]).to(dev)
-    vis = dw.d_vis_out.view(-1, C * 4 * 8)[rows].cpu().numpy().view(np.complex64).reshape(-1, C, 4)
+    vis = d_vis_out.view(-1, C * 4 * 8)[rows].cpu().numpy().view(np.complex64).reshape(-1, C, 4)
     g64 = oracle_grid(sw, fp64=True)
     v64 = oracle_degrid(sw, subs, fp64=True)
     md, ch = sw["metadata"], sw["channels"]
@@ -130,7 +176,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="ska_low")
-    ap.add_argument("--precision", default="bf16x3", help="headline form (the plain fp32 form is timed beside it)")
+    ap.add_argument("--precision", default="fp16x3", help="headline mixed form (the plain fp32 form is timed beside it)")
+    ap.add_argument("--no-batch", action="store_true", help="headline on the one-shot kernels instead of the batch")
     ap.add_argument("--cpu-every", type=int, default=0, help="CPU baseline on every k-th subgrid (0 = auto)")
     ap.add_argument("--check-every", type=int, default=0, help="FP64 accuracy sample: every k-th subgrid (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -167,16 +214,25 @@ def main():
     check_every = args.check_every or max(1, w["nr_subgrids"] // 160)
     check_idx = np.arange(check_every // 2, w["nr_subgrids"], check_every)
 
+    runners = [OneShot(dw, "fp32")]
+    if args.precision != "fp32":
+        runners.append(OneShot(dw, args.precision))
+        if not args.no_batch:
+            runners.append(Batched(w, d_vis, args.precision, stream))
     forms = []
-    for prec in dict.fromkeys(["fp32", args.precision]):
-        tg, td, step = time_precision(dw, prec, args.steps, args.warmup, stream)
-        rec = {"precision": prec, "gridder_ms": tg * 1e3, "degridder_ms": td * 1e3, "step_ms": step * 1e3,
+    for runner in runners:
+        prec = runner.precision
+        tg, td, step = time_form(runner, args.steps, args.warmup, stream)
+        rec = {"precision": prec, "path": runner.path, "gridder_ms": tg * 1e3, "degridder_ms": td * 1e3,
+               "step_ms": step * 1e3,
                "round_trip_mvis_s": nvis / (tg + td) / 1e6, "gridder_mvis_s": nvis / tg / 1e6,
                "degridder_mvis_s": nvis / td / 1e6,
                "gridder_tflops": flops / tg / 1e12, "degridder_tflops": flops / td / 1e12}
+        if hasattr(runner, "info"):
+            rec.update(runner.info)
         if not args.no_check:
-            rec["rel_l2_vs_fp64"] = accuracy_vs_fp64(dw, w, host_vis, check_idx)
-        print(f"bench: {prec}: gridder {tg * 1e3:.2f} ms, degridder {td * 1e3:.2f} ms, "
+            rec["rel_l2_vs_fp64"] = accuracy_vs_fp64(runner, w, host_vis, check_idx)
+        print(f"bench: {prec} {runner.path}: gridder {tg * 1e3:.2f} ms, degridder {td * 1e3:.2f} ms, "
               f"{rec['round_trip_mvis_s']:.1f} Mvis/s", file=sys.stderr, flush=True)
         forms.append(rec)
     head = forms[-1]
@@ -226,6 +282,7 @@ def main():
             "subgrids": int(w["nr_subgrids"]),
             "visibilities": nvis,
             "precision": head["precision"],
+            "path": head["path"],
         },
         "speedup_vs_plain_fp32": head["round_trip_mvis_s"] / plain["round_trip_mvis_s"],
         "forms": forms,

@@ -81,13 +81,49 @@ template <int DT> __device__ __forceinline__ void unpack2(uint32_t p, float &y0,
     }
 }
 
+// x - float(half of h) in one v_fma_mix_f32 (the half is read straight from the packed register).  The result
+// is exact: x and its RNE fp16 rounding are close enough for the difference to be a float.
+__device__ __forceinline__ float sub_f16_lo(float x, uint32_t h)
+{
+    float r;
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(x));
+    return r;
+}
+__device__ __forceinline__ float sub_f16_hi(float x, uint32_t h)
+{
+    float r;
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(x));
+    return r;
+}
+
 // hi = round(x), lo = round(x - hi), both packed as pairs.
 template <int DT> __device__ __forceinline__ void split2(float x0, float x1, uint32_t &hi, uint32_t &lo)
 {
     hi = pack2<DT>(x0, x1);
-    float h0, h1;
-    unpack2<DT>(hi, h0, h1);
-    lo = pack2<DT>(x0 - h0, x1 - h1);
+    if constexpr (DT == kFP16)
+    {
+        lo = pack2<DT>(sub_f16_lo(x0, hi), sub_f16_hi(x1, hi));
+    }
+    else
+    {
+        float h0, h1;
+        unpack2<DT>(hi, h0, h1);
+        lo = pack2<DT>(x0 - h0, x1 - h1);
+    }
+}
+
+// FP16 B operands are scaled by a power of two per LDS batch so that the largest staged value is about 2^14:
+// no overflow (65504) for any data, and the hi/lo split keeps its relative precision.  The accumulators are
+// rescaled exactly when a later batch needs a larger scale, and the final result is multiplied back.
+constexpr int kF16TargetExp = 14;
+
+__device__ __forceinline__ int scale_exponent(float maxabs)
+{
+    if (!(maxabs > 0.F))
+        return -126;
+    int e;
+    (void)frexpf(maxabs, &e); // maxabs in [2^(e-1), 2^e)
+    return e - kF16TargetExp;
 }
 
 // Phases of a 16 x 16 tile: D[i][j] = sum_k A[i][k] B[k][j], K = 4 (fourth component zero).
@@ -262,8 +298,22 @@ __device__ __forceinline__ bool entry_ok(int t, int c, int nr_timesteps, int nc)
     return t >= 0 && t < nr_timesteps && c >= 0 && c < nc;
 }
 
+// Largest value over the workgroup (non-negative floats order like their bit patterns).
+__device__ __forceinline__ float block_max(float v, unsigned &slot)
+{
+    if (threadIdx.x == 0)
+        slot = 0u;
+    __syncthreads();
+    atomicMax(&slot, __float_as_uint(v));
+    __syncthreads();
+    const float m = __uint_as_float(slot);
+    __syncthreads();
+    return m;
+}
+
 template <int PT, int VB> struct GridderShared
 {
+    unsigned maxbits;
     u32x4 b[(VB / 16) * 64];
     float coef[(VB / 16) * 64];
     float pix[kWaves * PT * 16 * 8];
@@ -271,6 +321,7 @@ template <int PT, int VB> struct GridderShared
 
 template <int VT, int PB> struct DegridderShared
 {
+    unsigned maxbits;
     u32x4 b[(PB / 16) * 64];
     float coord[(PB / 16) * 64];
     float vis[kWaves * VT * 16 * 8];
@@ -312,6 +363,7 @@ __device__ __forceinline__ void gridder_item(const KernelArgs &a, int s, int pbl
 #pragma unroll
     for (int t = 0; t < PT; ++t)
         acc[t] = (f32x4){0.F, 0.F, 0.F, 0.F};
+    int scale_exp = -126; // FP16: current power-of-two scale of the staged B operand
 
     Selection sel;
     const int V = ok ? selection_of(a, s, m.nr_timesteps, nc, sel) : 0;
@@ -323,42 +375,78 @@ __device__ __forceinline__ void gridder_item(const KernelArgs &a, int s, int pbl
     for (int v0 = 0; v0 < V; v0 += VB)
     {
         __syncthreads();
-        for (int q = tid; q < VB; q += kThreads)
+        // Stage visibilities: reduced phase coefficients and B-operand words (scaled for FP16).
+        constexpr int kPer = (VB + kThreads - 1) / kThreads;
+        cfloat val[kPer][4];
+        Reduced rd[kPer];
+        float vmax = 0.F;
+#pragma unroll
+        for (int i = 0; i < kPer; ++i)
         {
-            // Stage one visibility: reduced phase coefficients and its B-operand words.
+            const int q = tid + i * kThreads;
             const int v = v0 + q;
-            cfloat val[4] = {{0.F, 0.F}, {0.F, 0.F}, {0.F, 0.F}, {0.F, 0.F}};
-            Reduced rd = {0.F, 0.F, 0.F, 0.F};
+#pragma unroll
+            for (int pol = 0; pol < 4; ++pol)
+                val[i][pol] = {0.F, 0.F};
+            rd[i] = {0.F, 0.F, 0.F, 0.F};
             int t = 0, c = 0;
-            if (v < V)
+            if (q < VB && v < V)
                 sel.at(v, t, c);
-            if (v < V && entry_ok(t, c, m.nr_timesteps, nc))
+            if (q < VB && v < V && entry_ok(t, c, m.nr_timesteps, nc))
             {
                 const long long r = row0 + t;
-                rd = reduce_visibility(a.uvw + 3 * r, a.wavenumbers[c0 + c], off, delta, -1.F);
+                rd[i] = reduce_visibility(a.uvw + 3 * r, a.wavenumbers[c0 + c], off, delta, -1.F);
                 const float4 x0 = vis[(r * C + c0 + c) * 2], x1 = vis[(r * C + c0 + c) * 2 + 1];
-                val[0] = {x0.x, x0.y};
-                val[1] = {x0.z, x0.w};
-                val[2] = {x1.x, x1.y};
-                val[3] = {x1.z, x1.w};
-                rotate4(val, rd.rot);
+                val[i][0] = {x0.x, x0.y};
+                val[i][1] = {x0.z, x0.w};
+                val[i][2] = {x1.x, x1.y};
+                val[i][3] = {x1.z, x1.w};
+                rotate4(val[i], rd[i].rot);
+#pragma unroll
+                for (int pol = 0; pol < 4; ++pol)
+                    vmax = fmaxf(vmax, fmaxf(fabsf(val[i][pol].re), fabsf(val[i][pol].im)));
             }
+        }
+        if constexpr (DT == kFP16)
+        {
+            const int e_new = max(scale_exp, scale_exponent(block_max(vmax, sm.maxbits)));
+            if (e_new > scale_exp)
+            {
+                const float f = ldexpf(1.F, scale_exp - e_new);
+#pragma unroll
+                for (int t = 0; t < PT; ++t)
+                    acc[t] *= f;
+                scale_exp = e_new;
+            }
+            const float inv = ldexpf(1.F, -scale_exp);
+#pragma unroll
+            for (int i = 0; i < kPer; ++i)
+#pragma unroll
+                for (int pol = 0; pol < 4; ++pol)
+                    val[i][pol] = {val[i][pol].re * inv, val[i][pol].im * inv};
+        }
+#pragma unroll
+        for (int i = 0; i < kPer; ++i)
+        {
+            const int q = tid + i * kThreads;
+            if (q >= VB)
+                break;
             const int blk = q >> 4, qq = q & 15;
-            write_b_item<DT, SPLIT>(s_bw + blk * 256, qq, val);
+            write_b_item<DT, SPLIT>(s_bw + blk * 256, qq, val[i]);
             if constexpr (PM)
             {
                 float *cf = sm.coef + blk * 64 + qq;
-                cf[0] = rd.qu;
-                cf[16] = rd.qv;
-                cf[32] = rd.dw;
+                cf[0] = rd[i].qu;
+                cf[16] = rd[i].qv;
+                cf[32] = rd[i].dw;
                 cf[48] = 0.F;
             }
             else
             {
                 float *cf = sm.coef + blk * 64 + (qq >> 2) * 16 + (qq & 3);
-                cf[0] = rd.qu;
-                cf[4] = rd.qv;
-                cf[8] = rd.dw;
+                cf[0] = rd[i].qu;
+                cf[4] = rd[i].qv;
+                cf[8] = rd[i].dw;
             }
         }
         __syncthreads();
@@ -407,13 +495,14 @@ __device__ __forceinline__ void gridder_item(const KernelArgs &a, int s, int pbl
 
     // Sum the high and low column halves, gather each pixel's 8 values in LDS, apply A-term and taper.
     __syncthreads();
+    const float unscale = DT == kFP16 ? ldexpf(1.F, scale_exp) : 1.F;
 #pragma unroll
     for (int t = 0; t < PT; ++t)
     {
 #pragma unroll
         for (int j = 0; j < 4; ++j)
         {
-            const float v = acc[t][j] + __shfl_xor(acc[t][j], 8);
+            const float v = (acc[t][j] + __shfl_xor(acc[t][j], 8)) * unscale;
             const int col = lane & 15;
             if (col < 8)
                 sm.pix[((wave * PT + t) * 16 + (lane >> 4) * 4 + j) * 8 + col] = v;
@@ -506,15 +595,24 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
 #pragma unroll
     for (int t = 0; t < VT; ++t)
         acc[t] = (f32x4){0.F, 0.F, 0.F, 0.F};
+    int scale_exp = -126; // FP16: current power-of-two scale of the staged B operand
 
     for (int pb = 0; pb < P; pb += PB)
     {
         const int np = min(PB, P - pb);
         __syncthreads();
-        for (int q = tid; q < PB; q += kThreads)
+        constexpr int kPer = (PB + kThreads - 1) / kThreads;
+        cfloat px[kPer][4];
+        float cx[kPer], cy[kPer], cn[kPer];
+        float vmax = 0.F;
+#pragma unroll
+        for (int i = 0; i < kPer; ++i)
         {
-            cfloat px[4] = {{0.F, 0.F}, {0.F, 0.F}, {0.F, 0.F}, {0.F, 0.F}};
-            float l = 0.F, mm = 0.F, n = 0.F;
+            const int q = tid + i * kThreads;
+#pragma unroll
+            for (int pol = 0; pol < 4; ++pol)
+                px[i][pol] = {0.F, 0.F};
+            cx[i] = cy[i] = cn[i] = 0.F;
             if (q < np)
             {
                 const int p = pb + q;
@@ -523,35 +621,63 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
                 for (int pol = 0; pol < 4; ++pol)
                 {
                     const float2 g = sub[pol * P + p];
-                    px[pol] = {taper * g.x, taper * g.y};
+                    px[i][pol] = {taper * g.x, taper * g.y};
                 }
                 cfloat a1[4], a2[4];
                 load_jones(aterms, (abase + m.station1) * P + p, a1);
                 load_jones(aterms, (abase + m.station2) * P + p, a2);
                 if (a.p.convention == IDGX_ATERM_REFERENCE)
-                    apply_a1h_p_a2(px, a1, a2);
+                    apply_a1h_p_a2(px[i], a1, a2);
                 else
-                    apply_a1_p_a2h(px, a1, a2);
-                l = pixel_offset(p % S, S);
-                mm = pixel_offset(p / S, S);
-                n = pixel_n_f(pixel_lm_f(p % S, S, a.p.image_size), pixel_lm_f(p / S, S, a.p.image_size));
+                    apply_a1_p_a2h(px[i], a1, a2);
+                cx[i] = pixel_offset(p % S, S);
+                cy[i] = pixel_offset(p / S, S);
+                cn[i] = pixel_n_f(pixel_lm_f(p % S, S, a.p.image_size), pixel_lm_f(p / S, S, a.p.image_size));
+#pragma unroll
+                for (int pol = 0; pol < 4; ++pol)
+                    vmax = fmaxf(vmax, fmaxf(fabsf(px[i][pol].re), fabsf(px[i][pol].im)));
             }
+        }
+        if constexpr (DT == kFP16)
+        {
+            const int e_new = max(scale_exp, scale_exponent(block_max(vmax, sm.maxbits)));
+            if (e_new > scale_exp)
+            {
+                const float f = ldexpf(1.F, scale_exp - e_new);
+#pragma unroll
+                for (int t = 0; t < VT; ++t)
+                    acc[t] *= f;
+                scale_exp = e_new;
+            }
+            const float inv = ldexpf(1.F, -scale_exp);
+#pragma unroll
+            for (int i = 0; i < kPer; ++i)
+#pragma unroll
+                for (int pol = 0; pol < 4; ++pol)
+                    px[i][pol] = {px[i][pol].re * inv, px[i][pol].im * inv};
+        }
+#pragma unroll
+        for (int i = 0; i < kPer; ++i)
+        {
+            const int q = tid + i * kThreads;
+            if (q >= PB)
+                break;
             const int blk = q >> 4, qq = q & 15;
-            write_b_item<DT, SPLIT>(s_bw + blk * 256, qq, px);
+            write_b_item<DT, SPLIT>(s_bw + blk * 256, qq, px[i]);
             if constexpr (PM)
             {
                 float *r = sm.coord + blk * 64 + qq;
-                r[0] = l;
-                r[16] = mm;
-                r[32] = n;
+                r[0] = cx[i];
+                r[16] = cy[i];
+                r[32] = cn[i];
                 r[48] = 0.F;
             }
             else
             {
                 float *r = sm.coord + blk * 64 + (qq >> 2) * 16 + (qq & 3);
-                r[0] = l;
-                r[4] = mm;
-                r[8] = n;
+                r[0] = cx[i];
+                r[4] = cy[i];
+                r[8] = cn[i];
             }
         }
         __syncthreads();
@@ -599,13 +725,14 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
     }
 
     // Combine column halves, gather 8 values per visibility in LDS, write 32 B per visibility.
+    const float unscale = DT == kFP16 ? ldexpf(1.F, scale_exp) : 1.F;
 #pragma unroll
     for (int t = 0; t < VT; ++t)
     {
 #pragma unroll
         for (int j = 0; j < 4; ++j)
         {
-            const float v = acc[t][j] + __shfl_xor(acc[t][j], 8);
+            const float v = (acc[t][j] + __shfl_xor(acc[t][j], 8)) * unscale;
             const int col = lane & 15;
             if (col < 8)
                 sm.vis[((wave * VT + t) * 16 + (lane >> 4) * 4 + j) * 8 + col] = v;

@@ -127,3 +127,25 @@ def test_mixed_adjoint_dot_product():
     rhs = np.vdot(G.astype(np.complex128), gV)
     print(f"adjoint: |lhs - rhs| / |rhs| = {abs(lhs - rhs) / abs(rhs):.3e}")
     assert abs(lhs - rhs) <= 5e-5 * abs(rhs)
+
+
+@pytest.mark.parametrize("precision", ["fp16x3", "bf16x3"])
+def test_mixed_forms_handle_large_dynamic_range(precision):
+    """Visibilities and pixels far beyond the FP16 range (65504): the per-batch power-of-two scaling keeps the
+    FP16 forms exact to the split precision; BF16 has the FP32 exponent range anyway."""
+    w = observation.make_observation("small")
+    w["visibilities"] = (w["visibilities"] * np.float32(3e6)).astype(np.complex64)
+    w["visibilities"][::7] *= np.float32(1e-4)  # and a spread of magnitudes
+    dw = DeviceWorkload(w)
+    plan = dw.plan(precision=precision)
+    dw.grid(plan)
+    got = dw.subgrids()
+    g64 = oracle_grid(w, fp64=True)
+    eg = per_subgrid_rel_l2(got, g64).max()
+    rng = np.random.default_rng(2)
+    sub = (rng.standard_normal(got.shape) + 1j * rng.standard_normal(got.shape)).astype(np.complex64) * np.float32(2e7)
+    dw.d_subgrids.copy_(upload(sub))
+    dw.degrid(plan)
+    ev = rel_l2(dw.visibilities_out(), oracle_degrid(w, sub, fp64=True))
+    print(f"{precision} large range: gridder worst subgrid {eg:.3e}, degridder {ev:.3e}")
+    assert np.isfinite(got).all() and eg <= SPLIT_BAR and ev <= SPLIT_BAR
