@@ -95,8 +95,10 @@ class Batched:
         gs, ds = self.db.batch.shape()
         shapes = Batch.shapes()
         items = self.db.batch.items()
-        self.info = {"gridder_shape": {"tiles_per_wave": shapes[gs][0], "lds_visibilities": shapes[gs][1]},
-                     "degridder_shape": {"tiles_per_wave": shapes[ds][0], "lds_pixels": shapes[ds][1]},
+        self.info = {"gridder_shape": {"tiles_per_wave": shapes[gs][0], "lds_visibilities": shapes[gs][1],
+                                       "phase_mfma": shapes[gs][2]},
+                     "degridder_shape": {"tiles_per_wave": shapes[ds][0], "lds_pixels": shapes[ds][1],
+                                         "phase_mfma": shapes[ds][2]},
                      "queue_items": {"gridder": items[0], "degridder": items[1]},
                      "persistent_workgroups": {"gridder": items[2], "degridder": items[3]},
                      "autotune_ms": {"gridder": tg, "degridder": td}}

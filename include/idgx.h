@@ -217,9 +217,10 @@ typedef struct
 int idgx_batch_create(const idgx_batch_class *classes, int nr_classes, idgx_batch **batch);
 void idgx_batch_destroy(idgx_batch *batch);
 
-/* Tile shapes: shape i = (tiles of 16 pixels / visibilities per wave, LDS staging depth). */
+/* Tile shapes: shape i = (tiles of 16 pixels / visibilities per wave, LDS staging depth, phases on the matrix
+ * core). */
 int idgx_batch_shape_count(void);
-int idgx_batch_shape_info(int shape, int *tiles_per_wave, int *lds_batch);
+int idgx_batch_shape_info(int shape, int *tiles_per_wave, int *lds_batch, int *phase_mfma);
 int idgx_batch_set_shape(idgx_batch *batch, int gridder_shape, int degridder_shape);
 int idgx_batch_get_shape(const idgx_batch *batch, int *gridder_shape, int *degridder_shape);
 

@@ -263,7 +263,7 @@ extern "C" int idgx_batch_shape_count(void)
     return idgx::mixed_shape_count();
 }
 
-extern "C" int idgx_batch_shape_info(int shape, int *tiles_per_wave, int *lds_batch)
+extern "C" int idgx_batch_shape_info(int shape, int *tiles_per_wave, int *lds_batch, int *phase_mfma)
 {
     if (shape < 0 || shape >= idgx::mixed_shape_count())
         return IDGX_ERR_INVALID_ARGUMENT;
@@ -272,6 +272,8 @@ extern "C" int idgx_batch_shape_info(int shape, int *tiles_per_wave, int *lds_ba
         *tiles_per_wave = s.tiles;
     if (lds_batch)
         *lds_batch = s.batch;
+    if (phase_mfma)
+        *phase_mfma = s.phase_mfma;
     return IDGX_OK;
 }
 

@@ -46,6 +46,7 @@ struct MixedShape
 {
     int tiles;
     int batch;
+    int phase_mfma; // phases from v_mfma_f32_16x16x4_f32 instead of VALU fused multiply-adds
 };
 
 hipError_t launch_grid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream);

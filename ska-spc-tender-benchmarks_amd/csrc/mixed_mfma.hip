@@ -798,25 +798,25 @@ __device__ __forceinline__ bool next_item(const QueueArgs &q, int &cls, int &s, 
     return true;
 }
 
-template <int DT, bool SPLIT, int PT, int VB>
+template <int DT, bool SPLIT, int PT, int VB, bool PM>
 __global__ __launch_bounds__(kThreads) void gridder_mfma_queue(QueueArgs q)
 {
     __shared__ GridderShared<PT, VB> sm;
     int cls, s, blk;
     while (next_item(q, cls, s, blk))
     {
-        gridder_item<DT, SPLIT, PT, VB, false>(q.cls[cls], s, blk, sm);
+        gridder_item<DT, SPLIT, PT, VB, PM>(q.cls[cls], s, blk, sm);
         __syncthreads();
     }
 }
 
-template <int DT, bool SPLIT, int VT, int PB>
+template <int DT, bool SPLIT, int VT, int PB, bool PM>
 __global__ __launch_bounds__(kThreads) void degridder_mfma_queue(QueueArgs q)
 {
     __shared__ DegridderShared<VT, PB> sm;
     int cls, s, blk;
     while (next_item(q, cls, s, blk))
-        degridder_item<DT, SPLIT, VT, PB, false>(q.cls[cls], s, blk, sm);
+        degridder_item<DT, SPLIT, VT, PB, PM>(q.cls[cls], s, blk, sm);
 }
 
 __global__ void reset_counter_kernel(int *counter)
@@ -857,15 +857,18 @@ template <int DT, bool SPLIT> hipError_t degrid_mfma(const KernelArgs &a, bool p
     return hipGetLastError();
 }
 
-// Tile shapes the persistent kernels are built for (the autotuner picks among them): {tiles per wave, LDS batch}.
-constexpr MixedShape kShapes[] = {{4, 256}, {2, 256}, {8, 256}, {4, 128}, {4, 512}};
+// Tile shapes the persistent kernels are built for (the autotuner picks among them): {tiles per wave, LDS batch,
+// phases on the matrix core}.
+constexpr MixedShape kShapes[] = {{4, 256, 0}, {2, 256, 0}, {8, 256, 0}, {4, 128, 0},
+                                  {4, 512, 0}, {4, 256, 1}, {8, 256, 1}, {2, 256, 1}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 template <int DT, bool SPLIT, int I> void *queue_fn(bool gridder)
 {
+    constexpr MixedShape sh = kShapes[I];
     if (gridder)
-        return reinterpret_cast<void *>(&gridder_mfma_queue<DT, SPLIT, kShapes[I].tiles, kShapes[I].batch>);
-    return reinterpret_cast<void *>(&degridder_mfma_queue<DT, SPLIT, kShapes[I].tiles, kShapes[I].batch>);
+        return reinterpret_cast<void *>(&gridder_mfma_queue<DT, SPLIT, sh.tiles, sh.batch, sh.phase_mfma != 0>);
+    return reinterpret_cast<void *>(&degridder_mfma_queue<DT, SPLIT, sh.tiles, sh.batch, sh.phase_mfma != 0>);
 }
 
 template <int DT, bool SPLIT> void *queue_kernel(bool gridder, int shape)
@@ -882,6 +885,12 @@ template <int DT, bool SPLIT> void *queue_kernel(bool gridder, int shape)
         return queue_fn<DT, SPLIT, 3>(gridder);
     case 4:
         return queue_fn<DT, SPLIT, 4>(gridder);
+    case 5:
+        return queue_fn<DT, SPLIT, 5>(gridder);
+    case 6:
+        return queue_fn<DT, SPLIT, 6>(gridder);
+    case 7:
+        return queue_fn<DT, SPLIT, 7>(gridder);
     default:
         return nullptr;
     }

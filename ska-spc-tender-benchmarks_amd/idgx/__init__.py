@@ -139,7 +139,7 @@ def lib():
         L.idgx_batch_destroy.restype = None
         L.idgx_batch_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.idgx_batch_get_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
-        L.idgx_batch_shape_info.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.idgx_batch_shape_info.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
         L.idgx_batch_autotune.argtypes = [vp, ctypes.POINTER(BatchData), ctypes.c_int, vp,
                                           ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         L.idgx_batch_grid.argtypes = [vp, ctypes.POINTER(BatchData), vp]
@@ -327,9 +327,10 @@ class Batch:
     def shapes():
         out = []
         for i in range(lib().idgx_batch_shape_count()):
-            t, b = ctypes.c_int(), ctypes.c_int()
-            _check(lib().idgx_batch_shape_info(i, ctypes.byref(t), ctypes.byref(b)), "idgx_batch_shape_info")
-            out.append((t.value, b.value))
+            t, b, pm = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            _check(lib().idgx_batch_shape_info(i, ctypes.byref(t), ctypes.byref(b), ctypes.byref(pm)),
+                   "idgx_batch_shape_info")
+            out.append((t.value, b.value, pm.value))
         return out
 
     def autotune(self, data, repeats=2, stream=None):
