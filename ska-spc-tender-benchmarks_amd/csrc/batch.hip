@@ -71,10 +71,15 @@ struct Item
 std::vector<Item> build_items(const idgx_batch *b, bool gridder, int shape)
 {
     std::vector<Item> items;
-    const long long per = gridder ? idgx::grid_pixels_per_item(shape) : idgx::degrid_visibilities_per_item(shape);
+    const long long per = idgx::degrid_visibilities_per_item(shape);
     for (int k = 0; k < b->nr_classes; ++k)
     {
-        const long long P = static_cast<long long>(b->plans[k].subgrid_size) * b->plans[k].subgrid_size;
+        const int S = b->plans[k].subgrid_size;
+        const long long P = static_cast<long long>(S) * S;
+        const int gitems = idgx::grid_items_per_subgrid(shape, S);
+        std::vector<double> gpix(gitems);
+        for (int i = 0; i < gitems; ++i)
+            gpix[i] = static_cast<double>(idgx::grid_item_pixels(shape, S, i));
         for (int s = 0; s < b->plans[k].nr_subgrids; ++s)
         {
             const long long V = b->vis_count[k][s];
@@ -82,9 +87,8 @@ std::vector<Item> build_items(const idgx_batch *b, bool gridder, int shape)
             if (gridder)
             {
                 // Every pixel block is written (zeros for an empty subgrid); the epilogue costs ~ V of 1.
-                for (long long p0 = 0, blk = 0; p0 < P; p0 += per, ++blk)
-                    items.push_back({{tag, static_cast<uint32_t>(blk)},
-                                     static_cast<double>(std::min(per, P - p0)) * static_cast<double>(V + 1)});
+                for (int blk = 0; blk < gitems; ++blk)
+                    items.push_back({{tag, static_cast<uint32_t>(blk)}, gpix[blk] * static_cast<double>(V + 1)});
             }
             else
             {

@@ -49,6 +49,34 @@ struct MixedShape
     int phase_mfma; // phases from v_mfma_f32_16x16x4_f32 instead of VALU fused multiply-adds
 };
 
+// Pixel units of the mixed gridder.  The subgrid is cut into strips wd columns wide (wd the largest of 16, 8, 4, 2
+// that divides S) and row blocks of tiles * rt rows (rt = 16 / wd); one unit (strip, row block) holds one wave's
+// pixels, four consecutive units one workgroup item.  Row r of tile t of unit u is pixel
+//   x = (u % ns) * wd + r % wd,   y = (u / ns) * rows + t * rt + r / wd,
+// so a lane's column x is the same in every tile of its unit.
+struct GridUnits
+{
+    int wd, rt, ns, rows, units;
+};
+
+__host__ __device__ inline GridUnits grid_units(int S, int tiles)
+{
+    GridUnits g;
+    g.wd = S % 16 == 0 ? 16 : S % 8 == 0 ? 8 : S % 4 == 0 ? 4 : 2;
+    g.rt = 16 / g.wd;
+    g.ns = S / g.wd;
+    g.rows = tiles * g.rt;
+    g.units = g.ns * ((S + g.rows - 1) / g.rows);
+    return g;
+}
+
+// Pixel slots of the mixed degridder: rows padded to a multiple of 4 columns, so that the 4 pixels a lane group
+// takes from one 16-slot block share a row.  Slots with x >= S carry zero B words.
+__host__ __device__ inline int degrid_row_stride(int S)
+{
+    return (S + 3) & ~3;
+}
+
 hipError_t launch_grid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream);
 hipError_t launch_degrid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream);
 hipError_t launch_grid_mixed(const KernelArgs &a, int precision, bool phase_mfma, hipStream_t stream);
@@ -57,8 +85,9 @@ hipError_t launch_degrid_mixed(const KernelArgs &a, int precision, bool phase_mf
 // Persistent work-queue forms (the queue items are built on the host, see batch.cpp).
 int mixed_shape_count();
 MixedShape mixed_shape(int i);
-int grid_pixels_per_item(int shape);      // pixels one gridder item covers
-int degrid_visibilities_per_item(int shape); // visibilities one degridder item covers
+int grid_items_per_subgrid(int shape, int S);            // gridder items of one subgrid
+long long grid_item_pixels(int shape, int S, int item);  // pixels gridder item `item` of a subgrid writes
+int degrid_visibilities_per_item(int shape);             // visibilities one degridder item covers
 hipError_t launch_grid_mixed_queue(const QueueArgs &q, int precision, int shape, int workgroups, hipStream_t st);
 hipError_t launch_degrid_mixed_queue(const QueueArgs &q, int precision, int shape, int workgroups, hipStream_t st);
 hipError_t launch_reset_counter(int *counter, hipStream_t st);

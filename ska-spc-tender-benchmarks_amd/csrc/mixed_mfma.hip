@@ -82,7 +82,8 @@ template <int DT> __device__ __forceinline__ void unpack2(uint32_t p, float &y0,
 }
 
 // x - float(half of h) in one v_fma_mix_f32 (the half is read straight from the packed register).  The result
-// is exact: x and its RNE fp16 rounding are close enough for the difference to be a float.
+// is exact: x and its RNE fp16 rounding are close enough for the difference to be a float.  (Writing the rounded
+// residual straight into a register half with v_fma_mixlo/mixhi_f16 saves the pack but measured slower.)
 __device__ __forceinline__ float sub_f16_lo(float x, uint32_t h)
 {
     float r;
@@ -343,21 +344,28 @@ __device__ __forceinline__ void gridder_item(const KernelArgs &a, int s, int pbl
     channel_window(a, s, c0, nc);
     const long long row0 = first_row(a.metadata, m);
     const bool ok = subgrid_in_bounds(a.p, m, row0, c0, nc);
-    const int pbase = pblk * kPix;
     const long long C = a.p.nr_channels;
 
-    // Pixel coordinates of row lane%16 of each tile: (x', y', n); for PM only component lane/16 of them (the B
-    // operand of the phase MFMA).
-    float coord[PT], xp[PT], yp[PT], nn[PT];
+    // This wave's unit: a strip of gu.wd columns by gu.rows rows; lane row r of tile t is pixel (px, y0 + t*rt).
+    const GridUnits gu = grid_units(S, PT);
+    const int unit = kWaves * pblk + wave;
+    const bool active = unit < gu.units; // wave-uniform
+    const int px = (unit % gu.ns) * gu.wd + (lane & 15) % gu.wd;
+    const int py0 = (unit / gu.ns) * gu.rows + (lane & 15) / gu.wd;
+
+    // Pixel coordinates of row lane%16 of each tile: x' (the same in every tile), y', n; for PM only component
+    // lane/16 of them (the B operand of the phase MFMA).  Rows past the subgrid repeat the last row.
+    const float xp = pixel_offset(px, S);
+    const float lx = pixel_lm_f(px, S, a.p.image_size);
+    float coord[PT], yp[PT], nn[PT];
 #pragma unroll
     for (int t = 0; t < PT; ++t)
     {
-        const int p = min(pbase + (wave * PT + t) * 16 + (lane & 15), P - 1);
+        const int y = min(py0 + t * gu.rt, S - 1);
         const int kk = lane >> 4;
-        xp[t] = pixel_offset(p % S, S);
-        yp[t] = pixel_offset(p / S, S);
-        nn[t] = pixel_n_f(pixel_lm_f(p % S, S, a.p.image_size), pixel_lm_f(p / S, S, a.p.image_size));
-        coord[t] = kk == 0 ? xp[t] : kk == 1 ? yp[t] : kk == 2 ? nn[t] : 0.F;
+        yp[t] = pixel_offset(y, S);
+        nn[t] = pixel_n_f(lx, pixel_lm_f(y, S, a.p.image_size));
+        coord[t] = kk == 0 ? xp : kk == 1 ? yp[t] : kk == 2 ? nn[t] : 0.F;
     }
     f32x4 acc[PT];
 #pragma unroll
@@ -450,7 +458,7 @@ __device__ __forceinline__ void gridder_item(const KernelArgs &a, int s, int pbl
             }
         }
         __syncthreads();
-        const int nblk = min(kBlocks, (V - v0 + 15) >> 4);
+        const int nblk = active ? min(kBlocks, (V - v0 + 15) >> 4) : 0;
         for (int b = 0; b < nblk; ++b)
         {
             const u32x4 bf = sm.b[b * 64 + lane];
@@ -464,13 +472,18 @@ __device__ __forceinline__ void gridder_item(const KernelArgs &a, int s, int pbl
             }
             else
             {
+                // phi = qu x' + qv y' + dw n with the x' term formed once per visibility: 2 FMAs per phase.
                 const f32x4 *cf = reinterpret_cast<const f32x4 *>(sm.coef + b * 64 + (lane >> 4) * 16);
                 const f32x4 du = cf[0], dv = cf[1], dw = cf[2];
+                f32x4 ux;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    ux[j] = du[j] * xp;
 #pragma unroll
                 for (int t = 0; t < PT; ++t)
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        phs[t][j] = __builtin_fmaf(du[j], xp[t], __builtin_fmaf(dv[j], yp[t], dw[j] * nn[t]));
+                        phs[t][j] = __builtin_fmaf(dv[j], yp[t], __builtin_fmaf(dw[j], nn[t], ux[j]));
             }
 #pragma unroll
             for (int t = 0; t < PT; ++t)
@@ -513,9 +526,13 @@ __device__ __forceinline__ void gridder_item(const KernelArgs &a, int s, int pbl
     const float4 *aterms = reinterpret_cast<const float4 *>(a.aterms);
     for (int q = tid; q < kPix; q += kThreads)
     {
-        const int p = pbase + q;
-        if (p >= P)
-            break;
+        // Slot q = (wave w, tile t, row r) of the accumulator gather above.
+        const int w = q / (PT * 16), t = (q >> 4) % PT, r = q & 15;
+        const int u = kWaves * pblk + w;
+        const int y = (u / gu.ns) * gu.rows + t * gu.rt + r / gu.wd;
+        if (u >= gu.units || y >= S)
+            continue;
+        const int p = y * S + (u % gu.ns) * gu.wd + r % gu.wd;
         const float *v = sm.pix + q * 8;
         cfloat px[4] = {{v[0], v[1]}, {v[2], v[3]}, {v[4], v[5]}, {v[6], v[7]}};
         if (ok)
@@ -597,9 +614,11 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
         acc[t] = (f32x4){0.F, 0.F, 0.F, 0.F};
     int scale_exp = -126; // FP16: current power-of-two scale of the staged B operand
 
-    for (int pb = 0; pb < P; pb += PB)
+    // Pixel slots: rows of Sp = S rounded up to a multiple of 4, so each lane group's 4 slots share a row.
+    const int Sp = degrid_row_stride(S), nslots = S * Sp;
+    for (int pb = 0; pb < nslots; pb += PB)
     {
-        const int np = min(PB, P - pb);
+        const int np = min(PB, nslots - pb);
         __syncthreads();
         constexpr int kPer = (PB + kThreads - 1) / kThreads;
         cfloat px[kPer][4];
@@ -613,9 +632,15 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
             for (int pol = 0; pol < 4; ++pol)
                 px[i][pol] = {0.F, 0.F};
             cx[i] = cy[i] = cn[i] = 0.F;
+            const int sy = (pb + q) / Sp, sx = pb + q - sy * Sp;
             if (q < np)
             {
-                const int p = pb + q;
+                cx[i] = pixel_offset(sx, S);
+                cy[i] = pixel_offset(sy, S);
+            }
+            if (q < np && sx < S)
+            {
+                const int p = sy * S + sx;
                 const float taper = a.spheroidal[p];
 #pragma unroll
                 for (int pol = 0; pol < 4; ++pol)
@@ -630,9 +655,7 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
                     apply_a1h_p_a2(px[i], a1, a2);
                 else
                     apply_a1_p_a2h(px[i], a1, a2);
-                cx[i] = pixel_offset(p % S, S);
-                cy[i] = pixel_offset(p / S, S);
-                cn[i] = pixel_n_f(pixel_lm_f(p % S, S, a.p.image_size), pixel_lm_f(p / S, S, a.p.image_size));
+                cn[i] = pixel_n_f(pixel_lm_f(sx, S, a.p.image_size), pixel_lm_f(sy, S, a.p.image_size));
 #pragma unroll
                 for (int pol = 0; pol < 4; ++pol)
                     vmax = fmaxf(vmax, fmaxf(fabsf(px[i][pol].re), fabsf(px[i][pol].im)));
@@ -695,13 +718,18 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
             }
             else
             {
-                const f32x4 *g = reinterpret_cast<const f32x4 *>(sm.coord + b * 64 + (lane >> 4) * 16);
-                const f32x4 x4 = g[0], y4 = g[1], n4 = g[2];
+                // The lane group's 4 pixels share y': its term is formed once per visibility, 2 FMAs per phase.
+                const float *g = sm.coord + b * 64 + (lane >> 4) * 16;
+                const f32x4 x4 = *reinterpret_cast<const f32x4 *>(g), n4 = *reinterpret_cast<const f32x4 *>(g + 8);
+                const float y = g[4];
 #pragma unroll
                 for (int t = 0; t < VT; ++t)
+                {
+                    const float vy = qv[t] * y;
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        phs[t][j] = __builtin_fmaf(qu[t], x4[j], __builtin_fmaf(qv[t], y4[j], qw[t] * n4[j]));
+                        phs[t][j] = __builtin_fmaf(qu[t], x4[j], __builtin_fmaf(qw[t], n4[j], vy));
+                }
             }
 #pragma unroll
             for (int t = 0; t < VT; ++t)
@@ -830,9 +858,8 @@ constexpr int kDefaultPT = 4, kDefaultVB = 256, kDefaultVT = 4, kDefaultPB = 256
 
 template <int DT, bool SPLIT> hipError_t grid_mfma(const KernelArgs &a, bool pm, hipStream_t st)
 {
-    const int P = a.p.S * a.p.S;
-    const int pix = kWaves * kDefaultPT * 16;
-    const long long blocks = static_cast<long long>(a.p.nr_subgrids) * ((P + pix - 1) / pix);
+    const GridUnits gu = grid_units(a.p.S, kDefaultPT);
+    const long long blocks = static_cast<long long>(a.p.nr_subgrids) * ((gu.units + kWaves - 1) / kWaves);
     if (blocks > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
     if (pm)
@@ -968,9 +995,18 @@ MixedShape mixed_shape(int i)
     return kShapes[i];
 }
 
-int grid_pixels_per_item(int shape)
+int grid_items_per_subgrid(int shape, int S)
 {
-    return kWaves * kShapes[shape].tiles * 16;
+    return (grid_units(S, kShapes[shape].tiles).units + kWaves - 1) / kWaves;
+}
+
+long long grid_item_pixels(int shape, int S, int item)
+{
+    const GridUnits gu = grid_units(S, kShapes[shape].tiles);
+    long long n = 0;
+    for (int u = kWaves * item; u < min(kWaves * (item + 1), gu.units); ++u)
+        n += static_cast<long long>(gu.wd) * min(gu.rows, S - (u / gu.ns) * gu.rows);
+    return n;
 }
 
 int degrid_visibilities_per_item(int shape)

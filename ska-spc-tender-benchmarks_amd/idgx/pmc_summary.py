@@ -15,22 +15,35 @@ import json
 import os
 import sys
 
-KERNELS = ["gridder_plain", "degridder_plain", "gridder_mfma", "degridder_mfma", "subgrid_fft_kernel",
-           "adder_kernel", "splitter_kernel"]
+KERNELS = ["gridder_plain", "degridder_plain", "gridder_mfma_queue", "degridder_mfma_queue", "gridder_mfma",
+           "degridder_mfma", "subgrid_fft_kernel", "adder_kernel", "splitter_kernel"]
 
 
 def short_name(name):
+    """Kernel family plus template arguments ("gridder_mfma_queue<1, true, 8, 256, false>"), so that every
+    instantiation (precision, tile shape) keeps its own row; "copy" for the calibration copy."""
     for k in KERNELS:
-        if k in name and not (k.startswith("gridder") and "degridder" in name):
-            return k
-    if "copyBuffer" in name:
+        i = name.find(k)
+        if i < 0 or (k.startswith("gridder") and name[i - 2:i] == "de"):
+            continue
+        if name[i + len(k):i + len(k) + 1] not in ("<", "(", ""):
+            continue
+        j = i + len(k)
+        if name[j:j + 1] == "<":
+            depth = 0
+            for q in range(j, len(name)):
+                depth += {"<": 1, ">": -1}.get(name[q], 0)
+                if depth == 0:
+                    return name[i:q + 1]
+        return k
+    if "copyBuffer" in name or "copy" in name.lower():
         return "copy"
     return None
 
 
 def load(d):
     out = collections.defaultdict(lambda: collections.defaultdict(list))
-    for p in glob.glob(os.path.join(d, "*counter_collection.csv")):
+    for p in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
         for r in csv.DictReader(open(p)):
             k = short_name(r["Kernel_Name"])
             if k:
@@ -62,18 +75,22 @@ def main():
              f"{f_fetch:.3f}, WRITE_SIZE factor {f_write:.3f} (true bytes / counter bytes).", ""]
     if a.command:
         lines += ["Commands:", "", "```", a.command, "```", ""]
-    lines += ["| kernel | HBM read GB/launch | HBM write GB/launch | VALU instr | TRANS_F32 | FMA_F32 | MFMA MOPs BF16 |",
-              "|---|---|---|---|---|---|---|"]
-    for k in KERNELS:
-        if k not in fetch and k not in write:
+    lines += ["| kernel | launches | HBM read GB/launch | HBM write GB/launch | VALU instr | TRANS_F32 | FMA_F32 | "
+              "MFMA MOPs F16 | MFMA MOPs BF16 |",
+              "|---|---|---|---|---|---|---|---|---|"]
+    for k in sorted(set(fetch) | set(write)):
+        if k == "copy":
             continue
         rd = mean(fetch[k]["FETCH_SIZE"]) * 1024 * f_fetch if k in fetch else float("nan")
         wr = mean(write[k]["WRITE_SIZE"]) * 1024 * f_write if k in write else float("nan")
         s = sq.get(k, {})
         res["kernels"][k] = {"read_bytes": rd, "write_bytes": wr, "traffic_bytes": rd + wr,
                              **{c: mean(v) for c, v in s.items()}}
-        lines.append(f"| `{k}` | {rd / 1e9:.2f} | {wr / 1e9:.2f} | {mean(s.get('SQ_INSTS_VALU', [])):.3e} | "
+        n = len(fetch[k]["FETCH_SIZE"]) if k in fetch else len(write[k]["WRITE_SIZE"])
+        res["kernels"][k]["launches"] = n
+        lines.append(f"| `{k}` | {n} | {rd / 1e9:.2f} | {wr / 1e9:.2f} | {mean(s.get('SQ_INSTS_VALU', [])):.3e} | "
                      f"{mean(s.get('SQ_INSTS_VALU_TRANS_F32', [])):.3e} | {mean(s.get('SQ_INSTS_VALU_FMA_F32', [])):.3e} | "
+                     f"{mean(s.get('SQ_INSTS_VALU_MFMA_MOPS_F16', [])):.3e} | "
                      f"{mean(s.get('SQ_INSTS_VALU_MFMA_MOPS_BF16', [])):.3e} |")
     with open(a.out + ".md", "w") as f:
         f.write("\n".join(lines) + "\n")
