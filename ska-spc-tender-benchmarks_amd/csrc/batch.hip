@@ -9,8 +9,12 @@
 // together.  Persistent workgroups (as many as fit on the chip) take items through one atomic counter, which a
 // one-thread kernel clears before every launch (no memset nodes, so the sequence can be captured).
 //
-// Device memory (queues, counters) is allocated once in idgx_batch_create; the execution entry points allocate
-// nothing.
+// Degridder pixels: before the persistent degridder, one workgroup per subgrid applies taper and A-terms and
+// splits the pixels into the packed words the degridder stages (32 B per pixel slot), so the work items copy
+// them instead of recomputing them for every chunk of visibilities.
+//
+// Device memory (queues, counters, the prepared pixels: 32 B per pixel per subgrid) is allocated once in
+// idgx_batch_create; the execution entry points allocate nothing.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,6 +38,10 @@ struct idgx_batch
     uint2 *d_grid_items = nullptr;
     uint2 *d_degrid_items = nullptr;
     int *d_counters = nullptr; // [0] gridder, [1] degridder
+    // Degridder pixels prepared once per subgrid and pass (per class): split words, FP16 scales, slot n table.
+    uint4 *d_prepared[idgx::kMaxClasses] = {};
+    int *d_prepared_scale[idgx::kMaxClasses] = {};
+    float *d_slot_n[idgx::kMaxClasses] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
 };
@@ -167,6 +175,12 @@ int make_queue_args(const idgx_batch *b, const idgx_batch_data *d, bool gridder,
         a.subgrids_out = c.d_subgrids;
         a.vis_index = c.d_vis_index;
         a.vis_offset = reinterpret_cast<const long long *>(c.d_vis_offset);
+        if (!gridder)
+        {
+            a.prepared = b->d_prepared[k];
+            a.prepared_scale = b->d_prepared_scale[k];
+            a.slot_n = b->d_slot_n[k];
+        }
     }
     q.items = gridder ? b->d_grid_items : b->d_degrid_items;
     q.nr_items = gridder ? b->nr_grid_items : b->nr_degrid_items;
@@ -182,7 +196,11 @@ int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st)
         return rc;
     if (q.nr_items == 0)
         return IDGX_OK;
-    hipError_t e = idgx::launch_reset_counter(q.counter, st);
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < b->nr_classes && !gridder && e == hipSuccess; ++k)
+        e = idgx::launch_degrid_prepare(q.cls[k], b->precision, st);
+    if (e == hipSuccess)
+        e = idgx::launch_reset_counter(q.counter, st);
     if (e != hipSuccess)
         return to_status(e);
     e = gridder ? idgx::launch_grid_mixed_queue(q, b->precision, b->grid_shape, b->grid_wg, st)
@@ -248,6 +266,16 @@ extern "C" int idgx_batch_create(const idgx_batch_class *classes, int nr_classes
         e = hipMalloc(&b->d_degrid_items, b->degrid_capacity * sizeof(uint2));
     if (e == hipSuccess)
         e = hipMalloc(&b->d_counters, 2 * sizeof(int));
+    for (int k = 0; k < nr_classes && e == hipSuccess; ++k)
+    {
+        const int S = b->plans[k].subgrid_size;
+        const long long n = std::max(1, b->plans[k].nr_subgrids);
+        e = hipMalloc(&b->d_prepared[k], idgx::degrid_prepared_bytes(S, n));
+        if (e == hipSuccess)
+            e = hipMalloc(&b->d_prepared_scale[k], n * sizeof(int));
+        if (e == hipSuccess)
+            e = hipMalloc(&b->d_slot_n[k], static_cast<size_t>(S) * idgx::degrid_row_stride(S) * sizeof(float));
+    }
     int rc = to_status(e);
     if (rc == IDGX_OK)
         rc = upload_queue(b, true, 0);
@@ -447,5 +475,14 @@ extern "C" void idgx_batch_destroy(idgx_batch *b)
         (void)hipFree(b->d_degrid_items);
     if (b->d_counters)
         (void)hipFree(b->d_counters);
+    for (int k = 0; k < idgx::kMaxClasses; ++k)
+    {
+        if (b->d_prepared[k])
+            (void)hipFree(b->d_prepared[k]);
+        if (b->d_prepared_scale[k])
+            (void)hipFree(b->d_prepared_scale[k]);
+        if (b->d_slot_n[k])
+            (void)hipFree(b->d_slot_n[k]);
+    }
     delete b;
 }

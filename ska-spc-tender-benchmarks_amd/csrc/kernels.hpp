@@ -27,6 +27,12 @@ struct KernelArgs
     // channel window.  Null = every (t, c) of the subgrid's window.
     const uint32_t *vis_index;
     const long long *vis_offset;
+    // Mixed degridder pixels prepared once per subgrid (null: every work item computes them).  Per pixel slot
+    // 8 words: for each correlation the packed high and low (re, im) parts of taper x A-term-applied pixel,
+    // FP16 forms scaled by 2^-prepared_scale[s]; slot_n holds n of every slot (shared by the class).
+    const uint4 *prepared;
+    const int *prepared_scale;
+    const float *slot_n;
 };
 
 // Persistent-kernel work: up to kMaxClasses subgrid classes (one subgrid size each) share one launch; the queue
@@ -91,6 +97,9 @@ int degrid_visibilities_per_item(int shape);             // visibilities one deg
 hipError_t launch_grid_mixed_queue(const QueueArgs &q, int precision, int shape, int workgroups, hipStream_t st);
 hipError_t launch_degrid_mixed_queue(const QueueArgs &q, int precision, int shape, int workgroups, hipStream_t st);
 hipError_t launch_reset_counter(int *counter, hipStream_t st);
+// Degridder pixel preparation for a.prepared / a.prepared_scale / a.slot_n (writes through those pointers).
+size_t degrid_prepared_bytes(int S, long long nr_subgrids); // bytes of a.prepared
+hipError_t launch_degrid_prepare(const KernelArgs &a, int precision, hipStream_t st);
 int queue_workgroups(int precision, bool gridder, int shape); // resident workgroups for a persistent launch
 
 bool subgrid_fft_supported(int S);

@@ -153,18 +153,27 @@ __device__ __forceinline__ f32x4 mfma(uint32_t a0, uint32_t a1, uint32_t a2, uin
 // Writes the B-operand words of one k-pair item (a visibility for the gridder, a pixel for the degridder) with
 // complex values v[0..3] into a 1 KiB LDS block laid out as [lane][4 words]: item q of the block occupies word
 // (q % 4) of lanes (q / 4) * 16 + n, n = column.  Columns 0-7 carry the high parts, 8-15 the low parts.
+// Packed (re, im) high and low parts of the four correlations (low parts zero without the split).
 template <int DT, bool SPLIT>
-__device__ __forceinline__ void write_b_item(uint32_t *blk, int q, const cfloat *v)
+__device__ __forceinline__ void split_item(const cfloat *v, uint32_t *hi, uint32_t *lo)
+{
+#pragma unroll
+    for (int pol = 0; pol < 4; ++pol)
+    {
+        split2<DT>(v[pol].re, v[pol].im, hi[pol], lo[pol]);
+        if (!SPLIT)
+            lo[pol] = 0;
+    }
+}
+
+__device__ __forceinline__ void write_b_words(uint32_t *blk, int q, const uint32_t *hip, const uint32_t *lop)
 {
     const int kg = q >> 2, slot = q & 3;
     uint32_t *base = blk + kg * 64 + slot;
 #pragma unroll
     for (int pol = 0; pol < 4; ++pol)
     {
-        uint32_t hi, lo;
-        split2<DT>(v[pol].re, v[pol].im, hi, lo);
-        if (!SPLIT)
-            lo = 0;
+        const uint32_t hi = hip[pol], lo = lop[pol];
         // hi = (re | im << 16).  Column re: (cos row re, sin row -im); column im: (cos row im, sin row re).
         const uint32_t re_col_hi = (hi & 0xffffu) | ((hi ^ 0x80000000u) & 0xffff0000u);
         const uint32_t im_col_hi = (hi >> 16) | (hi << 16);
@@ -172,9 +181,16 @@ __device__ __forceinline__ void write_b_item(uint32_t *blk, int q, const cfloat 
         const uint32_t im_col_lo = (lo >> 16) | (lo << 16);
         base[(2 * pol) * 4] = re_col_hi;
         base[(2 * pol + 1) * 4] = im_col_hi;
-        base[(8 + 2 * pol) * 4] = SPLIT ? re_col_lo : 0u;
-        base[(8 + 2 * pol + 1) * 4] = SPLIT ? im_col_lo : 0u;
+        base[(8 + 2 * pol) * 4] = re_col_lo;
+        base[(8 + 2 * pol + 1) * 4] = im_col_lo;
     }
+}
+
+template <int DT, bool SPLIT> __device__ __forceinline__ void write_b_item(uint32_t *blk, int q, const cfloat *v)
+{
+    uint32_t hi[4], lo[4];
+    split_item<DT, SPLIT>(v, hi, lo);
+    write_b_words(blk, q, hi, lo);
 }
 
 __device__ __forceinline__ void channel_window(const KernelArgs &a, int s, int &c0, int &nc)
@@ -310,6 +326,29 @@ __device__ __forceinline__ float block_max(float v, unsigned &slot)
     const float m = __uint_as_float(slot);
     __syncthreads();
     return m;
+}
+
+// Degridder input pixel p: taper x (A1 P A2^H, or A1^H P A2 in the reference convention) of the subgrid values.
+__device__ __forceinline__ void degrid_pixel(const KernelArgs &a, const idgx_metadata &m, const float2 *sub, int p,
+                                             cfloat *px)
+{
+    const int P = a.p.S * a.p.S;
+    const float taper = a.spheroidal[p];
+#pragma unroll
+    for (int pol = 0; pol < 4; ++pol)
+    {
+        const float2 g = sub[pol * P + p];
+        px[pol] = {taper * g.x, taper * g.y};
+    }
+    const float4 *aterms = reinterpret_cast<const float4 *>(a.aterms);
+    const size_t abase = static_cast<size_t>(m.aterm_index) * a.p.nr_stations;
+    cfloat a1[4], a2[4];
+    load_jones(aterms, (abase + m.station1) * P + p, a1);
+    load_jones(aterms, (abase + m.station2) * P + p, a2);
+    if (a.p.convention == IDGX_ATERM_REFERENCE)
+        apply_a1h_p_a2(px, a1, a2);
+    else
+        apply_a1_p_a2h(px, a1, a2);
 }
 
 template <int PT, int VB> struct GridderShared
@@ -583,8 +622,6 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
     const CentreOffsets off = centre_offsets(a.p, m);
     const double delta = static_cast<double>(a.p.image_size) / S;
     const float2 *sub = reinterpret_cast<const float2 *>(a.subgrids) + static_cast<size_t>(s) * kCorr * P;
-    const float4 *aterms = reinterpret_cast<const float4 *>(a.aterms);
-    const size_t abase = static_cast<size_t>(m.aterm_index) * a.p.nr_stations;
     uint32_t *s_bw = reinterpret_cast<uint32_t *>(sm.b);
 
     // Reduced phase coefficients of visibility lane%16 of each tile; for PM only component lane/16 of them (the B
@@ -613,6 +650,8 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
     for (int t = 0; t < VT; ++t)
         acc[t] = (f32x4){0.F, 0.F, 0.F, 0.F};
     int scale_exp = -126; // FP16: current power-of-two scale of the staged B operand
+    if (DT == kFP16 && a.prepared)
+        scale_exp = a.prepared_scale[s];
 
     // Pixel slots: rows of Sp = S rounded up to a multiple of 4, so each lane group's 4 slots share a row.
     const int Sp = degrid_row_stride(S), nslots = S * Sp;
@@ -621,86 +660,116 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
         const int np = min(PB, nslots - pb);
         __syncthreads();
         constexpr int kPer = (PB + kThreads - 1) / kThreads;
-        cfloat px[kPer][4];
-        float cx[kPer], cy[kPer], cn[kPer];
-        float vmax = 0.F;
-#pragma unroll
-        for (int i = 0; i < kPer; ++i)
+        if (a.prepared)
         {
-            const int q = tid + i * kThreads;
-#pragma unroll
-            for (int pol = 0; pol < 4; ++pol)
-                px[i][pol] = {0.F, 0.F};
-            cx[i] = cy[i] = cn[i] = 0.F;
-            const int sy = (pb + q) / Sp, sx = pb + q - sy * Sp;
-            if (q < np)
-            {
-                cx[i] = pixel_offset(sx, S);
-                cy[i] = pixel_offset(sy, S);
-            }
-            if (q < np && sx < S)
-            {
-                const int p = sy * S + sx;
-                const float taper = a.spheroidal[p];
-#pragma unroll
-                for (int pol = 0; pol < 4; ++pol)
-                {
-                    const float2 g = sub[pol * P + p];
-                    px[i][pol] = {taper * g.x, taper * g.y};
-                }
-                cfloat a1[4], a2[4];
-                load_jones(aterms, (abase + m.station1) * P + p, a1);
-                load_jones(aterms, (abase + m.station2) * P + p, a2);
-                if (a.p.convention == IDGX_ATERM_REFERENCE)
-                    apply_a1h_p_a2(px[i], a1, a2);
-                else
-                    apply_a1_p_a2h(px[i], a1, a2);
-                cn[i] = pixel_n_f(pixel_lm_f(sx, S, a.p.image_size), pixel_lm_f(sy, S, a.p.image_size));
-#pragma unroll
-                for (int pol = 0; pol < 4; ++pol)
-                    vmax = fmaxf(vmax, fmaxf(fabsf(px[i][pol].re), fabsf(px[i][pol].im)));
-            }
-        }
-        if constexpr (DT == kFP16)
-        {
-            const int e_new = max(scale_exp, scale_exponent(block_max(vmax, sm.maxbits)));
-            if (e_new > scale_exp)
-            {
-                const float f = ldexpf(1.F, scale_exp - e_new);
-#pragma unroll
-                for (int t = 0; t < VT; ++t)
-                    acc[t] *= f;
-                scale_exp = e_new;
-            }
-            const float inv = ldexpf(1.F, -scale_exp);
+            // Prepared pixels: copy the split words and the slot coordinates into LDS.
+            const uint4 *src = a.prepared + (static_cast<size_t>(s) * nslots + pb) * 2;
 #pragma unroll
             for (int i = 0; i < kPer; ++i)
+            {
+                const int q = tid + i * kThreads;
+                if (q >= PB)
+                    break;
+                uint32_t hi[4] = {0u, 0u, 0u, 0u}, lo[4] = {0u, 0u, 0u, 0u};
+                float cx = 0.F, cy = 0.F, cn = 0.F;
+                if (q < np)
+                {
+                    const uint4 w0 = src[2 * q], w1 = src[2 * q + 1];
+                    hi[0] = w0.x, lo[0] = w0.y, hi[1] = w0.z, lo[1] = w0.w;
+                    hi[2] = w1.x, lo[2] = w1.y, hi[3] = w1.z, lo[3] = w1.w;
+                    const int sy = (pb + q) / Sp;
+                    cx = pixel_offset(pb + q - sy * Sp, S);
+                    cy = pixel_offset(sy, S);
+                    cn = a.slot_n[pb + q];
+                }
+                const int blk = q >> 4, qq = q & 15;
+                write_b_words(s_bw + blk * 256, qq, hi, lo);
+                if constexpr (PM)
+                {
+                    float *r = sm.coord + blk * 64 + qq;
+                    r[0] = cx;
+                    r[16] = cy;
+                    r[32] = cn;
+                    r[48] = 0.F;
+                }
+                else
+                {
+                    float *r = sm.coord + blk * 64 + (qq >> 2) * 16 + (qq & 3);
+                    r[0] = cx;
+                    r[4] = cy;
+                    r[8] = cn;
+                }
+            }
+        }
+        else
+        {
+            cfloat px[kPer][4];
+            float cx[kPer], cy[kPer], cn[kPer];
+            float vmax = 0.F;
+#pragma unroll
+            for (int i = 0; i < kPer; ++i)
+            {
+                const int q = tid + i * kThreads;
 #pragma unroll
                 for (int pol = 0; pol < 4; ++pol)
-                    px[i][pol] = {px[i][pol].re * inv, px[i][pol].im * inv};
-        }
+                    px[i][pol] = {0.F, 0.F};
+                cx[i] = cy[i] = cn[i] = 0.F;
+                const int sy = (pb + q) / Sp, sx = pb + q - sy * Sp;
+                if (q < np)
+                {
+                    cx[i] = pixel_offset(sx, S);
+                    cy[i] = pixel_offset(sy, S);
+                }
+                if (q < np && sx < S)
+                {
+                    degrid_pixel(a, m, sub, sy * S + sx, px[i]);
+                    cn[i] = pixel_n_f(pixel_lm_f(sx, S, a.p.image_size), pixel_lm_f(sy, S, a.p.image_size));
 #pragma unroll
-        for (int i = 0; i < kPer; ++i)
-        {
-            const int q = tid + i * kThreads;
-            if (q >= PB)
-                break;
-            const int blk = q >> 4, qq = q & 15;
-            write_b_item<DT, SPLIT>(s_bw + blk * 256, qq, px[i]);
-            if constexpr (PM)
-            {
-                float *r = sm.coord + blk * 64 + qq;
-                r[0] = cx[i];
-                r[16] = cy[i];
-                r[32] = cn[i];
-                r[48] = 0.F;
+                    for (int pol = 0; pol < 4; ++pol)
+                        vmax = fmaxf(vmax, fmaxf(fabsf(px[i][pol].re), fabsf(px[i][pol].im)));
+                }
             }
-            else
+            if constexpr (DT == kFP16)
             {
-                float *r = sm.coord + blk * 64 + (qq >> 2) * 16 + (qq & 3);
-                r[0] = cx[i];
-                r[4] = cy[i];
-                r[8] = cn[i];
+                const int e_new = max(scale_exp, scale_exponent(block_max(vmax, sm.maxbits)));
+                if (e_new > scale_exp)
+                {
+                    const float f = ldexpf(1.F, scale_exp - e_new);
+#pragma unroll
+                    for (int t = 0; t < VT; ++t)
+                        acc[t] *= f;
+                    scale_exp = e_new;
+                }
+                const float inv = ldexpf(1.F, -scale_exp);
+#pragma unroll
+                for (int i = 0; i < kPer; ++i)
+#pragma unroll
+                    for (int pol = 0; pol < 4; ++pol)
+                        px[i][pol] = {px[i][pol].re * inv, px[i][pol].im * inv};
+            }
+#pragma unroll
+            for (int i = 0; i < kPer; ++i)
+            {
+                const int q = tid + i * kThreads;
+                if (q >= PB)
+                    break;
+                const int blk = q >> 4, qq = q & 15;
+                write_b_item<DT, SPLIT>(s_bw + blk * 256, qq, px[i]);
+                if constexpr (PM)
+                {
+                    float *r = sm.coord + blk * 64 + qq;
+                    r[0] = cx[i];
+                    r[16] = cy[i];
+                    r[32] = cn[i];
+                    r[48] = 0.F;
+                }
+                else
+                {
+                    float *r = sm.coord + blk * 64 + (qq >> 2) * 16 + (qq & 3);
+                    r[0] = cx[i];
+                    r[4] = cy[i];
+                    r[8] = cn[i];
+                }
             }
         }
         __syncthreads();
@@ -850,6 +919,71 @@ __global__ __launch_bounds__(kThreads) void degridder_mfma_queue(QueueArgs q)
 __global__ void reset_counter_kernel(int *counter)
 {
     *counter = 0;
+}
+
+// One workgroup per subgrid: the taper x A-term-applied pixels of every slot, split into the packed words the
+// degridder stages (FP16: scaled by 2^-scale so that the largest value is about 2^14).  Workgroup 0 also writes
+// the slot n table.  Subgrids outside the arrays are skipped (the degridder skips them too).
+template <int DT, bool SPLIT> __global__ __launch_bounds__(kThreads) void degrid_prepare_kernel(KernelArgs a)
+{
+    __shared__ unsigned maxbits;
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const int S = a.p.S, P = S * S, Sp = degrid_row_stride(S), nslots = S * Sp;
+    float *slot_n = const_cast<float *>(a.slot_n);
+    if (s == 0)
+    {
+        for (int q = tid; q < nslots; q += kThreads)
+        {
+            const int sy = q / Sp, sx = q - sy * Sp;
+            slot_n[q] = sx < S ? pixel_n_f(pixel_lm_f(sx, S, a.p.image_size), pixel_lm_f(sy, S, a.p.image_size))
+                               : 0.F;
+        }
+    }
+    const idgx_metadata m = a.metadata[s];
+    int c0, nc;
+    channel_window(a, s, c0, nc);
+    const long long row0 = first_row(a.metadata, m);
+    if (!subgrid_in_bounds(a.p, m, row0, c0, nc) || nc == 0)
+        return; // uniform
+    const float2 *sub = reinterpret_cast<const float2 *>(a.subgrids) + static_cast<size_t>(s) * kCorr * P;
+    float inv = 1.F;
+    if constexpr (DT == kFP16)
+    {
+        float vmax = 0.F;
+        for (int q = tid; q < nslots; q += kThreads)
+        {
+            const int sy = q / Sp, sx = q - sy * Sp;
+            if (sx < S)
+            {
+                cfloat px[4];
+                degrid_pixel(a, m, sub, sy * S + sx, px);
+#pragma unroll
+                for (int pol = 0; pol < 4; ++pol)
+                    vmax = fmaxf(vmax, fmaxf(fabsf(px[pol].re), fabsf(px[pol].im)));
+            }
+        }
+        const int e = scale_exponent(block_max(vmax, maxbits));
+        if (tid == 0)
+            const_cast<int *>(a.prepared_scale)[s] = e;
+        inv = ldexpf(1.F, -e);
+    }
+    uint4 *dst = const_cast<uint4 *>(a.prepared) + static_cast<size_t>(s) * nslots * 2;
+    for (int q = tid; q < nslots; q += kThreads)
+    {
+        const int sy = q / Sp, sx = q - sy * Sp;
+        uint32_t hi[4] = {0u, 0u, 0u, 0u}, lo[4] = {0u, 0u, 0u, 0u};
+        if (sx < S)
+        {
+            cfloat px[4];
+            degrid_pixel(a, m, sub, sy * S + sx, px);
+#pragma unroll
+            for (int pol = 0; pol < 4; ++pol)
+                px[pol] = {px[pol].re * inv, px[pol].im * inv};
+            split_item<DT, SPLIT>(px, hi, lo);
+        }
+        dst[2 * q] = make_uint4(hi[0], lo[0], hi[1], lo[1]);
+        dst[2 * q + 1] = make_uint4(hi[2], lo[2], hi[3], lo[3]);
+    }
 }
 
 // ---- launchers ------------------------------------------------------------------------------------------------
@@ -1022,6 +1156,38 @@ hipError_t launch_grid_mixed_queue(const QueueArgs &q, int precision, int shape,
 hipError_t launch_degrid_mixed_queue(const QueueArgs &q, int precision, int shape, int workgroups, hipStream_t st)
 {
     return launch_queue(q, precision, false, shape, workgroups, st);
+}
+
+size_t degrid_prepared_bytes(int S, long long nr_subgrids)
+{
+    return static_cast<size_t>(nr_subgrids) * S * degrid_row_stride(S) * 2 * sizeof(uint4);
+}
+
+hipError_t launch_degrid_prepare(const KernelArgs &a, int precision, hipStream_t st)
+{
+    if (a.p.nr_subgrids <= 0)
+        return hipSuccess;
+    if (!a.prepared || !a.prepared_scale || !a.slot_n)
+        return hipErrorInvalidValue;
+    const dim3 grid(static_cast<unsigned>(a.p.nr_subgrids));
+    switch (precision)
+    {
+    case IDGX_MIXED_BF16X3:
+        degrid_prepare_kernel<kBF16, true><<<grid, kThreads, 0, st>>>(a);
+        break;
+    case IDGX_MIXED_FP16X3:
+        degrid_prepare_kernel<kFP16, true><<<grid, kThreads, 0, st>>>(a);
+        break;
+    case IDGX_MIXED_BF16:
+        degrid_prepare_kernel<kBF16, false><<<grid, kThreads, 0, st>>>(a);
+        break;
+    case IDGX_MIXED_FP16:
+        degrid_prepare_kernel<kFP16, false><<<grid, kThreads, 0, st>>>(a);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_reset_counter(int *counter, hipStream_t st)

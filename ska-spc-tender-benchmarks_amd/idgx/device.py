@@ -127,7 +127,7 @@ class DeviceBatch:
     work-queue kernels).  `classes` are workload dicts sharing uvw / wavenumbers / visibilities rows; each may
     carry 'vis_index' / 'vis_offset' selection lists."""
 
-    def __init__(self, classes, precision="bf16x3", device="cuda", visibilities_on_device=None):
+    def __init__(self, classes, precision="bf16x3", device="cuda", visibilities_on_device=None, aterm_convention=0):
         from . import Batch
 
         torch = _torch()
@@ -154,7 +154,7 @@ class DeviceBatch:
             plan = make_plan(nr_subgrids=n, subgrid_size=S, grid_size=w["grid_size"], nr_channels=self.C,
                              nr_stations=w["nr_stations"], nr_aterm_slots=w["nr_aterm_slots"], nr_rows=self.nr_rows,
                              max_nr_timesteps=w["max_nr_timesteps"], image_size=w["image_size"], w_step=w["w_step"],
-                             precision=precision)
+                             precision=precision, aterm_convention=aterm_convention)
             c["plan"] = plan
             self.classes.append(c)
             specs.append((plan, w["metadata"], w.get("channels"), w.get("vis_offset")))
