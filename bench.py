@@ -65,12 +65,38 @@ def cpu_baseline(w, every, nthreads):
     return dict(grid_s=t1 - t0, degrid_s=t2 - t1, nvis=nvis, nsub=idx.size)
 
 
+def kernel_key(precision, path, direction, shape=None):
+    """Name of the kernel instantiation that runs a pass, as rocprofv3 reports it (the rows of the PMC summary)."""
+    if precision == "fp32":
+        return f"{direction}_plain<4, 1>"
+    dt = 0 if precision.startswith("bf16") else 1
+    split = "true" if precision.endswith("x3") else "false"
+    if path == "one-shot":
+        return f"{direction}_mfma<{dt}, {split}, 4, 256, false>"
+    tiles, batch, pm = shape
+    return f"{direction}_mfma_queue<{dt}, {split}, {tiles}, {batch}, {'true' if pm else 'false'}>"
+
+
+def measured_traffic(config, key):
+    """HBM bytes per launch of kernel `key` from the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration) for this workload, or None when it was not profiled."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{config}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f).get("kernels", {}).get(key)
+    return None if k is None else {"bytes": k["traffic_bytes"], "read_bytes": k["read_bytes"],
+                                   "write_bytes": k["write_bytes"], "kernel": key,
+                                   "source": os.path.relpath(path, ROOT)}
+
+
 class OneShot:
     """One launch per direction over the whole plan (idgx_grid / idgx_degrid)."""
 
     def __init__(self, dw, precision):
         self.dw, self.precision, self.path = dw, precision, "one-shot"
         self.plan = dw.plan(precision=precision)
+        self.shapes = {"gridder": None, "degridder": None}
 
     def grid(self, stream):
         self.dw.grid(self.plan, stream=stream)
@@ -94,6 +120,7 @@ class Batched:
         tg, td = self.db.batch.autotune(self.db.data, repeats=2, stream=stream)
         gs, ds = self.db.batch.shape()
         shapes = Batch.shapes()
+        self.shapes = {"gridder": shapes[gs], "degridder": shapes[ds]}
         items = self.db.batch.items()
         self.info = {"gridder_shape": {"tiles_per_wave": shapes[gs][0], "lds_visibilities": shapes[gs][1],
                                        "phase_mfma": shapes[gs][2]},
@@ -244,6 +271,9 @@ def main():
     achieved = flops / t_dom / 1e12
     bytes_dom = algorithmic_bytes(md, ch, S, C, int(w["nr_aterm_slots"]), int(w["nr_stations"]), dominant)
     mfma_flops = 0 if head["precision"] == "fp32" else 128 * sincos * (1 if head["precision"].endswith("x3") else 0.5)
+    head_runner = runners[-1]
+    dom_key = kernel_key(head["precision"], head["path"], dominant, head_runner.shapes[dominant])
+    traffic = measured_traffic(args.config, dom_key)
 
     cpu = None
     if not args.no_cpu:
@@ -293,12 +323,14 @@ def main():
         "accuracy_sample": f"every {check_every}th subgrid ({check_idx.size} subgrids) vs the FP64 oracle",
         "roofline": {
             "bound": "mfma",
-            "kernel": dominant,
+            "kernel": dom_key,
+            "pass": dominant,
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": None if traffic is None else traffic["bytes"],
+            "traffic_detail": traffic,
             "flop_model": "P*(5*T + 34*V) + 134*P per subgrid (FP32-equivalent algorithmic flops), sincos separate",
             "mfma_utilisation": (mfma_flops / t_dom / 1e12) / 2500.0 if mfma_flops else None,
             "algorithmic_bytes": bytes_dom,

@@ -15,7 +15,7 @@ import json
 import os
 import sys
 
-KERNELS = ["gridder_plain", "degridder_plain", "gridder_mfma_queue", "degridder_mfma_queue", "gridder_mfma",
+KERNELS = ["gridder_plain", "degridder_plain", "gridder_mfma_queue", "degridder_mfma_queue", "degrid_prepare_kernel", "gridder_mfma",
            "degridder_mfma", "subgrid_fft_kernel", "adder_kernel", "splitter_kernel"]
 
 
