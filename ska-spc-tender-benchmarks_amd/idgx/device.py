@@ -104,28 +104,50 @@ class DeviceWorkload:
         return download(d_grid, np.complex64, (4, G, G))
 
 
-def selection_lists(w, flags):
+def selection_lists(w, flags, chunk_entries=1 << 26):
     """Visibility selection of every subgrid of workload w from a boolean flag mask [rows][C] (True = flagged):
-    returns (vis_index uint32 [(t << 16) | c], vis_offset int64 [n + 1]) relative to each subgrid's window."""
+    returns (vis_index uint32 [(t << 16) | c], vis_offset int64 [n + 1]) relative to each subgrid's window, entries
+    of a subgrid in (t, c) row-major order.  Vectorised over chunks of subgrids of about chunk_entries window
+    entries each."""
     md, ch = w["metadata"], w.get("channels")
     C = w["wavenumbers"].shape[0]
-    index, offset = [], [0]
-    for s in range(md.size):
-        r0 = int(md["time_offset"][s]) + int(md["baseline_offset"][s]) - int(md["baseline_offset"][0])
-        T = int(md["nr_timesteps"][s])
-        c0, nc = (0, C) if ch is None else (int(ch["channel_begin"][s]), int(ch["nr_channels"][s]))
-        keep = ~flags[r0:r0 + T, c0:c0 + nc]
-        t, c = np.nonzero(keep)
-        index.append(((t.astype(np.uint32) << 16) | c.astype(np.uint32)))
-        offset.append(offset[-1] + t.size)
+    n = md.size
+    r0 = md["time_offset"].astype(np.int64) + md["baseline_offset"].astype(np.int64) - \
+        (int(md["baseline_offset"][0]) if n else 0)
+    T = md["nr_timesteps"].astype(np.int64)
+    c0 = np.zeros(n, np.int64) if ch is None else ch["channel_begin"].astype(np.int64)
+    nc = np.full(n, C, np.int64) if ch is None else ch["nr_channels"].astype(np.int64)
+    window = T * nc
+    flat = flags.reshape(-1)
+    index, counts = [], np.zeros(n, np.int64)
+    s0 = 0
+    cum = np.cumsum(window)
+    while s0 < n:
+        base = cum[s0 - 1] if s0 else 0
+        s1 = max(s0 + 1, int(np.searchsorted(cum, base + chunk_entries, side="right")))
+        s1 = min(s1, n)
+        sub = np.arange(s0, s1)
+        # one entry per (subgrid, t): its subgrid, t, and channel count
+        st_s = np.repeat(sub, T[sub])
+        st_t = np.arange(st_s.size) - np.repeat(np.cumsum(T[sub]) - T[sub], T[sub])
+        # one entry per (subgrid, t, c)
+        e_st = np.repeat(np.arange(st_s.size), nc[st_s])
+        e_c = np.arange(e_st.size) - np.repeat(np.cumsum(nc[st_s]) - nc[st_s], nc[st_s])
+        e_s, e_t = st_s[e_st], st_t[e_st]
+        keep = ~flat[(r0[e_s] + e_t) * C + c0[e_s] + e_c]
+        index.append(((e_t[keep].astype(np.uint32) << 16) | e_c[keep].astype(np.uint32)))
+        counts[s0:s1] = np.bincount(e_s[keep] - s0, minlength=s1 - s0)
+        s0 = s1
     vi = np.concatenate(index) if index else np.zeros(0, np.uint32)
-    return np.ascontiguousarray(vi.astype(np.uint32)), np.asarray(offset, np.int64)
+    offset = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    return np.ascontiguousarray(vi.astype(np.uint32)), offset
 
 
 class DeviceBatch:
     """One or more subgrid classes of an observation resident in HBM, run through the batch API (persistent
     work-queue kernels).  `classes` are workload dicts sharing uvw / wavenumbers / visibilities rows; each may
-    carry 'vis_index' / 'vis_offset' selection lists."""
+    carry 'vis_index' / 'vis_offset' selection lists (host 'vis_offset' is always needed; 'd_vis_index' /
+    'd_vis_offset' device tensors, when present, are used instead of uploading the host lists)."""
 
     def __init__(self, classes, precision="bf16x3", device="cuda", visibilities_on_device=None, aterm_convention=0):
         from . import Batch
@@ -147,8 +169,10 @@ class DeviceBatch:
             c = dict(w=w, S=S, n=n,
                      d_metadata=upload(w["metadata"], device),
                      d_channels=None if w.get("channels") is None else upload(w["channels"], device),
-                     d_vis_index=None if w.get("vis_index") is None else upload(w["vis_index"], device),
-                     d_vis_offset=None if w.get("vis_offset") is None else upload(w["vis_offset"], device),
+                     d_vis_index=w["d_vis_index"] if w.get("d_vis_index") is not None else
+                     None if w.get("vis_index") is None else upload(w["vis_index"], device),
+                     d_vis_offset=w["d_vis_offset"] if w.get("d_vis_offset") is not None else
+                     None if w.get("vis_offset") is None else upload(w["vis_offset"], device),
                      d_spheroidal=upload(w["spheroidal"], device), d_aterms=upload(w["aterms"], device),
                      d_subgrids=torch.zeros(max(n, 1) * 4 * S * S * 8, dtype=torch.uint8, device=device))
             plan = make_plan(nr_subgrids=n, subgrid_size=S, grid_size=w["grid_size"], nr_channels=self.C,

@@ -84,75 +84,138 @@ def identity_aterms(n_slots, n_stations, S):
     return a
 
 
-def plan_subgrids(uvw_m, frequencies, s1, s2, *, subgrid_size, grid_size, image_size, timesteps_per_subgrid,
+def jones_aterms(rng, n_slots, n_stations, S, sigma=0.1):
+    """Direction-dependent Jones matrices [slot][station][y][x][xx, xy, yx, yy]: J = I + E with E a smooth
+    (quadratic in the pixel coordinates) complex perturbation whose entries have standard deviation ~sigma, each
+    pixel's matrix renormalised to |det J| = 1."""
+    t = (np.arange(S) + 0.5) / (S / 2.0) - 1.0  # pixel centres in [-1, 1]
+    yy, xx = np.meshgrid(t, t, indexing="ij")
+    basis = np.stack([np.ones_like(xx), xx, yy, xx * yy, xx * xx - 1 / 3, yy * yy - 1 / 3])  # [6][S][S]
+    nb = basis.shape[0]
+    coef = (rng.standard_normal((n_slots, n_stations, 4, nb, 2)) @ np.array([1.0, 1j])) * (sigma / np.sqrt(nb))
+    e = np.einsum("abkn,nyx->abyxk", coef, basis)
+    j = e.copy()
+    j[..., 0] += 1.0
+    j[..., 3] += 1.0
+    det = j[..., 0] * j[..., 3] - j[..., 1] * j[..., 2]
+    j /= np.sqrt(np.abs(det))[..., None]
+    return np.ascontiguousarray(j.astype(np.complex64))
+
+
+def time_segments(n_baselines, n_times, timesteps_per_subgrid, aterm_interval=None):
+    """Fixed time chunks for every baseline, cut at A-term slot boundaries too: (baseline, t0, t1) arrays."""
+    bounds = set(range(0, n_times, timesteps_per_subgrid))
+    if aterm_interval:
+        bounds |= set(range(0, n_times, aterm_interval))
+    bounds = sorted(bounds) + [n_times]
+    t0 = np.repeat(np.asarray(bounds[:-1], np.int64)[:, None], n_baselines, axis=1)
+    t1 = np.repeat(np.asarray(bounds[1:], np.int64)[:, None], n_baselines, axis=1)
+    bl = np.repeat(np.arange(n_baselines, dtype=np.int64)[None, :], len(bounds) - 1, axis=0)
+    return bl.ravel(), t0.ravel(), t1.ravel()
+
+
+def ragged_time_segments(rng, baselines, n_times, t_min, t_max, aterm_interval=None):
+    """Per-baseline time chunks with lengths log-uniform in [t_min, t_max] (the last one truncated at n_times), cut
+    at A-term slot boundaries too: (baseline, t0, t1) arrays, baseline-major."""
+    baselines = np.asarray(baselines, np.int64)
+    k = -(-n_times // t_min) + 1
+    lengths = np.rint(np.exp(rng.uniform(np.log(t_min), np.log(t_max), (baselines.size, k)))).astype(np.int64)
+    ends = np.minimum(np.cumsum(lengths, axis=1), n_times)
+    starts = np.concatenate([np.zeros((baselines.size, 1), np.int64), ends[:, :-1]], axis=1)
+    valid = ends > starts
+    bl = np.repeat(baselines[:, None], k, axis=1)[valid]
+    t0, t1 = starts[valid], ends[valid]
+    if aterm_interval:
+        cuts = np.arange(aterm_interval, n_times, aterm_interval)
+        pieces = []
+        for b_, a_, z_ in zip(bl, t0, t1):
+            edges = [a_] + [int(c) for c in cuts if a_ < c < z_] + [z_]
+            pieces += [(b_, e0, e1) for e0, e1 in zip(edges[:-1], edges[1:])]
+        bl, t0, t1 = (np.asarray(x, np.int64) for x in zip(*pieces))
+    return bl, t0, t1
+
+
+def plan_segments(uvw_m, frequencies, s1, s2, seg_bl, seg_t0, seg_t1, *, subgrid_size, grid_size, image_size,
                   support=8, aterm_interval=None):
-    """IDG plan: returns (metadata, channels) for rows laid out as row = baseline * n_times + t."""
+    """IDG plan over time segments: segment i covers rows seg_bl[i] * n_times + [seg_t0[i], seg_t1[i]) (rows laid
+    out as row = baseline * n_times + t); a segment must not cross an A-term slot boundary.  Its channels are
+    grouped greedily so that the uv footprint of every (timestep, channel) sample of a group spans at most
+    S - support grid cells in u and in v; each group becomes one subgrid centred on its footprint, dropped if it
+    would leave the grid.  Returns (metadata, channels) sorted by (first row, first channel)."""
     n_bl, n_times, _ = uvw_m.shape
     S, G = subgrid_size, grid_size
     width = S - support
     scale = frequencies.astype(np.float64) / SPEED_OF_LIGHT * image_size  # cells per metre, per channel
-    # Time chunks (cut at A-term slot boundaries too).
-    bounds = list(range(0, n_times, timesteps_per_subgrid))
-    if aterm_interval:
-        bounds = sorted(set(bounds) | set(range(0, n_times, aterm_interval)))
-    bounds.append(n_times)
-    meta_parts, chan_parts = [], []
-    for t0, t1 in zip(bounds[:-1], bounds[1:]):
-        seg = uvw_m[:, t0:t1, :]
-        umin, umax = seg[..., 0].min(1), seg[..., 0].max(1)
-        vmin, vmax = seg[..., 1].min(1), seg[..., 1].max(1)
-        start = np.zeros(n_bl, np.int64)
-        all_bl = np.arange(n_bl)
+    seg_bl, seg_t0, seg_t1 = (np.asarray(x, np.int64) for x in (seg_bl, seg_t0, seg_t1))
+    n_seg = seg_bl.size
+    # Per-segment extent of u and v (metres): reductions over [start, end) of the flattened rows.
+    start = seg_bl * n_times + seg_t0
+    end = seg_bl * n_times + seg_t1
+    bounds = np.stack([start, end], axis=1).ravel()
 
-        def footprint(b, a, z):
-            # grid-cell extent of the samples of baselines b with channels in [a, z]
-            fa, fz = scale[a], scale[z]
-            lo_u = np.minimum(umin[b] * fa, umin[b] * fz)
-            hi_u = np.maximum(umax[b] * fa, umax[b] * fz)
-            lo_v = np.minimum(vmin[b] * fa, vmin[b] * fz)
-            hi_v = np.maximum(vmax[b] * fa, vmax[b] * fz)
-            return lo_u, hi_u, lo_v, hi_v
+    def seg_reduce(op, comp):
+        a = np.append(uvw_m[..., comp].reshape(-1), 0.0)
+        return op.reduceat(a, bounds)[::2] if n_seg else np.zeros(0)
 
-        groups = []  # (baseline idx array, c_begin, c_end_inclusive)
-        C = len(frequencies)
-        for c in range(1, C + 1):
-            if c < C:
-                lo_u, hi_u, lo_v, hi_v = footprint(all_bl, start, np.full(n_bl, c))
-                fits = (hi_u - lo_u <= width) & (hi_v - lo_v <= width)
-                close = ~fits
-            else:
-                close = np.ones(n_bl, bool)
-            idx = np.nonzero(close)[0]
-            if idx.size:
-                groups.append((idx, start[idx].copy(), np.full(idx.size, c - 1)))
-                start[idx] = c
-        bl = np.concatenate([g[0] for g in groups])
-        ca = np.concatenate([g[1] for g in groups])
-        cb = np.concatenate([g[2] for g in groups])
-        lo_u, hi_u, lo_v, hi_v = footprint(bl, ca, cb)
-        x0 = np.floor((lo_u + hi_u) / 2 + G / 2 - S / 2 + 0.5).astype(np.int64)
-        y0 = np.floor((lo_v + hi_v) / 2 + G / 2 - S / 2 + 0.5).astype(np.int64)
-        keep = (x0 >= 0) & (x0 + S <= G) & (y0 >= 0) & (y0 + S <= G)
-        bl, ca, cb, x0, y0 = bl[keep], ca[keep], cb[keep], x0[keep], y0[keep]
-        md = np.zeros(bl.size, METADATA_DTYPE)
-        md["baseline_offset"] = 0
-        md["time_offset"] = bl * n_times + t0
-        md["nr_timesteps"] = t1 - t0
-        md["aterm_index"] = (t0 // aterm_interval) if aterm_interval else 0
-        md["station1"] = s1[bl]
-        md["station2"] = s2[bl]
-        md["x"] = x0
-        md["y"] = y0
-        md["z"] = 0
-        ch = np.zeros(bl.size, CHANNELS_DTYPE)
-        ch["channel_begin"] = ca
-        ch["nr_channels"] = cb - ca + 1
-        meta_parts.append(md)
-        chan_parts.append(ch)
-    md = np.concatenate(meta_parts)
-    ch = np.concatenate(chan_parts)
+    umin, umax = seg_reduce(np.minimum, 0), seg_reduce(np.maximum, 0)
+    vmin, vmax = seg_reduce(np.minimum, 1), seg_reduce(np.maximum, 1)
+
+    def footprint(i, a, z):
+        # grid-cell extent of the samples of segments i with channels in [a, z]
+        fa, fz = scale[a], scale[z]
+        lo_u = np.minimum(umin[i] * fa, umin[i] * fz)
+        hi_u = np.maximum(umax[i] * fa, umax[i] * fz)
+        lo_v = np.minimum(vmin[i] * fa, vmin[i] * fz)
+        hi_v = np.maximum(vmax[i] * fa, vmax[i] * fz)
+        return lo_u, hi_u, lo_v, hi_v
+
+    groups = []  # (segment idx array, c_begin, c_end_inclusive)
+    first = np.zeros(n_seg, np.int64)
+    every = np.arange(n_seg)
+    C = len(frequencies)
+    for c in range(1, C + 1):
+        if c < C:
+            lo_u, hi_u, lo_v, hi_v = footprint(every, first, np.full(n_seg, c))
+            close = ~((hi_u - lo_u <= width) & (hi_v - lo_v <= width))
+        else:
+            close = np.ones(n_seg, bool)
+        idx = np.nonzero(close)[0]
+        if idx.size:
+            groups.append((idx, first[idx].copy(), np.full(idx.size, c - 1)))
+            first[idx] = c
+    seg = np.concatenate([g[0] for g in groups]) if groups else np.zeros(0, np.int64)
+    ca = np.concatenate([g[1] for g in groups]) if groups else np.zeros(0, np.int64)
+    cb = np.concatenate([g[2] for g in groups]) if groups else np.zeros(0, np.int64)
+    lo_u, hi_u, lo_v, hi_v = footprint(seg, ca, cb)
+    x0 = np.floor((lo_u + hi_u) / 2 + G / 2 - S / 2 + 0.5).astype(np.int64)
+    y0 = np.floor((lo_v + hi_v) / 2 + G / 2 - S / 2 + 0.5).astype(np.int64)
+    keep = (x0 >= 0) & (x0 + S <= G) & (y0 >= 0) & (y0 + S <= G)
+    seg, ca, cb, x0, y0 = seg[keep], ca[keep], cb[keep], x0[keep], y0[keep]
+    bl, t0, t1 = seg_bl[seg], seg_t0[seg], seg_t1[seg]
+    md = np.zeros(seg.size, METADATA_DTYPE)
+    md["baseline_offset"] = 0
+    md["time_offset"] = bl * n_times + t0
+    md["nr_timesteps"] = t1 - t0
+    md["aterm_index"] = (t0 // aterm_interval) if aterm_interval else 0
+    md["station1"] = s1[bl]
+    md["station2"] = s2[bl]
+    md["x"] = x0
+    md["y"] = y0
+    md["z"] = 0
+    ch = np.zeros(seg.size, CHANNELS_DTYPE)
+    ch["channel_begin"] = ca
+    ch["nr_channels"] = cb - ca + 1
     order = np.lexsort((ch["channel_begin"], md["time_offset"]))
     return np.ascontiguousarray(md[order]), np.ascontiguousarray(ch[order])
+
+
+def plan_subgrids(uvw_m, frequencies, s1, s2, *, subgrid_size, grid_size, image_size, timesteps_per_subgrid,
+                  support=8, aterm_interval=None):
+    """IDG plan with the same time chunks for every baseline: returns (metadata, channels) for rows laid out as
+    row = baseline * n_times + t."""
+    bl, t0, t1 = time_segments(uvw_m.shape[0], uvw_m.shape[1], timesteps_per_subgrid, aterm_interval)
+    return plan_segments(uvw_m, frequencies, s1, s2, bl, t0, t1, subgrid_size=subgrid_size, grid_size=grid_size,
+                         image_size=image_size, support=support, aterm_interval=aterm_interval)
 
 
 # name -> parameters of the seeded synthetic configurations
@@ -166,6 +229,26 @@ CONFIGS = {
     "ska_low": dict(n_stations=128, radius_m=20000.0, core_fraction=0.5, core_radius_m=1000.0, n_times=256,
                     f_lo=150e6, f_hi=180e6, n_channels=64, subgrid_size=32, timesteps_per_subgrid=128,
                     grid_size=4096, image_size=0.1, dec_deg=-30.0, zero_w=False, seed=2),
+    # Full imaging cycle: the small layout with the w-term, S=64 on a 16384^2 grid, Jones A-terms per station that
+    # change every 64 timesteps (the planner cuts subgrids at the A-term boundaries).
+    "full_cycle": dict(n_stations=10, radius_m=1000.0, core_fraction=0.0, core_radius_m=0.0, n_times=128,
+                       f_lo=150e6, f_hi=165e6, n_channels=16, subgrid_size=64, timesteps_per_subgrid=128,
+                       grid_size=16384, image_size=0.1, dec_deg=-30.0, zero_w=False, aterm_interval=64,
+                       jones=True, seed=6),
+}
+
+# Batched uneven workloads (make_batched_observation): baselines split into length terciles with subgrid sizes
+# 24 / 32 / 48, per-baseline time chunks log-uniform in t_range, per-baseline flag fraction U(0, flag_max).
+BATCHED_CONFIGS = {
+    # 256 stations (half in a 1 km core, the rest in a 20 km disk), 512 x 1 s, 128 ch 150-180 MHz, G = 8192.
+    "batched_uneven": dict(n_stations=256, radius_m=20000.0, core_fraction=0.5, core_radius_m=1000.0, n_times=512,
+                           f_lo=150e6, f_hi=180e6, n_channels=128, subgrid_sizes=(24, 32, 48), t_range=(16, 256),
+                           grid_size=8192, image_size=0.1, dec_deg=-30.0, zero_w=False, flag_max=0.6, seed=4),
+    # The same construction at a size the oracle checks in seconds.
+    "batched_uneven_small": dict(n_stations=12, radius_m=3000.0, core_fraction=0.5, core_radius_m=500.0,
+                                 n_times=96, f_lo=150e6, f_hi=180e6, n_channels=16, subgrid_sizes=(24, 32, 48),
+                                 t_range=(8, 64), grid_size=2048, image_size=0.1, dec_deg=-30.0, zero_w=False,
+                                 flag_max=0.6, seed=5),
 }
 
 
@@ -189,20 +272,23 @@ def make_observation(name=None, *, visibilities=True, **overrides):
                            image_size=cfg["image_size"], timesteps_per_subgrid=cfg["timesteps_per_subgrid"],
                            aterm_interval=cfg.get("aterm_interval"))
     n_rows = len(s1) * cfg["n_times"]
+    n_slots = -(-cfg["n_times"] // cfg["aterm_interval"]) if cfg.get("aterm_interval") else 1
+    aterms = jones_aterms(np.random.default_rng(cfg["seed"] + 7), n_slots, cfg["n_stations"], S) \
+        if cfg.get("jones") else identity_aterms(n_slots, cfg["n_stations"], S)
     w = dict(
         config=cfg,
         uvw=np.ascontiguousarray(uvw32.reshape(n_rows, 3)),
         frequencies=freqs.astype(np.float32),
         wavenumbers=wavenumbers,
         spheroidal=prolate_spheroidal_taper(S),
-        aterms=identity_aterms(1, cfg["n_stations"], S),
+        aterms=aterms,
         metadata=md,
         channels=ch,
         nr_subgrids=md.size,
         subgrid_size=S,
         grid_size=cfg["grid_size"],
         nr_stations=cfg["n_stations"],
-        nr_aterm_slots=1,
+        nr_aterm_slots=n_slots,
         image_size=np.float32(cfg["image_size"]),
         w_step=0.0,
         max_nr_timesteps=int(md["nr_timesteps"].max()) if md.size else 0,
@@ -212,6 +298,70 @@ def make_observation(name=None, *, visibilities=True, **overrides):
     if visibilities:
         w["visibilities"] = random_visibilities(n_rows, cfg["n_channels"], cfg["seed"])
     return w
+
+
+def baseline_flags(rng_fraction, rng_flags, n_baselines, n_times, n_channels, flag_max, rows_per_chunk=1 << 16):
+    """Boolean flags [rows][C] (True = flagged): baseline b flags each (timestep, channel) sample with
+    probability f_b ~ U(0, flag_max).  Generated in row chunks (float32 uniforms) to bound the memory."""
+    frac = rng_fraction.uniform(0.0, flag_max, n_baselines).astype(np.float32)
+    n_rows = n_baselines * n_times
+    flags = np.empty((n_rows, n_channels), bool)
+    for r0 in range(0, n_rows, rows_per_chunk):
+        r1 = min(n_rows, r0 + rows_per_chunk)
+        f = frac[np.arange(r0, r1) // n_times][:, None]
+        flags[r0:r1] = rng_flags.random((r1 - r0, n_channels), dtype=np.float32) < f
+    return flags
+
+
+def make_batched_observation(name=None, *, visibilities=True, flags=True, **overrides):
+    """A batched uneven workload: one subgrid class per baseline-length tercile (subgrid sizes 24 / 32 / 48 by
+    default), ragged per-baseline time chunks, per-baseline flagging expressed as visibility selection lists.
+
+    Returns dict(config, classes=[workload dicts sharing uvw / wavenumbers / visibilities rows, each with its
+    'vis_index' / 'vis_offset'], flags, nr_rows, nr_visibilities = selected visibilities)."""
+    from .device import selection_lists
+
+    cfg = dict(BATCHED_CONFIGS[name]) if name else {}
+    cfg.update(overrides)
+    rng = np.random.default_rng(cfg["seed"])
+    enu = station_layout(rng, cfg["n_stations"], cfg["radius_m"], cfg["core_fraction"], cfg["core_radius_m"])
+    s1, s2 = baseline_pairs(cfg["n_stations"])
+    n_bl, n_times, C = len(s1), cfg["n_times"], cfg["n_channels"]
+    uvw = earth_rotation_uvw(enu, s1, s2, n_times, cfg["dec_deg"])
+    if cfg.get("zero_w"):
+        uvw[..., 2] = 0.0
+    uvw32 = uvw.astype(np.float32)
+    del uvw
+    uvw64 = uvw32.astype(np.float64)
+    freqs = np.linspace(cfg["f_lo"], cfg["f_hi"], C)
+    wavenumbers = (2 * np.pi * freqs / SPEED_OF_LIGHT).astype(np.float32)
+    n_rows = n_bl * n_times
+    shared = dict(uvw=np.ascontiguousarray(uvw32.reshape(n_rows, 3)), frequencies=freqs.astype(np.float32),
+                  wavenumbers=wavenumbers, nr_rows=n_rows, grid_size=cfg["grid_size"],
+                  nr_stations=cfg["n_stations"], image_size=np.float32(cfg["image_size"]), w_step=0.0)
+    if visibilities:
+        shared["visibilities"] = random_visibilities(n_rows, C, cfg["seed"])
+    fl = baseline_flags(np.random.default_rng(cfg["seed"] + 2), np.random.default_rng(cfg["seed"] + 3), n_bl, n_times,
+                        C, cfg["flag_max"]) if flags else None
+    length = np.linalg.norm(enu[s2] - enu[s1], axis=1)
+    terciles = np.array_split(np.argsort(length, kind="stable"), len(cfg["subgrid_sizes"]))
+    seg_rng = np.random.default_rng(cfg["seed"] + 1)
+    classes = []
+    for k, S in enumerate(cfg["subgrid_sizes"]):
+        bl, t0, t1 = ragged_time_segments(seg_rng, np.sort(terciles[k]), n_times, *cfg["t_range"])
+        md, ch = plan_segments(uvw64, freqs, s1, s2, bl, t0, t1, subgrid_size=S, grid_size=cfg["grid_size"],
+                               image_size=cfg["image_size"])
+        c = dict(shared, config=cfg, metadata=md, channels=ch, nr_subgrids=md.size, subgrid_size=S,
+                 spheroidal=prolate_spheroidal_taper(S), aterms=identity_aterms(1, cfg["n_stations"], S),
+                 nr_aterm_slots=1, max_nr_timesteps=int(md["nr_timesteps"].max()) if md.size else 0)
+        if fl is not None:
+            c["vis_index"], c["vis_offset"] = selection_lists(c, fl)
+            c["nr_visibilities"] = int(c["vis_offset"][-1])
+        else:
+            c["nr_visibilities"] = int((md["nr_timesteps"].astype(np.int64) * ch["nr_channels"]).sum())
+        classes.append(c)
+    return dict(shared, config=cfg, classes=classes, flags=fl,
+                nr_visibilities=sum(c["nr_visibilities"] for c in classes))
 
 
 def random_visibilities(n_rows, n_channels, seed):

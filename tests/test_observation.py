@@ -1,0 +1,98 @@
+"""Workload generators and the planner (host only): time segments, the batched uneven configuration, selection
+lists, Jones A-terms and the A-term-aware full-cycle plan."""
+import numpy as np
+
+from idgx import observation
+from idgx.device import selection_lists
+
+
+def _selection_by_loop(w, flags):
+    """Straightforward per-subgrid restatement of the selection lists (np.nonzero over each window)."""
+    md, ch = w["metadata"], w["channels"]
+    index, offset = [np.zeros(0, np.uint32)], [0]
+    for s in range(md.size):
+        r0, T = int(md["time_offset"][s]), int(md["nr_timesteps"][s])
+        c0, nc = int(ch["channel_begin"][s]), int(ch["nr_channels"][s])
+        t, c = np.nonzero(~flags[r0:r0 + T, c0:c0 + nc])
+        index.append((t.astype(np.uint32) << 16) | c.astype(np.uint32))
+        offset.append(offset[-1] + t.size)
+    return np.concatenate(index), np.asarray(offset, np.int64)
+
+
+def test_fixed_time_segments_tile_every_baseline_and_respect_aterm_slots():
+    bl, t0, t1 = observation.time_segments(5, 300, 128, aterm_interval=64)
+    for b in range(5):
+        sel = bl == b
+        edges = np.concatenate([t0[sel], t1[sel][-1:]])
+        assert list(edges) == [0, 64, 128, 192, 256, 300]
+    assert ((t0 // 64) == ((t1 - 1) // 64)).all()
+
+
+def test_ragged_segments_are_log_uniform_chunks_that_tile_the_observation():
+    rng = np.random.default_rng(3)
+    bl, t0, t1 = observation.ragged_time_segments(rng, np.arange(400), 512, 16, 256, aterm_interval=100)
+    L = t1 - t0
+    assert (L >= 1).all() and (L <= 256).all()
+    for b in (0, 17, 399):
+        sel = bl == b
+        assert t0[sel][0] == 0 and t1[sel][-1] == 512 and (t0[sel][1:] == t1[sel][:-1]).all()
+    assert ((t0 // 100) == ((t1 - 1) // 100)).all()
+    # without cuts, the untruncated chunk lengths are log-uniform in [16, 256]: log2 is uniform in [4, 8]
+    bl, t0, t1 = observation.ragged_time_segments(np.random.default_rng(5), np.arange(200), 8192, 16, 256)
+    lg = np.log2((t1 - t0)[t1 < 8192])
+    assert lg.min() >= 4 - 0.05 and lg.max() <= 8 + 0.01
+    assert abs(np.median(lg) - 6) < 0.1 and abs(np.mean(lg < 5) - 0.25) < 0.03
+
+
+def test_selection_lists_match_the_per_subgrid_restatement():
+    w = observation.make_observation("small", visibilities=False)
+    flags = np.random.default_rng(4).random((w["nr_rows"], w["wavenumbers"].shape[0])) < 0.35
+    for chunk in (1 << 26, 3000):
+        vi, vo = selection_lists(w, flags, chunk_entries=chunk)
+        ri, ro = _selection_by_loop(w, flags)
+        assert np.array_equal(vi, ri) and np.array_equal(vo, ro)
+
+
+def test_batched_uneven_small_structure():
+    bo = observation.make_batched_observation("batched_uneven_small")
+    cfg = bo["config"]
+    C = bo["wavenumbers"].shape[0]
+    assert [c["subgrid_size"] for c in bo["classes"]] == [24, 32, 48]
+    # flags: per-baseline fractions in [0, 0.6): the overall fraction is near 0.3
+    assert 0.2 < bo["flags"].mean() < 0.4
+    cover = np.zeros((bo["nr_rows"], C), np.int8)
+    stations = set()
+    for c in bo["classes"]:
+        md, ch, S = c["metadata"], c["channels"], c["subgrid_size"]
+        assert (md["x"] >= 0).all() and (md["x"] + S <= cfg["grid_size"]).all()
+        assert (md["nr_timesteps"] <= cfg["t_range"][1]).all()
+        for s in range(md.size):
+            r0, nt = int(md["time_offset"][s]), int(md["nr_timesteps"][s])
+            c0, nc = int(ch["channel_begin"][s]), int(ch["nr_channels"][s])
+            cover[r0:r0 + nt, c0:c0 + nc] += 1
+        vi, vo = _selection_by_loop(c, bo["flags"])
+        assert np.array_equal(c["vis_index"], vi) and np.array_equal(c["vis_offset"], vo)
+        stations |= {(int(a), int(b)) for a, b in zip(md["station1"], md["station2"])}
+    assert cover.max() == 1  # classes and subgrids never overlap
+    assert cover.mean() > 0.99
+    assert len(stations) == bo["classes"][0]["nr_stations"] * (bo["classes"][0]["nr_stations"] - 1) // 2
+    assert bo["nr_visibilities"] == int((cover.astype(bool) & ~bo["flags"]).sum())
+
+
+def test_jones_aterms_are_smooth_unimodular_and_not_identity():
+    a = observation.jones_aterms(np.random.default_rng(1), 2, 5, 32)
+    J = a.reshape(2, 5, 32, 32, 2, 2).astype(np.complex128)
+    assert np.allclose(np.abs(np.linalg.det(J)), 1.0, atol=1e-5)
+    off = np.abs(J - np.eye(2)).mean()
+    assert 0.02 < off < 0.3
+    # smooth: neighbouring pixels differ far less than the perturbation itself
+    assert np.abs(np.diff(J, axis=3)).max() < 0.2 * np.abs(J - np.eye(2)).max()
+
+
+def test_full_cycle_plan_cuts_at_aterm_boundaries():
+    w = observation.make_observation("full_cycle", visibilities=False)
+    md = w["metadata"]
+    t_local = md["time_offset"] % w["config"]["n_times"]
+    assert w["nr_aterm_slots"] == 2 and w["aterms"].shape == (2, 10, 64, 64, 4)
+    assert (md["aterm_index"] == t_local // 64).all() and (md["nr_timesteps"] == 64).all()
+    assert (md["x"] >= 0).all() and (md["x"] + 64 <= 16384).all()
