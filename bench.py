@@ -35,6 +35,14 @@ def flop_model(md, ch, S, C_all):
     return flops, sincos, int(V.sum()), int(T.sum())
 
 
+def flop_model_selected(md, vis_offset, S):
+    """flop_model with V_s = the selected visibilities of subgrid s (vis_offset[s+1] - vis_offset[s])."""
+    P = S * S
+    T = md["nr_timesteps"].astype(np.int64)
+    V = np.diff(np.asarray(vis_offset, np.int64))
+    return int((P * (5 * T + 34 * V) + 134 * P).sum()), int((P * V).sum()), int(V.sum())
+
+
 def algorithmic_bytes(md, ch, S, C_all, n_slots, n_stations, direction):
     """HBM bytes one pass must move: visibilities 32 B, uvw 12 B per timestep, metadata 36 B and channel window
     8 B per subgrid, wavenumbers, taper, the A-terms once, and the subgrids (32 B per pixel)."""
@@ -111,12 +119,12 @@ class OneShot:
 class Batched:
     """Persistent work-queue kernels over the whole plan (idgx_batch_*), tile shape chosen by the autotuner."""
 
-    def __init__(self, w, d_vis, precision, stream):
+    def __init__(self, classes, d_vis, precision, stream):
         from idgx import Batch
         from idgx.device import DeviceBatch
 
         self.precision, self.path = precision, "batched"
-        self.db = DeviceBatch([w], precision=precision, visibilities_on_device=d_vis)
+        self.db = DeviceBatch(classes, precision=precision, visibilities_on_device=d_vis)
         tg, td = self.db.batch.autotune(self.db.data, repeats=2, stream=stream)
         gs, ds = self.db.batch.shape()
         shapes = Batch.shapes()
@@ -163,6 +171,42 @@ def time_form(runner, steps, warmup, stream):
     return t_grid, t_degrid, wall / steps
 
 
+def time_cycle(runner, dw, steps, warmup, stream):
+    """Average time (ms) of each stage of an imaging cycle on resident buffers: clear the master grid, grid, subgrid
+    FFT, add to the grid, split from the grid, inverse subgrid FFT, degrid (HIP events on `stream`)."""
+    import torch
+
+    from idgx import FFT_BACKWARD, FFT_FORWARD
+
+    plan = dw.plan()
+    d_sub, _ = runner.buffers()
+    d_grid = dw.grid_buffer()
+    stages = [("grid_clear", lambda: d_grid.zero_()),
+              ("gridder", lambda: runner.grid(stream)),
+              ("subgrid_fft", lambda: dw.fft(plan, FFT_FORWARD, d_sub, stream)),
+              ("adder", lambda: dw.add(plan, d_grid, d_sub, stream)),
+              ("splitter", lambda: dw.split(plan, d_grid, d_sub, stream)),
+              ("subgrid_ifft", lambda: dw.fft(plan, FFT_BACKWARD, d_sub, stream)),
+              ("degridder", lambda: runner.degrid(stream))]
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            for _, f in stages:
+                f()
+        torch.cuda.synchronize()
+        acc = [0.0] * len(stages)
+        for _ in range(steps):
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
+            ev[0].record(stream)
+            for i, (_, f) in enumerate(stages):
+                f()
+                ev[i + 1].record(stream)
+            torch.cuda.synchronize()
+            for i in range(len(stages)):
+                acc[i] += ev[i].elapsed_time(ev[i + 1])
+    del d_grid
+    return {name: acc[i] / steps for i, (name, _) in enumerate(stages)}
+
+
 def accuracy_vs_fp64(runner, w, host_vis, idx):
     """Relative L2 error vs the FP64 oracle of the gridder output and the degridder output (degridding the same
     subgrids) on the sample of subgrids idx; returns (whole-sample, worst subgrid) per direction."""
@@ -199,27 +243,202 @@ def accuracy_vs_fp64(runner, w, host_vis, idx):
     }
 
 
+def class_sample(bo, k, idx, d_vis, d_flags):
+    """Host copy of subgrids idx of class k of a batched workload: the subset workload (rows gathered), its
+    visibilities with the flagged samples zeroed, and the selection mask of its windows."""
+    import torch
+
+    from idgx import observation
+
+    c = bo["classes"][k]
+    C = int(bo["wavenumbers"].shape[0])
+    sw = observation.subset(dict(c, visibilities=None), idx)
+    rows = torch.from_numpy(sw["source_rows"]).to(d_vis.device)
+    vis = d_vis.view(-1, C * 8)[rows].cpu().numpy().view(np.complex64).reshape(-1, C, 4)
+    fl = d_flags[rows].cpu().numpy()
+    sel = np.zeros(fl.shape, bool)
+    md, ch = sw["metadata"], sw["channels"]
+    for s in range(md.size):
+        r0, nt = int(md["time_offset"][s]), int(md["nr_timesteps"][s])
+        c0, nc = int(ch["channel_begin"][s]), int(ch["nr_channels"][s])
+        sel[r0:r0 + nt, c0:c0 + nc] = True
+    sel &= ~fl
+    sw["visibilities"] = np.where(fl[..., None], 0, vis).astype(np.complex64)
+    return sw, sel, rows
+
+
+def run_batched_config(args):
+    """Batched uneven workload: every subgrid class in one persistent launch per direction, flags as selection
+    lists, all built in HBM; throughput counts the selected (unflagged) visibilities."""
+    import torch
+
+    from idgx import observation
+    from idgx.device import device_flags, device_selection_lists
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from helpers import oracle_degrid, oracle_grid, per_subgrid_rel_l2, rel_l2
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    t_setup = time.perf_counter()
+    bo = observation.make_batched_observation(args.config, visibilities=False, flags=False)
+    cfg = bo["config"]
+    C, n_rows = int(bo["wavenumbers"].shape[0]), int(bo["nr_rows"])
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(cfg["seed"]))
+    d_vis = torch.randn(n_rows * C * 8, generator=gen, device=dev, dtype=torch.float32)
+    n_bl = cfg["n_stations"] * (cfg["n_stations"] - 1) // 2
+    d_flags = device_flags(n_bl, cfg["n_times"], C, cfg["flag_max"], cfg["seed"], device=dev)
+    flops = sincos = nvis = 0
+    for c in bo["classes"]:
+        d_vi, d_off, h_off = device_selection_lists(c, d_flags)
+        c.update(d_vis_index=d_vi, d_vis_offset=d_off, vis_offset=h_off, nr_visibilities=int(h_off[-1]))
+        f, sc, v = flop_model_selected(c["metadata"], h_off, int(c["subgrid_size"]))
+        flops, sincos, nvis = flops + f, sincos + sc, nvis + v
+    torch.cuda.synchronize()
+    n_sub = sum(int(c["nr_subgrids"]) for c in bo["classes"])
+    window = n_bl * cfg["n_times"] * C
+    print(f"bench: {args.config}: {n_sub} subgrids in {len(bo['classes'])} classes "
+          f"(S = {', '.join(str(c['subgrid_size']) for c in bo['classes'])}), {nvis} selected visibilities of "
+          f"{window} ({1 - nvis / window:.1%} flagged or off-grid), setup {time.perf_counter() - t_setup:.1f} s",
+          file=sys.stderr, flush=True)
+
+    precisions = [args.precision] if args.precision != "fp32" else ["fp16x3"]
+    forms = []
+    runners = []
+    for prec in precisions:
+        runner = Batched(bo["classes"], d_vis, prec, stream)
+        tg, td, step = time_form(runner, args.steps, args.warmup, stream)
+        rec = {"precision": prec, "path": runner.path, "gridder_ms": tg * 1e3, "degridder_ms": td * 1e3,
+               "step_ms": step * 1e3, "round_trip_mvis_s": nvis / (tg + td) / 1e6,
+               "gridder_mvis_s": nvis / tg / 1e6, "degridder_mvis_s": nvis / td / 1e6,
+               "gridder_tflops": flops / tg / 1e12, "degridder_tflops": flops / td / 1e12}
+        rec.update(runner.info)
+        print(f"bench: {prec} batched: gridder {tg * 1e3:.2f} ms, degridder {td * 1e3:.2f} ms, "
+              f"{rec['round_trip_mvis_s']:.1f} Mvis/s", file=sys.stderr, flush=True)
+        if not args.no_check:
+            acc = {}
+            for k, c in enumerate(bo["classes"]):
+                every = args.check_every or max(1, int(c["nr_subgrids"]) // 60)
+                idx = np.arange(every // 2, int(c["nr_subgrids"]), every)
+                sw, sel, rows = class_sample(bo, k, idx, d_vis, d_flags)
+                S = int(c["subgrid_size"])
+                d_sub = runner.db.classes[k]["d_subgrids"]
+                subs = d_sub.view(-1, 4 * S * S * 8)[torch.from_numpy(idx).to(dev)].cpu().numpy()
+                subs = subs.view(np.complex64).reshape(idx.size, 4, S, S)
+                vis_out = runner.db.d_vis_out.view(-1, C * 32)[rows].cpu().numpy().view(np.complex64)
+                vis_out = vis_out.reshape(-1, C, 4)
+                g64 = oracle_grid(sw, fp64=True)
+                v64 = oracle_degrid(sw, subs, fp64=True)
+                acc[f"S{S}"] = {"subgrids": int(idx.size), "gridder": rel_l2(subs, g64),
+                                "gridder_worst_subgrid": float(per_subgrid_rel_l2(subs, g64).max()),
+                                "degridder": rel_l2(vis_out[sel], v64[sel])}
+            rec["rel_l2_vs_fp64"] = acc
+        forms.append(rec)
+        runners.append(runner)
+    head, runner = forms[-1], runners[-1]
+    tg, td = head["gridder_ms"] / 1e3, head["degridder_ms"] / 1e3
+    dominant, t_dom = ("degridder", td) if td > tg else ("gridder", tg)
+    achieved = flops / t_dom / 1e12
+    dom_key = kernel_key(head["precision"], "batched", dominant, runner.shapes[dominant])
+    traffic = measured_traffic(args.config, dom_key)
+
+    cpu = None
+    if not args.no_cpu:
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t_cpu = v_cpu = 0.0
+        n_cpu = 0
+        for k, c in enumerate(bo["classes"]):
+            every = args.cpu_every or max(1, int(c["nr_subgrids"]) // 40)
+            idx = np.arange(0, int(c["nr_subgrids"]), every)
+            sw, sel, _ = class_sample(bo, k, idx, d_vis, d_flags)
+            t0 = time.perf_counter()
+            sub = oracle_grid(sw, nthreads=nthreads)
+            oracle_degrid(sw, sub, nthreads=nthreads)
+            t_cpu += time.perf_counter() - t0
+            v_cpu += int(sel.sum())
+            n_cpu += idx.size
+        cpu = {"value": v_cpu / t_cpu / 1e6, "unit": "Mvis/s", "cores": nthreads, "kind": "port",
+               "sample": f"about 40 subgrids per class ({n_cpu} subgrids, {int(v_cpu)} selected visibilities): "
+                         f"oracle FP32 restatement of the reference OpenMP CPU path, gridder + degridder, flagged "
+                         f"samples zeroed (the CPU path has no selection lists, so it processes every sample of "
+                         f"each window); value counts the selected visibilities"}
+
+    out = {
+        "metric": "Mvis/s gridder+degridder, 1 MI355X: FP32 vs MFMA mixed at rel-err<=5e-5",
+        "value": head["round_trip_mvis_s"],
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["step_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"bf16x3": "bf16x3-mfma+f32acc", "fp16x3": "fp16x3-mfma+f32acc"}.get(head["precision"],
+                                                                                   head["precision"] + "-mfma"),
+        "data": "synthetic (seeded layout/uvw/plan; N(0,1) complex64 visibilities and per-baseline flags generated "
+                "in HBM)",
+        "config": {
+            "workload": f"{args.config}: {cfg['n_stations']} stations, {cfg['n_times']} x 1 s, {C} channels "
+                        f"{cfg['f_lo'] / 1e6:.0f}-{cfg['f_hi'] / 1e6:.0f} MHz, S = "
+                        f"{'/'.join(str(c['subgrid_size']) for c in bo['classes'])} by baseline-length tercile, "
+                        f"time chunks log-uniform in {list(cfg['t_range'])}, per-baseline flag fraction "
+                        f"U(0, {cfg['flag_max']}), G={cfg['grid_size']}, image {cfg['image_size']} rad",
+            "subgrids": n_sub,
+            "visibilities": nvis,
+            "samples_in_windows": window,
+            "precision": head["precision"],
+            "path": "batched",
+            "classes": [{"subgrid_size": int(c["subgrid_size"]), "subgrids": int(c["nr_subgrids"]),
+                         "visibilities": int(c["nr_visibilities"])} for c in bo["classes"]],
+        },
+        "speedup_vs_plain_fp32": None,
+        "forms": forms,
+        "gflops_per_pass": flops / 1e9,
+        "gsincos_per_pass": sincos / 1e9,
+        "roofline": {
+            "bound": "mfma", "kernel": dom_key, "pass": dominant, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None if traffic is None else traffic["bytes"], "traffic_detail": traffic,
+            "flop_model": "P*(5*T + 34*V) + 134*P per subgrid, V = selected visibilities (FP32-equivalent "
+                          "algorithmic flops), sincos separate",
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="ska_low")
+    ap.add_argument("--config", default="ska_low",
+                    help="ska_low (default), small, full_cycle, or a batched configuration: batched_uneven, "
+                         "batched_uneven_small")
     ap.add_argument("--precision", default="fp16x3", help="headline mixed form (the plain fp32 form is timed beside it)")
     ap.add_argument("--no-batch", action="store_true", help="headline on the one-shot kernels instead of the batch")
     ap.add_argument("--cpu-every", type=int, default=0, help="CPU baseline on every k-th subgrid (0 = auto)")
     ap.add_argument("--check-every", type=int, default=0, help="FP64 accuracy sample: every k-th subgrid (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--cycle", action="store_true", help="also time the imaging-cycle stages (default for full_cycle)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the hot path is per-GPU; see the sharding design); "
               "run with --gpus 1", file=sys.stderr)
         return 2
 
+    from idgx import observation
+
+    if args.config in observation.BATCHED_CONFIGS:
+        return run_batched_config(args)
+
     import torch
 
-    from idgx import observation
     from idgx.device import DeviceWorkload
 
     dev = torch.device("cuda:0")
@@ -247,7 +466,7 @@ def main():
     if args.precision != "fp32":
         runners.append(OneShot(dw, args.precision))
         if not args.no_batch:
-            runners.append(Batched(w, d_vis, args.precision, stream))
+            runners.append(Batched([w], d_vis, args.precision, stream))
     forms = []
     for runner in runners:
         prec = runner.precision
@@ -266,6 +485,14 @@ def main():
         forms.append(rec)
     head = forms[-1]
     plain = forms[0]
+    cycle = None
+    if args.cycle or args.config == "full_cycle":
+        cycle = time_cycle(runners[-1], dw, args.steps, args.warmup, stream)
+        total = sum(cycle.values())
+        cycle = {"stages_ms": cycle, "total_ms": total, "cycle_mvis_s": nvis / (total / 1e3) / 1e6,
+                 "precision": head["precision"], "path": head["path"]}
+        print(f"bench: imaging cycle ({head['precision']} {head['path']}): " +
+              ", ".join(f"{k} {v:.2f} ms" for k, v in cycle["stages_ms"].items()), file=sys.stderr, flush=True)
     tg, td = head["gridder_ms"] / 1e3, head["degridder_ms"] / 1e3
     dominant, t_dom = ("degridder", td) if td > tg else ("gridder", tg)
     achieved = flops / t_dom / 1e12
@@ -338,6 +565,8 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if cycle is not None:
+        out["imaging_cycle"] = cycle
     print(json.dumps(out))
     return 0
 

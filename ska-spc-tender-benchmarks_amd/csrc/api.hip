@@ -237,7 +237,8 @@ extern "C" int idgx_add_subgrids(const idgx_plan *plan, const idgx_adder_plan *a
     if (!d_metadata || !d_order || !d_tile_start || !d_subgrids || !d_grid)
         return IDGX_ERR_INVALID_ARGUMENT;
     return to_status(idgx::launch_adder(d_metadata, d_order, d_tile_start, ap->color_begin, plan->subgrid_size,
-                                        plan->grid_size, d_subgrids, d_grid, static_cast<hipStream_t>(stream)));
+                                        ap->tile_size, plan->grid_size, d_subgrids, d_grid,
+                                        static_cast<hipStream_t>(stream)));
 }
 
 extern "C" int idgx_split_subgrids(const idgx_plan *plan, const idgx_metadata *d_metadata, const void *d_grid,

@@ -5,11 +5,13 @@
 // adder and read as zero by the splitter.
 //
 // Conflict-free adder.  The grid is cut into tiles of T >= S cells; a subgrid belongs to the tile holding its
-// corner and can only touch that tile and its right/lower neighbours.  Tiles are coloured by the parity of their
-// tile coordinates: two different tiles of one colour are at least two tiles apart in x or y, so their subgrids
-// never touch the same cell.  One launch per colour; one workgroup per tile adds that tile's subgrids one after
-// another (a barrier between subgrids), so every grid cell sees a fixed summation order: the result is
-// deterministic and needs no atomics.  The tile lists are built on the host from the plan
+// corner and can only touch that tile's 2T x 2T region (the tile and its right/lower neighbours).  Tiles are
+// coloured by the parity of their tile coordinates: two different tiles of one colour are at least two tiles
+// apart in x or y, so their regions are disjoint.  One launch per colour.  A workgroup takes one correlation of
+// one row block (at most 1024 cells) of one quadrant of a tile's region, walks the tile's subgrids in plan order
+// accumulating the covered cells in registers, and then adds each accumulated cell to the grid once.  Every
+// cell therefore sees a fixed summation order (deterministic, no atomics), and a tile crowded with subgrids is
+// spread over 16 x (row blocks) workgroups instead of one.  The tile lists are built on the host from the plan
 // (idgx_adder_plan_build).
 #include "idgx_device.hpp"
 #include "kernels.hpp"
@@ -20,33 +22,85 @@ namespace
 {
 constexpr int kThreads = 256;
 
-__global__ __launch_bounds__(kThreads) void adder_kernel(const idgx_metadata *md, const int32_t *order,
-                                                         const int32_t *tile_start, int tile_begin, int S, int G,
-                                                         const float2 *subgrids, float2 *grid)
+constexpr int kAdderCells = 4; // grid cells per thread of the adder
+
+__device__ __forceinline__ int floordiv_dev(int a, int b)
 {
-    const int tile = tile_begin + blockIdx.x;
+    return a >= 0 ? a / b : -((-a + b - 1) / b);
+}
+
+// Rows of a tile quadrant one adder workgroup covers (at most kThreads * kAdderCells cells).
+__host__ __device__ inline int adder_rows(int T)
+{
+    const int r = (kThreads * kAdderCells) / T;
+    return r < T ? r : T;
+}
+
+__global__ __launch_bounds__(kThreads) void adder_kernel(const idgx_metadata *md, const int32_t *order,
+                                                         const int32_t *tile_start, int tile_begin, int S, int T,
+                                                         int G, const float2 *subgrids, float2 *grid)
+{
+    const int R = adder_rows(T), nrb = (T + R - 1) / R;
+    const int per_tile = 16 * nrb;
+    const int tile = tile_begin + static_cast<int>(blockIdx.x) / per_tile;
+    const int sub_b = static_cast<int>(blockIdx.x) % per_tile;
+    const int pol = sub_b / (4 * nrb), quad = (sub_b / nrb) & 3, rb = sub_b % nrb;
     const int first = tile_start[tile], last = tile_start[tile + 1];
+    const idgx_metadata m0 = md[order[first]];
+    // Cells [by, by + nrows) x [bx, bx + T) of the grid.  The tile origin is taken from the tile index as the plan
+    // keys it (clamped to -1 .. G/T + 1), so every subgrid of the tile agrees on it.
+    const int ntiles_axis = floordiv_dev(G, T) + 3;
+    const int tx = min(max(floordiv_dev(m0.x, T) + 1, 0), ntiles_axis - 1) - 1;
+    const int ty = min(max(floordiv_dev(m0.y, T) + 1, 0), ntiles_axis - 1) - 1;
+    const int bx = tx * T + (quad & 1) * T;
+    const int by = ty * T + (quad >> 1) * T + rb * R;
+    const int nrows = min(R, T - rb * R);
+    const int ncells = nrows * T;
     const int P = S * S;
-    const size_t GG = static_cast<size_t>(G) * G;
+    float2 acc[kAdderCells];
+    bool touched[kAdderCells];
+    int cx[kAdderCells], cy[kAdderCells];
+#pragma unroll
+    for (int k = 0; k < kAdderCells; ++k)
+    {
+        const int c = threadIdx.x + k * kThreads;
+        cy[k] = by + c / T;
+        cx[k] = bx + c % T;
+        acc[k] = make_float2(0.F, 0.F);
+        touched[k] = false;
+        if (c >= ncells)
+            cy[k] = -0x40000000; // never covered
+    }
     for (int i = first; i < last; ++i)
     {
         const int s = order[i];
         const int x0 = md[s].x, y0 = md[s].y;
-        const float2 *sub = subgrids + static_cast<size_t>(s) * kCorr * P;
-        for (int q = threadIdx.x; q < kCorr * P; q += kThreads)
+        if (x0 + S <= bx || x0 >= bx + T || y0 + S <= by || y0 >= by + nrows)
+            continue; // uniform: no overlap with this block
+        const float2 *sub = subgrids + (static_cast<size_t>(s) * kCorr + pol) * P;
+#pragma unroll
+        for (int k = 0; k < kAdderCells; ++k)
         {
-            const int pol = q / P, rem = q - pol * P;
-            const int y = rem / S, x = rem - y * S;
-            const int gy = y0 + y, gx = x0 + x;
-            if (gy >= 0 && gy < G && gx >= 0 && gx < G)
+            const int dx = cx[k] - x0, dy = cy[k] - y0;
+            if (dx >= 0 && dx < S && dy >= 0 && dy < S)
             {
-                float2 *g = grid + pol * GG + static_cast<size_t>(gy) * G + gx;
-                const float2 v = sub[q];
-                const float2 o = *g;
-                *g = make_float2(o.x + v.x, o.y + v.y);
+                const float2 v = sub[dy * S + dx];
+                acc[k].x += v.x;
+                acc[k].y += v.y;
+                touched[k] = true;
             }
         }
-        __syncthreads(); // the next subgrid of this tile may overlap this one
+    }
+    const size_t GG = static_cast<size_t>(G) * G;
+#pragma unroll
+    for (int k = 0; k < kAdderCells; ++k)
+    {
+        if (touched[k] && cx[k] >= 0 && cx[k] < G && cy[k] >= 0 && cy[k] < G)
+        {
+            float2 *g = grid + pol * GG + static_cast<size_t>(cy[k]) * G + cx[k];
+            const float2 o = *g;
+            *g = make_float2(o.x + acc[k].x, o.y + acc[k].y);
+        }
     }
 }
 
@@ -72,15 +126,23 @@ __global__ __launch_bounds__(kThreads) void splitter_kernel(const idgx_metadata 
 } // namespace
 
 hipError_t launch_adder(const idgx_metadata *md, const int32_t *order, const int32_t *tile_start,
-                        const int32_t *color_begin, int S, int G, const void *subgrids, void *grid, hipStream_t st)
+                        const int32_t *color_begin, int S, int T, int G, const void *subgrids, void *grid,
+                        hipStream_t st)
 {
+    if (T < S || T <= 0)
+        return hipErrorInvalidValue;
+    const int R = adder_rows(T);
+    const long long per_tile = 16LL * ((T + R - 1) / R);
     for (int c = 0; c < 4; ++c)
     {
-        const int n = color_begin[c + 1] - color_begin[c];
+        const long long n = color_begin[c + 1] - color_begin[c];
         if (n <= 0)
             continue;
-        adder_kernel<<<n, kThreads, 0, st>>>(md, order, tile_start, color_begin[c], S, G,
-                                             static_cast<const float2 *>(subgrids), static_cast<float2 *>(grid));
+        if (n * per_tile > 0x7fffffffLL)
+            return hipErrorInvalidConfiguration;
+        adder_kernel<<<static_cast<unsigned>(n * per_tile), kThreads, 0, st>>>(
+            md, order, tile_start, color_begin[c], S, T, G, static_cast<const float2 *>(subgrids),
+            static_cast<float2 *>(grid));
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess)
             return e;

@@ -143,6 +143,63 @@ def selection_lists(w, flags, chunk_entries=1 << 26):
     return np.ascontiguousarray(vi.astype(np.uint32)), offset
 
 
+def device_flags(n_baselines, n_times, n_channels, flag_max, seed, device="cuda", rows_per_chunk=1 << 20):
+    """Boolean flags [rows][C] generated in HBM: baseline b flags each (timestep, channel) sample with probability
+    f_b ~ U(0, flag_max) (f_b drawn as on the host, numpy seed + 2; the per-sample uniforms from a torch generator
+    seeded with seed + 3)."""
+    torch = _torch()
+    frac = np.random.default_rng(seed + 2).uniform(0.0, flag_max, n_baselines).astype(np.float32)
+    d_frac = torch.from_numpy(frac).to(device)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed + 3)
+    n_rows = n_baselines * n_times
+    flags = torch.empty((n_rows, n_channels), dtype=torch.bool, device=device)
+    for r0 in range(0, n_rows, rows_per_chunk):
+        r1 = min(n_rows, r0 + rows_per_chunk)
+        f = d_frac[torch.arange(r0, r1, device=device) // n_times][:, None]
+        flags[r0:r1] = torch.rand((r1 - r0, n_channels), generator=gen, device=device) < f
+    return flags
+
+
+def device_selection_lists(w, d_flags, chunk_entries=1 << 27):
+    """selection_lists computed in HBM from a device flag mask [rows][C]: returns (d_vis_index int32 tensor holding
+    the uint32 entries, d_vis_offset int64 tensor, vis_offset numpy int64)."""
+    torch = _torch()
+    dev = d_flags.device
+    md, ch = w["metadata"], w.get("channels")
+    C = int(d_flags.shape[1])
+    n = md.size
+    r0 = md["time_offset"].astype(np.int64) + md["baseline_offset"].astype(np.int64) - \
+        (int(md["baseline_offset"][0]) if n else 0)
+    T = md["nr_timesteps"].astype(np.int64)
+    c0 = np.zeros(n, np.int64) if ch is None else ch["channel_begin"].astype(np.int64)
+    nc = np.full(n, C, np.int64) if ch is None else ch["nr_channels"].astype(np.int64)
+    d_r0, d_T, d_c0, d_nc = (torch.from_numpy(x).to(dev) for x in (r0, T, c0, nc))
+    cum = np.cumsum(T * nc)
+    flat = d_flags.reshape(-1)
+    out, counts = [], torch.zeros(n, dtype=torch.int64, device=dev)
+    s0 = 0
+    while s0 < n:
+        base = cum[s0 - 1] if s0 else 0
+        s1 = min(n, max(s0 + 1, int(np.searchsorted(cum, base + chunk_entries, side="right"))))
+        sub = torch.arange(s0, s1, device=dev)
+        Ts = d_T[sub]
+        st_s = torch.repeat_interleave(sub, Ts)
+        st_t = torch.arange(st_s.numel(), device=dev) - torch.repeat_interleave(torch.cumsum(Ts, 0) - Ts, Ts)
+        ncs = d_nc[st_s]
+        e_st = torch.repeat_interleave(torch.arange(st_s.numel(), device=dev), ncs)
+        e_c = torch.arange(e_st.numel(), device=dev) - torch.repeat_interleave(torch.cumsum(ncs, 0) - ncs, ncs)
+        e_s, e_t = st_s[e_st], st_t[e_st]
+        keep = ~flat[(d_r0[e_s] + e_t) * C + d_c0[e_s] + e_c]
+        entry = (e_t[keep] << 16) | e_c[keep]
+        out.append(torch.where(entry >= (1 << 31), entry - (1 << 32), entry).to(torch.int32))
+        counts[s0:s1] = torch.bincount(e_s[keep] - s0, minlength=s1 - s0)
+        s0 = s1
+    d_vi = torch.cat(out) if out else torch.zeros(0, dtype=torch.int32, device=dev)
+    d_off = torch.cat([torch.zeros(1, dtype=torch.int64, device=dev), torch.cumsum(counts, 0)])
+    return d_vi, d_off, d_off.cpu().numpy()
+
+
 class DeviceBatch:
     """One or more subgrid classes of an observation resident in HBM, run through the batch API (persistent
     work-queue kernels).  `classes` are workload dicts sharing uvw / wavenumbers / visibilities rows; each may

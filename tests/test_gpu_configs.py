@@ -111,3 +111,16 @@ def test_full_cycle_adjoint_on_the_16k_grid(cycle, precision):
     print(f"full_cycle adjoint {precision}: rel diff {abs(lhs - rhs) / abs(rhs):.2e}; box {y1 - y0}x{x1 - x0}")
     assert abs(outside) <= 1e-3 * float(np.abs(gV).sum()), "the adder wrote outside the subgrid footprints"
     assert abs(lhs - rhs) <= 2e-5 * abs(rhs)
+
+
+def test_device_selection_lists_match_the_host_lists(uneven):
+    import torch
+
+    from idgx.device import device_selection_lists
+
+    bo = uneven
+    d_flags = torch.from_numpy(bo["flags"]).cuda()
+    for c in bo["classes"]:
+        d_vi, d_off, h_off = device_selection_lists(c, d_flags, chunk_entries=4096)
+        assert np.array_equal(h_off, c["vis_offset"]) and np.array_equal(d_off.cpu().numpy(), c["vis_offset"])
+        assert np.array_equal(d_vi.cpu().numpy().view(np.uint32), c["vis_index"])

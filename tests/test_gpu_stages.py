@@ -123,10 +123,15 @@ def _overlapping_subgrids(rng, n, S, G):
     return md
 
 
-def test_adder_and_splitter_match_oracle_exactly_and_are_adjoint():
+@pytest.mark.parametrize("S,G", [(16, 128), (24, 200), (48, 256), (64, 320)])
+def test_adder_and_splitter_match_oracle_exactly_and_are_adjoint(S, G):
+    """Integer-valued data (exact sums), piles of subgrids in one tile, subgrids hanging over every edge and a few
+    far outside the grid (their tile index is clamped by the plan)."""
     rng = np.random.default_rng(2)
-    S, G, n = 16, 128, 300
+    n = 300
     md = _overlapping_subgrids(rng, n, S, G)
+    md["x"][6:9] = [-3 * S, G + 2 * S, -S + 3]
+    md["y"][6:9] = [5, -4 * S, G - 2]
     sub = (rng.integers(-8, 9, (n, 4, S, S)) + 1j * rng.integers(-8, 9, (n, 4, S, S))).astype(np.complex64)
     dw = DeviceWorkload(_workload(S, G, md))
     plan = dw.plan()
