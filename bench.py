@@ -207,6 +207,33 @@ def time_cycle(runner, dw, steps, warmup, stream):
     return {name: acc[i] / steps for i, (name, _) in enumerate(stages)}
 
 
+def time_pcie(nvis_bytes, sub_bytes, repeats=3):
+    """Host<->device copy times (s) from pinned host memory for one gridder pass (visibilities in, subgrids out)
+    and one degridder pass (subgrids in, visibilities out), as a caller handing over host buffers would pay."""
+    import torch
+
+    out = {}
+    for name, (nin, nout) in {"gridder": (nvis_bytes, sub_bytes), "degridder": (sub_bytes, nvis_bytes)}.items():
+        h_in = torch.empty(nin, dtype=torch.uint8, pin_memory=True)
+        h_out = torch.empty(nout, dtype=torch.uint8, pin_memory=True)
+        d_in = torch.empty(nin, dtype=torch.uint8, device="cuda")
+        d_out = torch.empty(nout, dtype=torch.uint8, device="cuda")
+        best_in = best_out = float("inf")
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            d_in.copy_(h_in, non_blocking=True)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            h_out.copy_(d_out, non_blocking=True)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            best_in, best_out = min(best_in, t1 - t0), min(best_out, t2 - t1)
+        out[name] = {"in_s": best_in, "out_s": best_out, "in_gbs": nin / best_in / 1e9, "out_gbs": nout / best_out / 1e9}
+        del h_in, h_out, d_in, d_out
+    return out
+
+
 def accuracy_vs_fp64(runner, w, host_vis, idx):
     """Relative L2 error vs the FP64 oracle of the gridder output and the degridder output (degridding the same
     subgrids) on the sample of subgrids idx; returns (whole-sample, worst subgrid) per direction."""
@@ -426,6 +453,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cycle", action="store_true", help="also time the imaging-cycle stages (default for full_cycle)")
+    ap.add_argument("--pcie", action="store_true", help="also time the host transfers a host-buffer caller pays")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the hot path is per-GPU; see the sharding design); "
@@ -567,6 +595,12 @@ def main():
     }
     if cycle is not None:
         out["imaging_cycle"] = cycle
+    if args.pcie:
+        pc = time_pcie(nvis * 32, int(w["nr_subgrids"]) * 4 * S * S * 8)
+        t_pc = tg + td + sum(v["in_s"] + v["out_s"] for v in pc.values())
+        out["pcie_inclusive"] = {"copies": pc, "round_trip_mvis_s": nvis / t_pc / 1e6,
+                                 "note": "host buffers handed over per pass from pinned memory (not the value)"}
+        print(f"bench: PCIe-inclusive round trip {nvis / t_pc / 1e6:.1f} Mvis/s", file=sys.stderr, flush=True)
     print(json.dumps(out))
     return 0
 
