@@ -9,9 +9,10 @@
 // (1 + 1/S)}; backward = diag((-1)^x) DFT- diag(conj(post)).
 //
 // Layout: one workgroup per subgrid (4 correlation planes) for S <= 32, one per plane above.  The planes are
-// loaded coalesced into LDS (rows padded by one element), every thread transforms one row held entirely in
-// registers (a compile-time mixed radix-2/3 recursion, twiddles from an LDS table), then one column, and the
-// result is stored coalesced.  The pass is HBM-bound: 2 x 8 B per complex sample per direction.
+// loaded with 16-byte loads, all in flight before the first LDS store, into LDS (rows padded by one element);
+// every thread transforms one row held entirely in registers (a compile-time mixed radix-2/3 recursion,
+// twiddles from an LDS table), then one column, and the result is stored the same way.  The pass is HBM-bound:
+// 2 x 8 B per complex sample per direction.
 #include "idgx_device.hpp"
 #include "kernels.hpp"
 
@@ -110,33 +111,45 @@ template <int S, int PLANES, bool FORWARD>
 __global__ __launch_bounds__(S *PLANES) void subgrid_fft_kernel(float2 *subgrids, long long nr_planes)
 {
     constexpr int LD = S + 1; // padded row length
+    constexpr int kThreadsFft = S * PLANES;
+    constexpr int kPairs = PLANES * S * S / 2;      // float4 (two complex samples) per workgroup
+    constexpr int kPerThread = kPairs / kThreadsFft; // = S / 2
     __shared__ cfloat s_data[PLANES * S * LD];
     __shared__ float2 s_tw[S];
     __shared__ float2 s_post[S];
 
     const int tid = threadIdx.x;
     const long long plane0 = static_cast<long long>(blockIdx.x) * PLANES;
-    const int nthreads = S * PLANES;
-    for (int k = tid; k < S; k += nthreads)
+    // Twiddles e^{2 pi i k / S} and the output factors e^{i pi (k - S/2)(1 + 1/S)} = (-1)^(k - S/2)
+    // e^{i pi (k - S/2) / S}, from exactly reduced arguments.
+    for (int k = tid; k < S; k += kThreadsFft)
     {
-        double sn, cs;
-        sincospi(2.0 * k / S, &sn, &cs);
-        s_tw[k] = make_float2(static_cast<float>(cs), static_cast<float>(sn));
-        sincospi((k - S / 2) * (1.0 + 1.0 / S), &sn, &cs);
-        s_post[k] = make_float2(static_cast<float>(cs), static_cast<float>(sn));
+        float sn, cs;
+        sincospif(2.F * static_cast<float>(k) / static_cast<float>(S), &sn, &cs);
+        s_tw[k] = make_float2(cs, sn);
+        sincospif(static_cast<float>(k - S / 2) / static_cast<float>(S), &sn, &cs);
+        const float sg = ((k - S / 2) & 1) ? -1.F : 1.F;
+        s_post[k] = make_float2(sg * cs, sg * sn);
     }
-    // Coalesced load of PLANES consecutive planes (only planes < nr_planes exist).
-    for (int i = tid; i < PLANES * S * S; i += nthreads)
+    // Coalesced 16-byte loads of PLANES consecutive planes, all issued before the first LDS store (only planes
+    // < nr_planes exist; the others read as zero).
+    const float4 *src = reinterpret_cast<const float4 *>(subgrids) + plane0 * (S * S / 2);
+    float4 v[kPerThread];
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j)
     {
-        const int p = i / (S * S), rem = i - p * S * S;
+        const int i = tid + j * kThreadsFft;
+        const int p = (2 * i) / (S * S);
+        v[j] = plane0 + p < nr_planes ? src[i] : make_float4(0.F, 0.F, 0.F, 0.F);
+    }
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j)
+    {
+        const int e = 2 * (tid + j * kThreadsFft);
+        const int p = e / (S * S), rem = e - p * S * S;
         const int r = rem / S, c = rem - r * S;
-        cfloat v = {0.F, 0.F};
-        if (plane0 + p < nr_planes)
-        {
-            const float2 g = subgrids[(plane0 + p) * S * S + rem];
-            v = {g.x, g.y};
-        }
-        s_data[(p * S + r) * LD + c] = v;
+        s_data[(p * S + r) * LD + c] = {v[j].x, v[j].y};
+        s_data[(p * S + r) * LD + c + 1] = {v[j].z, v[j].w};
     }
     __syncthreads();
     const int p = tid / S, line = tid - p * S;
@@ -144,15 +157,22 @@ __global__ __launch_bounds__(S *PLANES) void subgrid_fft_kernel(float2 *subgrids
     __syncthreads();
     transform_line<S, FORWARD>(&s_data[p * S * LD + line], LD, s_tw, s_post); // column `line`
     __syncthreads();
-    for (int i = tid; i < PLANES * S * S; i += nthreads)
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j)
     {
-        const int q = i / (S * S), rem = i - q * S * S;
+        const int e = 2 * (tid + j * kThreadsFft);
+        const int q = e / (S * S), rem = e - q * S * S;
         const int r = rem / S, c = rem - r * S;
-        if (plane0 + q < nr_planes)
-        {
-            const cfloat v = s_data[(q * S + r) * LD + c];
-            subgrids[(plane0 + q) * S * S + rem] = make_float2(v.re, v.im);
-        }
+        const cfloat a = s_data[(q * S + r) * LD + c], b = s_data[(q * S + r) * LD + c + 1];
+        v[j] = make_float4(a.re, a.im, b.re, b.im);
+    }
+    float4 *dst = reinterpret_cast<float4 *>(subgrids) + plane0 * (S * S / 2);
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j)
+    {
+        const int i = tid + j * kThreadsFft;
+        if (plane0 + (2 * i) / (S * S) < nr_planes)
+            dst[i] = v[j];
     }
 }
 
