@@ -22,7 +22,8 @@ namespace
 {
 constexpr int kThreads = 256;
 
-constexpr int kAdderCells = 4; // grid cells per thread of the adder
+constexpr int kAdderCells = 4;  // grid cells per thread of the adder
+constexpr int kAdderUnroll = 4; // subgrids whose loads the adder keeps in flight together
 
 __device__ __forceinline__ int floordiv_dev(int a, int b)
 {
@@ -71,25 +72,43 @@ __global__ __launch_bounds__(kThreads) void adder_kernel(const idgx_metadata *md
         if (c >= ncells)
             cy[k] = -0x40000000; // never covered
     }
-    for (int i = first; i < last; ++i)
+    // kAdderUnroll subgrids per step: their loads are all in flight together, then added in plan order.
+    for (int i0 = first; i0 < last; i0 += kAdderUnroll)
     {
-        const int s = order[i];
-        const int x0 = md[s].x, y0 = md[s].y;
-        if (x0 + S <= bx || x0 >= bx + T || y0 + S <= by || y0 >= by + nrows)
-            continue; // uniform: no overlap with this block
-        const float2 *sub = subgrids + (static_cast<size_t>(s) * kCorr + pol) * P;
+        float2 val[kAdderUnroll][kAdderCells];
+        bool hit[kAdderUnroll][kAdderCells];
 #pragma unroll
-        for (int k = 0; k < kAdderCells; ++k)
+        for (int u = 0; u < kAdderUnroll; ++u)
         {
-            const int dx = cx[k] - x0, dy = cy[k] - y0;
-            if (dx >= 0 && dx < S && dy >= 0 && dy < S)
+            const int i = i0 + u;
+            int s = 0, x0 = 0, y0 = 0;
+            bool any = i < last;
+            if (any)
             {
-                const float2 v = sub[dy * S + dx];
-                acc[k].x += v.x;
-                acc[k].y += v.y;
-                touched[k] = true;
+                s = order[i];
+                x0 = md[s].x;
+                y0 = md[s].y;
+                any = !(x0 + S <= bx || x0 >= bx + T || y0 + S <= by || y0 >= by + nrows); // uniform
+            }
+            const float2 *sub = subgrids + (static_cast<size_t>(s) * kCorr + pol) * P;
+#pragma unroll
+            for (int k = 0; k < kAdderCells; ++k)
+            {
+                const int dx = cx[k] - x0, dy = cy[k] - y0;
+                hit[u][k] = any && dx >= 0 && dx < S && dy >= 0 && dy < S;
+                val[u][k] = hit[u][k] ? sub[dy * S + dx] : make_float2(0.F, 0.F);
             }
         }
+#pragma unroll
+        for (int u = 0; u < kAdderUnroll; ++u)
+#pragma unroll
+            for (int k = 0; k < kAdderCells; ++k)
+                if (hit[u][k])
+                {
+                    acc[k].x += val[u][k].x;
+                    acc[k].y += val[u][k].y;
+                    touched[k] = true;
+                }
     }
     const size_t GG = static_cast<size_t>(G) * G;
 #pragma unroll
