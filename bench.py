@@ -377,7 +377,7 @@ def run_batched_config(args):
         t_cpu = v_cpu = 0.0
         n_cpu = 0
         for k, c in enumerate(bo["classes"]):
-            every = args.cpu_every or max(1, int(c["nr_subgrids"]) // 40)
+            every = args.cpu_every or max(1, int(c["nr_subgrids"]) // 400)
             idx = np.arange(0, int(c["nr_subgrids"]), every)
             sw, sel, _ = class_sample(bo, k, idx, d_vis, d_flags)
             t0 = time.perf_counter()
@@ -387,7 +387,7 @@ def run_batched_config(args):
             v_cpu += int(sel.sum())
             n_cpu += idx.size
         cpu = {"value": v_cpu / t_cpu / 1e6, "unit": "Mvis/s", "cores": nthreads, "kind": "port",
-               "sample": f"about 40 subgrids per class ({n_cpu} subgrids, {int(v_cpu)} selected visibilities): "
+               "sample": f"about 400 subgrids per class ({n_cpu} subgrids, {int(v_cpu)} selected visibilities): "
                          f"oracle FP32 restatement of the reference OpenMP CPU path, gridder + degridder, flagged "
                          f"samples zeroed (the CPU path has no selection lists, so it processes every sample of "
                          f"each window); value counts the selected visibilities"}
@@ -533,7 +533,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        every = args.cpu_every or max(1, w["nr_subgrids"] // 256)
+        every = args.cpu_every or max(1, w["nr_subgrids"] // 2560)
         r = cpu_baseline(dict(w, visibilities=host_vis), every, nthreads)
         cpu = {
             "value": r["nvis"] / (r["grid_s"] + r["degrid_s"]) / 1e6,
