@@ -160,3 +160,25 @@ def test_autotuner_picks_a_shape_and_keeps_results():
     db.degrid()
     assert rel_l2(db.subgrids(), sub0) < 1e-6
     assert rel_l2(db.visibilities_out(), vis0) < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp16x3", "bf16x3"])
+def test_every_tile_shape_gives_the_same_results(precision):
+    """All compiled tile shapes (tiles per wave, LDS depth, phases on the matrix core) agree with the default one
+    to rounding, on a plan with mixed subgrid sizes' edge cases (S = 24 rows not a multiple of the tile)."""
+    w = observation.make_observation("small")
+    db = DeviceBatch([w], precision=precision)
+    db.grid()
+    db.degrid()
+    sub0, vis0 = db.subgrids(), db.visibilities_out()
+    for k in range(len(Batch.shapes())):
+        db.batch.set_shape(k, k)
+        db.classes[0]["d_subgrids"].zero_()
+        db.d_vis_out.zero_()
+        db.grid()
+        sub = db.subgrids()
+        db.degrid()
+        vis = db.visibilities_out()
+        eg, ev = rel_l2(sub, sub0), rel_l2(vis, vis0)
+        print(f"{precision} shape {Batch.shapes()[k]}: gridder {eg:.2e}, degridder {ev:.2e}")
+        assert eg < 2e-6 and ev < 2e-6
