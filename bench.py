@@ -173,7 +173,8 @@ def time_form(runner, steps, warmup, stream):
 
 def time_cycle(runner, dw, steps, warmup, stream):
     """Average time (ms) of each stage of an imaging cycle on resident buffers: clear the master grid, grid, subgrid
-    FFT, add to the grid, split from the grid, inverse subgrid FFT, degrid (HIP events on `stream`)."""
+    FFT, add to the grid, master-grid FFT to the image and back (deconvolution would sit between the two), split
+    from the grid, inverse subgrid FFT, degrid (HIP events on `stream`)."""
     import torch
 
     from idgx import FFT_BACKWARD, FFT_FORWARD
@@ -185,6 +186,8 @@ def time_cycle(runner, dw, steps, warmup, stream):
               ("gridder", lambda: runner.grid(stream)),
               ("subgrid_fft", lambda: dw.fft(plan, FFT_FORWARD, d_sub, stream)),
               ("adder", lambda: dw.add(plan, d_grid, d_sub, stream)),
+              ("grid_fft_to_image", lambda: dw.grid_fft(plan, d_grid, 1, FFT_BACKWARD, stream)),
+              ("grid_fft_to_uv", lambda: dw.grid_fft(plan, d_grid, 1, FFT_FORWARD, stream)),
               ("splitter", lambda: dw.split(plan, d_grid, d_sub, stream)),
               ("subgrid_ifft", lambda: dw.fft(plan, FFT_BACKWARD, d_sub, stream)),
               ("degridder", lambda: runner.degrid(stream))]

@@ -147,16 +147,25 @@ typedef struct
     int32_t tile_size;
     int32_t nr_tiles;
     int32_t color_begin[5];
+    int32_t z_min;     /* w-stacking: subgrid s goes to layer metadata[s].z - z_min ... */
+    int32_t nr_layers; /* ... of a grid complex64 [nr_layers][4][G][G]; 1 = a single grid, z ignored */
 } idgx_adder_plan;
 
-/* Builds the adder order on the host from the plan's metadata.  h_order has nr_subgrids entries, h_tile_start
- * nr_subgrids + 1 (only nr_tiles + 1 are used). */
+/* Builds the adder order on the host from the plan's metadata (single grid, z ignored).  h_order has nr_subgrids
+ * entries, h_tile_start nr_subgrids + 1 (only nr_tiles + 1 are used). */
 int idgx_adder_plan_build(const idgx_plan *plan, const idgx_metadata *h_metadata, idgx_adder_plan *adder_plan,
                           int32_t *h_order, int32_t *h_tile_start);
 
-/* Adder: d_grid[pol][y0 + y][x0 + x] += d_subgrids[s][pol][y][x] for every subgrid (cells outside the grid are
- * skipped).  d_grid is complex64 [4][G][G]; d_order / d_tile_start are the device copies of the arrays built by
- * idgx_adder_plan_build.  The summation order per cell is fixed, so results are bit-reproducible. */
+/* The same for w-stacking onto nr_layers grids: every metadata[s].z must lie in [z_min, z_min + nr_layers)
+ * (IDGX_ERR_INVALID_METADATA otherwise).  A subgrid's phase reference in the gridder is
+ * w_offset = (z + 0.5) w_step_in_lambda, as in gridding/src/gridding_benchmark.cpp:80. */
+int idgx_adder_plan_build_layered(const idgx_plan *plan, const idgx_metadata *h_metadata, int z_min, int nr_layers,
+                                  idgx_adder_plan *adder_plan, int32_t *h_order, int32_t *h_tile_start);
+
+/* Adder: d_grid[layer][pol][y0 + y][x0 + x] += d_subgrids[s][pol][y][x] for every subgrid (cells outside the
+ * grid are skipped); layer = z - z_min for a layered adder plan, else 0.  d_grid is complex64
+ * [nr_layers][4][G][G]; d_order / d_tile_start are the device copies of the arrays built by
+ * idgx_adder_plan_build(_layered).  The summation order per cell is fixed, so results are bit-reproducible. */
 int idgx_add_subgrids(const idgx_plan *plan, const idgx_adder_plan *adder_plan, const idgx_metadata *d_metadata,
                       const int32_t *d_order, const int32_t *d_tile_start, const void *d_subgrids, void *d_grid,
                       void *stream);
@@ -165,6 +174,32 @@ int idgx_add_subgrids(const idgx_plan *plan, const idgx_adder_plan *adder_plan, 
  * adder. */
 int idgx_split_subgrids(const idgx_plan *plan, const idgx_metadata *d_metadata, const void *d_grid, void *d_subgrids,
                         void *stream);
+
+/* Splitter from w-layers: subgrid s reads layer metadata[s].z - z_min of d_grid (complex64 [nr_layers][4][G][G]);
+ * a subgrid of no layer reads zeros.  The adjoint of the layered adder. */
+int idgx_split_subgrids_layered(const idgx_plan *plan, const idgx_metadata *d_metadata, int z_min, int nr_layers,
+                                const void *d_grid, void *d_subgrids, void *stream);
+
+/* ---- Master grid and w-stacking (the step after the adder; the reference's fft/ benchmark times a plain 2-D FFT
+ * of this size, fft/inc/fft_benchmark.h:65-99) -------------------------------------------------------------------
+ *
+ * Master-grid cell K is at u = (K - G/2) / image_size and image pixel X at l = (X - G/2) image_size / G.  In place
+ * on d_grid (complex64 [nr_layers][4][G][G]), in each dimension:
+ *   FORWARD  (image -> uv):  G[K] = sum_X I[X] exp(+2 pi i (K - G/2)(X - G/2) / G)
+ *   BACKWARD (uv -> image):  I[X] = sum_K G[K] exp(-2 pi i (K - G/2)(X - G/2) / G)
+ * unnormalised (BACKWARD(FORWARD(I)) = G^2 I), the signs of idgx_subgrid_fft.  G must be a multiple of 4.  The DFT
+ * is rocFFT (plans cached per device, G and batch; the first call for a size allocates them). */
+int idgx_grid_fft(const idgx_plan *plan, void *d_grid, int nr_layers, int direction, void *stream);
+
+/* W-stacking in the image domain, with w_z = (z + 0.5) w_step_in_lambda for layer z - z_min and
+ * n = 1 - sqrt(1 - l^2 - m^2) at image pixel (X, Y):
+ *   to_image:   d_image[pol][Y][X] = sum_z d_layers[z - z_min][pol][Y][X] exp(-2 pi i w_z n)
+ *   from_image: d_layers[z - z_min][pol][Y][X] = d_image[pol][Y][X] exp(+2 pi i w_z n)   (the adjoint)
+ * d_image is complex64 [4][G][G] and must not alias d_layers. */
+int idgx_wstack_to_image(const idgx_plan *plan, const void *d_layers, int z_min, int nr_layers, void *d_image,
+                         void *stream);
+int idgx_wstack_from_image(const idgx_plan *plan, const void *d_image, int z_min, int nr_layers, void *d_layers,
+                           void *stream);
 
 /* ---- Batched execution: one persistent launch per direction over a whole plan ---------------------------------
  *

@@ -168,32 +168,42 @@ extern "C" int idgx_subgrid_fft(const idgx_plan *plan, void *d_subgrids, int dir
                                               direction == IDGX_FFT_FORWARD, static_cast<hipStream_t>(stream)));
 }
 
-extern "C" int idgx_adder_plan_build(const idgx_plan *plan, const idgx_metadata *md, idgx_adder_plan *ap,
-                                     int32_t *order, int32_t *tile_start)
+namespace
+{
+// Tiles keyed by (colour, layer, tile row, tile column); sorting by key puts each colour's tiles together, and two
+// tiles of one colour never touch the same cell of the same layer.
+int build_adder_plan(const idgx_plan *plan, const idgx_metadata *md, int z_min, int nr_layers, idgx_adder_plan *ap,
+                     int32_t *order, int32_t *tile_start)
 {
     const int rc = check_plan_struct(plan);
     if (rc != IDGX_OK)
         return rc;
-    if (!ap || (plan->nr_subgrids > 0 && (!md || !order || !tile_start)))
+    if (!ap || nr_layers < 1 || (plan->nr_subgrids > 0 && (!md || !order || !tile_start)))
         return IDGX_ERR_INVALID_ARGUMENT;
     const int n = plan->nr_subgrids;
     const int T = plan->subgrid_size;
     auto floordiv = [](long long a, long long b) { return a >= 0 ? a / b : -((-a + b - 1) / b); };
     std::vector<std::pair<long long, int>> keys(n);
     const long long ntiles_axis = floordiv(plan->grid_size, T) + 3; // tiles from -1 .. G/T + 1
+    const long long L = nr_layers;
     for (int s = 0; s < n; ++s)
     {
+        const long long layer = nr_layers > 1 ? static_cast<long long>(md[s].z) - z_min : 0;
+        if (layer < 0 || layer >= L)
+            return IDGX_ERR_INVALID_METADATA;
         const long long tx = floordiv(md[s].x, T) + 1, ty = floordiv(md[s].y, T) + 1;
         const long long color = ((tx - 1) & 1) + 2 * ((ty - 1) & 1);
         const long long tx_c = std::min(std::max(tx, 0LL), ntiles_axis - 1);
         const long long ty_c = std::min(std::max(ty, 0LL), ntiles_axis - 1);
-        keys[s] = {(color * ntiles_axis + ty_c) * ntiles_axis + tx_c, s};
+        keys[s] = {((color * L + layer) * ntiles_axis + ty_c) * ntiles_axis + tx_c, s};
     }
     std::stable_sort(keys.begin(), keys.end(),
                      [](const std::pair<long long, int> &a, const std::pair<long long, int> &b) {
                          return a.first < b.first;
                      });
     ap->tile_size = T;
+    ap->z_min = nr_layers > 1 ? z_min : 0;
+    ap->nr_layers = nr_layers;
     int tiles = 0;
     for (int c = 0; c <= 4; ++c)
         ap->color_begin[c] = 0;
@@ -203,7 +213,7 @@ extern "C" int idgx_adder_plan_build(const idgx_plan *plan, const idgx_metadata 
         order[i] = keys[i].second;
         if (keys[i].first != prev)
         {
-            const int color = static_cast<int>(keys[i].first / (ntiles_axis * ntiles_axis));
+            const int color = static_cast<int>(keys[i].first / (L * ntiles_axis * ntiles_axis));
             tile_start[tiles] = i;
             for (int c = color + 1; c <= 4; ++c)
                 ap->color_begin[c] = tiles + 1;
@@ -218,6 +228,19 @@ extern "C" int idgx_adder_plan_build(const idgx_plan *plan, const idgx_metadata 
     ap->color_begin[4] = tiles;
     return IDGX_OK;
 }
+} // namespace
+
+extern "C" int idgx_adder_plan_build(const idgx_plan *plan, const idgx_metadata *md, idgx_adder_plan *ap,
+                                     int32_t *order, int32_t *tile_start)
+{
+    return build_adder_plan(plan, md, 0, 1, ap, order, tile_start);
+}
+
+extern "C" int idgx_adder_plan_build_layered(const idgx_plan *plan, const idgx_metadata *md, int z_min,
+                                             int nr_layers, idgx_adder_plan *ap, int32_t *order, int32_t *tile_start)
+{
+    return build_adder_plan(plan, md, z_min, nr_layers, ap, order, tile_start);
+}
 
 extern "C" int idgx_add_subgrids(const idgx_plan *plan, const idgx_adder_plan *ap, const idgx_metadata *d_metadata,
                                  const int32_t *d_order, const int32_t *d_tile_start, const void *d_subgrids,
@@ -227,7 +250,7 @@ extern "C" int idgx_add_subgrids(const idgx_plan *plan, const idgx_adder_plan *a
     if (rc != IDGX_OK)
         return rc;
     if (!ap || ap->tile_size < plan->subgrid_size || ap->nr_tiles < 0 || ap->color_begin[0] != 0 ||
-        ap->color_begin[4] != ap->nr_tiles)
+        ap->color_begin[4] != ap->nr_tiles || ap->nr_layers < 1)
         return IDGX_ERR_INVALID_ARGUMENT;
     for (int c = 0; c < 4; ++c)
         if (ap->color_begin[c] > ap->color_begin[c + 1])
@@ -237,7 +260,7 @@ extern "C" int idgx_add_subgrids(const idgx_plan *plan, const idgx_adder_plan *a
     if (!d_metadata || !d_order || !d_tile_start || !d_subgrids || !d_grid)
         return IDGX_ERR_INVALID_ARGUMENT;
     return to_status(idgx::launch_adder(d_metadata, d_order, d_tile_start, ap->color_begin, plan->subgrid_size,
-                                        ap->tile_size, plan->grid_size, d_subgrids, d_grid,
+                                        ap->tile_size, plan->grid_size, ap->z_min, ap->nr_layers, d_subgrids, d_grid,
                                         static_cast<hipStream_t>(stream)));
 }
 
@@ -251,8 +274,62 @@ extern "C" int idgx_split_subgrids(const idgx_plan *plan, const idgx_metadata *d
         return IDGX_OK;
     if (!d_metadata || !d_grid || !d_subgrids)
         return IDGX_ERR_INVALID_ARGUMENT;
-    return to_status(idgx::launch_splitter(d_metadata, plan->nr_subgrids, plan->subgrid_size, plan->grid_size, d_grid,
-                                           d_subgrids, static_cast<hipStream_t>(stream)));
+    return to_status(idgx::launch_splitter(d_metadata, plan->nr_subgrids, plan->subgrid_size, plan->grid_size, 0, 1,
+                                           d_grid, d_subgrids, static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int idgx_split_subgrids_layered(const idgx_plan *plan, const idgx_metadata *d_metadata, int z_min,
+                                           int nr_layers, const void *d_grid, void *d_subgrids, void *stream)
+{
+    const int rc = check_plan_struct(plan);
+    if (rc != IDGX_OK)
+        return rc;
+    if (nr_layers < 1)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    if (plan->nr_subgrids == 0)
+        return IDGX_OK;
+    if (!d_metadata || !d_grid || !d_subgrids)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    return to_status(idgx::launch_splitter(d_metadata, plan->nr_subgrids, plan->subgrid_size, plan->grid_size,
+                                           nr_layers > 1 ? z_min : 0, nr_layers, d_grid, d_subgrids,
+                                           static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int idgx_grid_fft(const idgx_plan *plan, void *d_grid, int nr_layers, int direction, void *stream)
+{
+    const int rc = check_plan_struct(plan);
+    if (rc != IDGX_OK)
+        return rc;
+    if (nr_layers < 1 || !d_grid || (direction != IDGX_FFT_FORWARD && direction != IDGX_FFT_BACKWARD))
+        return IDGX_ERR_INVALID_ARGUMENT;
+    if (!idgx::grid_fft_supported(plan->grid_size))
+        return IDGX_ERR_UNSUPPORTED;
+    return to_status(idgx::launch_grid_fft(d_grid, plan->grid_size, nr_layers, direction == IDGX_FFT_FORWARD,
+                                           static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int idgx_wstack_to_image(const idgx_plan *plan, const void *d_layers, int z_min, int nr_layers,
+                                    void *d_image, void *stream)
+{
+    const int rc = check_plan_struct(plan);
+    if (rc != IDGX_OK)
+        return rc;
+    if (nr_layers < 1 || !d_layers || !d_image || d_layers == d_image)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    return to_status(idgx::launch_wstack(d_layers, d_image, plan->grid_size, z_min, nr_layers, plan->image_size,
+                                         plan->w_step_in_lambda, true, static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int idgx_wstack_from_image(const idgx_plan *plan, const void *d_image, int z_min, int nr_layers,
+                                      void *d_layers, void *stream)
+{
+    const int rc = check_plan_struct(plan);
+    if (rc != IDGX_OK)
+        return rc;
+    if (nr_layers < 1 || !d_layers || !d_image || d_layers == d_image)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    return to_status(idgx::launch_wstack(d_image, d_layers, plan->grid_size, z_min, nr_layers, plan->image_size,
+                                         plan->w_step_in_lambda, false, static_cast<hipStream_t>(stream)));
 }
 
 extern "C" const char *idgx_strerror(int status)

@@ -1,6 +1,7 @@
 // Adder (uv subgrids onto the master grid) and splitter (master grid into uv subgrids) for gfx950.
 //
-// Grid layout: complex64 [4][G][G] (one plane per correlation).  Subgrid s covers grid cells
+// Grid layout: complex64 [4][G][G] (one plane per correlation), or [nr_layers][4][G][G] for w-stacking, where
+// subgrid s belongs to layer metadata[s].z - z_min.  Subgrid s covers grid cells
 // [y0, y0 + S) x [x0, x0 + S) with (x0, y0) = metadata[s].coordinate; cells outside the grid are skipped by the
 // adder and read as zero by the splitter.
 //
@@ -39,7 +40,8 @@ __host__ __device__ inline int adder_rows(int T)
 
 __global__ __launch_bounds__(kThreads) void adder_kernel(const idgx_metadata *md, const int32_t *order,
                                                          const int32_t *tile_start, int tile_begin, int S, int T,
-                                                         int G, const float2 *subgrids, float2 *grid)
+                                                         int G, int z_min, int nr_layers, const float2 *subgrids,
+                                                         float2 *grid)
 {
     const int R = adder_rows(T), nrb = (T + R - 1) / R;
     const int per_tile = 16 * nrb;
@@ -48,6 +50,10 @@ __global__ __launch_bounds__(kThreads) void adder_kernel(const idgx_metadata *md
     const int pol = sub_b / (4 * nrb), quad = (sub_b / nrb) & 3, rb = sub_b % nrb;
     const int first = tile_start[tile], last = tile_start[tile + 1];
     const idgx_metadata m0 = md[order[first]];
+    // All subgrids of a tile share one w-layer (the plan keys tiles by layer); single-layer plans ignore z.
+    const int layer = nr_layers > 1 ? m0.z - z_min : 0;
+    if (layer < 0 || layer >= nr_layers)
+        return; // uniform; the layered plan builder rejects such subgrids
     // Cells [by, by + nrows) x [bx, bx + T) of the grid.  The tile origin is taken from the tile index as the plan
     // keys it (clamped to -1 .. G/T + 1), so every subgrid of the tile agrees on it.
     const int ntiles_axis = floordiv_dev(G, T) + 3;
@@ -111,25 +117,29 @@ __global__ __launch_bounds__(kThreads) void adder_kernel(const idgx_metadata *md
                 }
     }
     const size_t GG = static_cast<size_t>(G) * G;
+    float2 *plane = grid + (static_cast<size_t>(layer) * kCorr + pol) * GG;
 #pragma unroll
     for (int k = 0; k < kAdderCells; ++k)
     {
         if (touched[k] && cx[k] >= 0 && cx[k] < G && cy[k] >= 0 && cy[k] < G)
         {
-            float2 *g = grid + pol * GG + static_cast<size_t>(cy[k]) * G + cx[k];
+            float2 *g = plane + static_cast<size_t>(cy[k]) * G + cx[k];
             const float2 o = *g;
             *g = make_float2(o.x + acc[k].x, o.y + acc[k].y);
         }
     }
 }
 
-__global__ __launch_bounds__(kThreads) void splitter_kernel(const idgx_metadata *md, int S, int G, const float2 *grid,
-                                                            float2 *subgrids)
+__global__ __launch_bounds__(kThreads) void splitter_kernel(const idgx_metadata *md, int S, int G, int z_min,
+                                                            int nr_layers, const float2 *grid, float2 *subgrids)
 {
     const long long s = blockIdx.x;
     const int x0 = md[s].x, y0 = md[s].y;
     const int P = S * S;
     const size_t GG = static_cast<size_t>(G) * G;
+    const int layer = nr_layers > 1 ? md[s].z - z_min : 0;
+    const bool on = layer >= 0 && layer < nr_layers; // uniform: a subgrid of no layer reads zeros
+    grid += static_cast<size_t>(on ? layer : 0) * kCorr * GG;
     float2 *sub = subgrids + static_cast<size_t>(s) * kCorr * P;
     for (int q = threadIdx.x; q < kCorr * P; q += kThreads)
     {
@@ -137,7 +147,7 @@ __global__ __launch_bounds__(kThreads) void splitter_kernel(const idgx_metadata 
         const int y = rem / S, x = rem - y * S;
         const int gy = y0 + y, gx = x0 + x;
         float2 v = make_float2(0.F, 0.F);
-        if (gy >= 0 && gy < G && gx >= 0 && gx < G)
+        if (on && gy >= 0 && gy < G && gx >= 0 && gx < G)
             v = grid[pol * GG + static_cast<size_t>(gy) * G + gx];
         sub[q] = v;
     }
@@ -145,8 +155,8 @@ __global__ __launch_bounds__(kThreads) void splitter_kernel(const idgx_metadata 
 } // namespace
 
 hipError_t launch_adder(const idgx_metadata *md, const int32_t *order, const int32_t *tile_start,
-                        const int32_t *color_begin, int S, int T, int G, const void *subgrids, void *grid,
-                        hipStream_t st)
+                        const int32_t *color_begin, int S, int T, int G, int z_min, int nr_layers,
+                        const void *subgrids, void *grid, hipStream_t st)
 {
     if (T < S || T <= 0)
         return hipErrorInvalidValue;
@@ -160,7 +170,7 @@ hipError_t launch_adder(const idgx_metadata *md, const int32_t *order, const int
         if (n * per_tile > 0x7fffffffLL)
             return hipErrorInvalidConfiguration;
         adder_kernel<<<static_cast<unsigned>(n * per_tile), kThreads, 0, st>>>(
-            md, order, tile_start, color_begin[c], S, T, G, static_cast<const float2 *>(subgrids),
+            md, order, tile_start, color_begin[c], S, T, G, z_min, nr_layers, static_cast<const float2 *>(subgrids),
             static_cast<float2 *>(grid));
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess)
@@ -169,13 +179,13 @@ hipError_t launch_adder(const idgx_metadata *md, const int32_t *order, const int
     return hipSuccess;
 }
 
-hipError_t launch_splitter(const idgx_metadata *md, long long nr_subgrids, int S, int G, const void *grid,
-                           void *subgrids, hipStream_t st)
+hipError_t launch_splitter(const idgx_metadata *md, long long nr_subgrids, int S, int G, int z_min, int nr_layers,
+                           const void *grid, void *subgrids, hipStream_t st)
 {
     if (nr_subgrids > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
     splitter_kernel<<<static_cast<unsigned>(nr_subgrids), kThreads, 0, st>>>(
-        md, S, G, static_cast<const float2 *>(grid), static_cast<float2 *>(subgrids));
+        md, S, G, z_min, nr_layers, static_cast<const float2 *>(grid), static_cast<float2 *>(subgrids));
     return hipGetLastError();
 }
 } // namespace idgx

@@ -105,8 +105,13 @@ int queue_workgroups(int precision, bool gridder, int shape); // resident workgr
 bool subgrid_fft_supported(int S);
 hipError_t launch_subgrid_fft(void *subgrids, long long nr_subgrids, int S, bool forward, hipStream_t stream);
 hipError_t launch_adder(const idgx_metadata *md, const int32_t *order, const int32_t *tile_start,
-                        const int32_t *color_begin, int S, int T, int G, const void *subgrids, void *grid,
-                        hipStream_t st);
-hipError_t launch_splitter(const idgx_metadata *md, long long nr_subgrids, int S, int G, const void *grid,
-                           void *subgrids, hipStream_t st);
+                        const int32_t *color_begin, int S, int T, int G, int z_min, int nr_layers,
+                        const void *subgrids, void *grid, hipStream_t st);
+hipError_t launch_splitter(const idgx_metadata *md, long long nr_subgrids, int S, int G, int z_min, int nr_layers,
+                           const void *grid, void *subgrids, hipStream_t st);
+
+bool grid_fft_supported(int G);
+hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward, hipStream_t st);
+hipError_t launch_wstack(const void *src, void *dst, int G, int z_min, int nr_layers, float image_size,
+                         float w_step, bool to_image, hipStream_t st);
 } // namespace idgx

@@ -81,7 +81,8 @@ FFT_BACKWARD = 1
 class AdderPlan(ctypes.Structure):
     """idgx_adder_plan (include/idgx.h)."""
 
-    _fields_ = [("tile_size", ctypes.c_int32), ("nr_tiles", ctypes.c_int32), ("color_begin", ctypes.c_int32 * 5)]
+    _fields_ = [("tile_size", ctypes.c_int32), ("nr_tiles", ctypes.c_int32), ("color_begin", ctypes.c_int32 * 5),
+                ("z_min", ctypes.c_int32), ("nr_layers", ctypes.c_int32)]
 
 
 STAGE_GRID = 1
@@ -133,6 +134,12 @@ def lib():
         L.idgx_subgrid_fft.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, vp]
         L.idgx_adder_plan_build.argtypes = [ctypes.POINTER(Plan), vp, ctypes.POINTER(AdderPlan), vp, vp]
         L.idgx_add_subgrids.argtypes = [ctypes.POINTER(Plan), ctypes.POINTER(AdderPlan)] + [vp] * 6
+        L.idgx_adder_plan_build_layered.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.POINTER(AdderPlan), vp, vp]
+        L.idgx_split_subgrids_layered.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
+        L.idgx_grid_fft.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp]
+        L.idgx_wstack_to_image.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.idgx_wstack_from_image.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.idgx_split_subgrids.argtypes = [ctypes.POINTER(Plan)] + [vp] * 4
         L.idgx_batch_create.argtypes = [ctypes.POINTER(BatchClass), ctypes.c_int, ctypes.POINTER(vp)]
         L.idgx_batch_destroy.argtypes = [vp]
@@ -245,6 +252,20 @@ def adder_plan_build(plan, metadata):
     return ap, order[:n], tile_start[:ap.nr_tiles + 1]
 
 
+def adder_plan_build_layered(plan, metadata, z_min, nr_layers):
+    """idgx_adder_plan_build_layered: the adder order for w-stacking onto nr_layers grids (layer = z - z_min)."""
+    n = int(plan.nr_subgrids)
+    ap = AdderPlan()
+    order = np.zeros(max(n, 1), np.int32)
+    tile_start = np.zeros(n + 1, np.int32)
+    rc = lib().idgx_adder_plan_build_layered(ctypes.byref(plan), _hptr(np.ascontiguousarray(metadata)),
+                                             int(z_min), int(nr_layers), ctypes.byref(ap), _hptr(order),
+                                             _hptr(tile_start))
+    if rc != OK:
+        raise IdgxError(rc, "idgx_adder_plan_build_layered")
+    return ap, order[:n], tile_start[:ap.nr_tiles + 1]
+
+
 def add_subgrids(plan, adder_plan, d_metadata, d_order, d_tile_start, d_subgrids, d_grid, stream=None):
     rc = lib().idgx_add_subgrids(ctypes.byref(plan), ctypes.byref(adder_plan), _dptr(d_metadata), _dptr(d_order),
                                  _dptr(d_tile_start), _dptr(d_subgrids), _dptr(d_grid), _stream(stream))
@@ -257,6 +278,28 @@ def split_subgrids(plan, d_metadata, d_grid, d_subgrids, stream=None):
                                    _stream(stream))
     if rc != OK:
         raise IdgxError(rc, "idgx_split_subgrids")
+
+
+def split_subgrids_layered(plan, d_metadata, z_min, nr_layers, d_grid, d_subgrids, stream=None):
+    _check(lib().idgx_split_subgrids_layered(ctypes.byref(plan), _dptr(d_metadata), int(z_min), int(nr_layers),
+                                             _dptr(d_grid), _dptr(d_subgrids), _stream(stream)),
+           "idgx_split_subgrids_layered")
+
+
+def grid_fft(plan, d_grid, nr_layers, direction, stream=None):
+    """idgx_grid_fft: in-place centred 2-D transform of a master grid complex64 [nr_layers][4][G][G]."""
+    _check(lib().idgx_grid_fft(ctypes.byref(plan), _dptr(d_grid), int(nr_layers), int(direction), _stream(stream)),
+           "idgx_grid_fft")
+
+
+def wstack_to_image(plan, d_layers, z_min, nr_layers, d_image, stream=None):
+    _check(lib().idgx_wstack_to_image(ctypes.byref(plan), _dptr(d_layers), int(z_min), int(nr_layers),
+                                      _dptr(d_image), _stream(stream)), "idgx_wstack_to_image")
+
+
+def wstack_from_image(plan, d_image, z_min, nr_layers, d_layers, stream=None):
+    _check(lib().idgx_wstack_from_image(ctypes.byref(plan), _dptr(d_image), int(z_min), int(nr_layers),
+                                        _dptr(d_layers), _stream(stream)), "idgx_wstack_from_image")
 
 
 def _check(rc, what):

@@ -5,7 +5,8 @@ torch provides the HBM allocations and the stream (plumbing only); every result 
 import numpy as np
 
 from . import METADATA_DTYPE, CHANNELS_DTYPE, ATERM_ADJOINT, PLAIN_FP32, make_plan, check_plan, grid, degrid, \
-    IdgxError, OK, FFT_FORWARD, FFT_BACKWARD, subgrid_fft, adder_plan_build, add_subgrids, split_subgrids
+    IdgxError, OK, FFT_FORWARD, FFT_BACKWARD, subgrid_fft, adder_plan_build, add_subgrids, split_subgrids, \
+    adder_plan_build_layered, split_subgrids_layered, grid_fft, wstack_to_image, wstack_from_image
 
 
 def _torch():
@@ -99,9 +100,47 @@ class DeviceWorkload:
     def split(self, plan, d_grid, d_subgrids=None, stream=None):
         split_subgrids(plan, self.d_metadata, d_grid, self.d_subgrids if d_subgrids is None else d_subgrids, stream)
 
-    def grid_array(self, d_grid):
+    def grid_array(self, d_grid, nr_layers=1):
         G = int(self.w["grid_size"])
-        return download(d_grid, np.complex64, (4, G, G))
+        shape = (4, G, G) if nr_layers == 1 else (nr_layers, 4, G, G)
+        return download(d_grid, np.complex64, shape)
+
+    # ---- w-stacking: layers z_min .. z_min + nr_layers - 1 of the plan's metadata z ------------------------------
+    def layers(self):
+        """(z_min, nr_layers) spanned by the plan's w-layers."""
+        z = self.w["metadata"]["z"]
+        return (int(z.min()), int(z.max()) - int(z.min()) + 1) if z.size else (0, 1)
+
+    def layer_buffer(self, nr_layers):
+        """A zeroed complex64 [nr_layers][4][G][G] stack of master grids in HBM (flat bytes)."""
+        torch = _torch()
+        G = int(self.w["grid_size"])
+        return torch.zeros(nr_layers * 4 * G * G * 8, dtype=torch.uint8, device=self.device)
+
+    def add_layered(self, plan, d_layers, d_subgrids=None, stream=None):
+        z_min, nl = self.layers()
+        if getattr(self, "_adder_layered", None) is None:
+            ap, order, tile_start = adder_plan_build_layered(plan, self.w["metadata"], z_min, nl)
+            self._adder_layered = (ap, upload(order, self.device), upload(tile_start, self.device))
+        ap, d_order, d_tile_start = self._adder_layered
+        add_subgrids(plan, ap, self.d_metadata, d_order, d_tile_start,
+                     self.d_subgrids if d_subgrids is None else d_subgrids, d_layers, stream)
+
+    def split_layered(self, plan, d_layers, d_subgrids=None, stream=None):
+        z_min, nl = self.layers()
+        split_subgrids_layered(plan, self.d_metadata, z_min, nl, d_layers,
+                               self.d_subgrids if d_subgrids is None else d_subgrids, stream)
+
+    def grid_fft(self, plan, d_grid, nr_layers, direction, stream=None):
+        grid_fft(plan, d_grid, nr_layers, direction, stream)
+
+    def to_image(self, plan, d_layers, d_image, stream=None):
+        z_min, nl = self.layers()
+        wstack_to_image(plan, d_layers, z_min, nl, d_image, stream)
+
+    def from_image(self, plan, d_image, d_layers, stream=None):
+        z_min, nl = self.layers()
+        wstack_from_image(plan, d_image, z_min, nl, d_layers, stream)
 
 
 def selection_lists(w, flags, chunk_entries=1 << 26):

@@ -136,12 +136,14 @@ def ragged_time_segments(rng, baselines, n_times, t_min, t_max, aterm_interval=N
 
 
 def plan_segments(uvw_m, frequencies, s1, s2, seg_bl, seg_t0, seg_t1, *, subgrid_size, grid_size, image_size,
-                  support=8, aterm_interval=None):
+                  support=8, aterm_interval=None, w_step=None):
     """IDG plan over time segments: segment i covers rows seg_bl[i] * n_times + [seg_t0[i], seg_t1[i]) (rows laid
     out as row = baseline * n_times + t); a segment must not cross an A-term slot boundary.  Its channels are
     grouped greedily so that the uv footprint of every (timestep, channel) sample of a group spans at most
     S - support grid cells in u and in v; each group becomes one subgrid centred on its footprint, dropped if it
-    would leave the grid.  Returns (metadata, channels) sorted by (first row, first channel)."""
+    would leave the grid.  With w_step (wavelengths), the subgrid's w-layer z = floor(w / w_step) puts its mean w
+    (over its timesteps, at its centre frequency) within w_step / 2 of its phase reference (z + 0.5) w_step.
+    Returns (metadata, channels) sorted by (first row, first channel)."""
     n_bl, n_times, _ = uvw_m.shape
     S, G = subgrid_size, grid_size
     width = S - support
@@ -159,6 +161,7 @@ def plan_segments(uvw_m, frequencies, s1, s2, seg_bl, seg_t0, seg_t1, *, subgrid
 
     umin, umax = seg_reduce(np.minimum, 0), seg_reduce(np.maximum, 0)
     vmin, vmax = seg_reduce(np.minimum, 1), seg_reduce(np.maximum, 1)
+    wmean = seg_reduce(np.add, 2) / np.maximum(seg_t1 - seg_t0, 1) if w_step else None
 
     def footprint(i, a, z):
         # grid-cell extent of the samples of segments i with channels in [a, z]
@@ -202,6 +205,9 @@ def plan_segments(uvw_m, frequencies, s1, s2, seg_bl, seg_t0, seg_t1, *, subgrid
     md["x"] = x0
     md["y"] = y0
     md["z"] = 0
+    if w_step:
+        f_c = 0.5 * (frequencies[ca] + frequencies[cb])
+        md["z"] = np.floor(wmean[seg] * f_c / SPEED_OF_LIGHT / w_step).astype(np.int64)
     ch = np.zeros(seg.size, CHANNELS_DTYPE)
     ch["channel_begin"] = ca
     ch["nr_channels"] = cb - ca + 1
@@ -210,12 +216,12 @@ def plan_segments(uvw_m, frequencies, s1, s2, seg_bl, seg_t0, seg_t1, *, subgrid
 
 
 def plan_subgrids(uvw_m, frequencies, s1, s2, *, subgrid_size, grid_size, image_size, timesteps_per_subgrid,
-                  support=8, aterm_interval=None):
+                  support=8, aterm_interval=None, w_step=None):
     """IDG plan with the same time chunks for every baseline: returns (metadata, channels) for rows laid out as
     row = baseline * n_times + t."""
     bl, t0, t1 = time_segments(uvw_m.shape[0], uvw_m.shape[1], timesteps_per_subgrid, aterm_interval)
     return plan_segments(uvw_m, frequencies, s1, s2, bl, t0, t1, subgrid_size=subgrid_size, grid_size=grid_size,
-                         image_size=image_size, support=support, aterm_interval=aterm_interval)
+                         image_size=image_size, support=support, aterm_interval=aterm_interval, w_step=w_step)
 
 
 # name -> parameters of the seeded synthetic configurations
@@ -235,6 +241,11 @@ CONFIGS = {
                        f_lo=150e6, f_hi=165e6, n_channels=16, subgrid_size=64, timesteps_per_subgrid=128,
                        grid_size=16384, image_size=0.1, dec_deg=-30.0, zero_w=False, aterm_interval=64,
                        jones=True, seed=6),
+    # W-stacking: the small layout pointed 27 degrees from zenith (|w| up to ~500 wavelengths), 32-timestep
+    # subgrids, w-layers 100 wavelengths apart.
+    "wstack": dict(n_stations=10, radius_m=1000.0, core_fraction=0.0, core_radius_m=0.0, n_times=128,
+                   f_lo=150e6, f_hi=165e6, n_channels=16, subgrid_size=32, timesteps_per_subgrid=32,
+                   grid_size=2048, image_size=0.1, dec_deg=0.0, zero_w=False, w_step=100.0, seed=7),
 }
 
 # Batched uneven workloads (make_batched_observation): baselines split into length terciles with subgrid sizes
@@ -270,7 +281,7 @@ def make_observation(name=None, *, visibilities=True, **overrides):
     S = cfg["subgrid_size"]
     md, ch = plan_subgrids(uvw32.astype(np.float64), freqs, s1, s2, subgrid_size=S, grid_size=cfg["grid_size"],
                            image_size=cfg["image_size"], timesteps_per_subgrid=cfg["timesteps_per_subgrid"],
-                           aterm_interval=cfg.get("aterm_interval"))
+                           aterm_interval=cfg.get("aterm_interval"), w_step=cfg.get("w_step"))
     n_rows = len(s1) * cfg["n_times"]
     n_slots = -(-cfg["n_times"] // cfg["aterm_interval"]) if cfg.get("aterm_interval") else 1
     aterms = jones_aterms(np.random.default_rng(cfg["seed"] + 7), n_slots, cfg["n_stations"], S) \
@@ -290,7 +301,7 @@ def make_observation(name=None, *, visibilities=True, **overrides):
         nr_stations=cfg["n_stations"],
         nr_aterm_slots=n_slots,
         image_size=np.float32(cfg["image_size"]),
-        w_step=0.0,
+        w_step=float(cfg.get("w_step") or 0.0),
         max_nr_timesteps=int(md["nr_timesteps"].max()) if md.size else 0,
         nr_rows=n_rows,
         nr_visibilities=int((md["nr_timesteps"].astype(np.int64) * ch["nr_channels"]).sum()),

@@ -153,3 +153,30 @@ def test_adder_plan_is_conflict_free_and_complete():
             others = np.delete(boxes, i, axis=0)
             overlap = (others[:, 0] < x1) & (x0 < others[:, 1]) & (others[:, 2] < y1) & (y0 < others[:, 3])
             assert not overlap.any()
+
+
+def test_master_grid_and_layered_entry_points_check_arguments_on_the_host():
+    """Argument checks that return before any device work: grid FFT sizes, layered adder plan z range."""
+    import ctypes
+
+    import idgx
+    from idgx import METADATA_DTYPE
+
+    md = np.zeros(3, METADATA_DTYPE)
+    md["nr_timesteps"] = 1
+    md["station2"] = 1
+    md["z"] = [0, 2, 5]
+    plan = idgx.make_plan(nr_subgrids=3, subgrid_size=16, grid_size=130, nr_channels=1, nr_stations=2,
+                          nr_aterm_slots=1, nr_rows=3, max_nr_timesteps=1, image_size=0.1)
+    L = idgx.lib()
+    dummy = ctypes.c_void_p(1)
+    assert L.idgx_grid_fft(ctypes.byref(plan), dummy, 1, idgx.FFT_FORWARD, None) == idgx.ERR_UNSUPPORTED  # 130 % 4
+    plan.grid_size = 128
+    assert L.idgx_grid_fft(ctypes.byref(plan), dummy, 0, idgx.FFT_FORWARD, None) == idgx.ERR_INVALID_ARGUMENT
+    assert L.idgx_grid_fft(ctypes.byref(plan), dummy, 1, 7, None) == idgx.ERR_INVALID_ARGUMENT
+    with pytest.raises(idgx.IdgxError):
+        idgx.adder_plan_build_layered(plan, md, 0, 5)  # z = 5 is outside layers 0 .. 4
+    ap, order, tile_start = idgx.adder_plan_build_layered(plan, md, 0, 6)
+    assert (ap.z_min, ap.nr_layers, ap.nr_tiles) == (0, 6, 3) and sorted(order) == [0, 1, 2]
+    ap1, _, _ = idgx.adder_plan_build(plan, md)
+    assert (ap1.nr_layers, ap1.nr_tiles) == (1, 1)  # single grid: z ignored, one tile
