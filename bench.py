@@ -16,6 +16,11 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ska-spc-tender-benchmarks_amd"))
+# The CPUs this process may use, taken before any OpenMP runtime starts: with OMP_PROC_BIND set, the runtime pins
+# the main thread to one of them, and a child process would inherit that single CPU.
+HOST_CPUS = sorted(os.sched_getaffinity(0))
+# CPU baseline threads stay on neighbouring cores.
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 import numpy as np  # noqa: E402
 
@@ -55,8 +60,23 @@ def algorithmic_bytes(md, ch, S, C_all, n_slots, n_stations, direction):
     return b
 
 
-def cpu_baseline(w, every, nthreads):
-    """Oracle FP32 restatement (same loop nest / OpenMP structure as the reference CPU path) on every k-th subgrid."""
+def cpu_model():
+    """Host CPU model name and the number of logical CPUs this process may use."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, len(HOST_CPUS)
+
+
+def cpu_baseline(w, every, nthreads, repeats=3):
+    """Oracle FP32 restatement (same loop nest / OpenMP structure as the reference CPU path) on every k-th subgrid;
+    median of `repeats` runs per direction."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from idgx import observation
@@ -64,13 +84,40 @@ def cpu_baseline(w, every, nthreads):
 
     idx = np.arange(0, w["nr_subgrids"], every)
     sw = observation.subset(w, idx)
-    t0 = time.perf_counter()
-    sub = oracle_grid(sw, nthreads=nthreads)
-    t1 = time.perf_counter()
-    oracle_degrid(sw, sub, nthreads=nthreads)
-    t2 = time.perf_counter()
+    tg, td = [], []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        sub = oracle_grid(sw, nthreads=nthreads)
+        t1 = time.perf_counter()
+        oracle_degrid(sw, sub, nthreads=nthreads)
+        t2 = time.perf_counter()
+        tg.append(t1 - t0)
+        td.append(t2 - t1)
     nvis = sw["nr_visibilities"]
-    return dict(grid_s=t1 - t0, degrid_s=t2 - t1, nvis=nvis, nsub=idx.size)
+    return dict(grid_s=float(np.median(tg)), degrid_s=float(np.median(td)), nvis=nvis, nsub=idx.size)
+
+
+def reference_as_shipped(nthreads):
+    """The reference CPU launchers themselves, compiled from its sources as shipped (-g, i.e. -O0; oracle/_ref),
+    on the reference generator's workload of the small configuration's shape (10 stations, 1 time slot, 128
+    timesteps, 16 channels, S=32, G=1024: 92,160 visibilities).  None when the binary was not built."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.path.exists(exe):
+        return None
+    env = dict(os.environ, OMP_NUM_THREADS=str(nthreads), OMP_PROC_BIND="close")
+    out = subprocess.run([exe, "-", "10", "1", "128", "16", "32", "1024"], env=env, capture_output=True, text=True,
+                         timeout=300, check=True, preexec_fn=lambda: os.sched_setaffinity(0, HOST_CPUS)
+                         ).stdout.strip().splitlines()[-1]
+    t = json.loads(out)
+    nvis = 45 * 128 * 16
+    tg, td = t["gridder_us"] / 1e6, t["degridder_us"] / 1e6
+    return {"value": nvis / (tg + td) / 1e6, "unit": "Mvis/s", "cores": nthreads, "kind": "reference",
+            "sample": f"reference gridding/src/{{gridding,degridding}}_benchmark.cpp CPU launchers as shipped (-g), "
+                      f"reference generator srand(0), 10 stations x 1 slot x 128 timesteps x 16 channels, S=32, "
+                      f"G=1024 ({nvis} visibilities); gridder {nvis / tg / 1e6:.4f} Mvis/s, degridder "
+                      f"{nvis / td / 1e6:.4f} Mvis/s"}
 
 
 def kernel_key(precision, path, direction, shape=None):
@@ -536,18 +583,24 @@ def main():
     cpu = None
     if not args.no_cpu:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        every = args.cpu_every or max(1, w["nr_subgrids"] // 2560)
+        every = args.cpu_every or max(1, w["nr_subgrids"] // 850)
         r = cpu_baseline(dict(w, visibilities=host_vis), every, nthreads)
+        model, ncpu = cpu_model()
         cpu = {
             "value": r["nvis"] / (r["grid_s"] + r["degrid_s"]) / 1e6,
             "unit": "Mvis/s",
             "cores": nthreads,
             "kind": "port",
-            "sample": f"every {every}th subgrid of the plan ({r['nsub']} subgrids, {r['nvis']} visibilities); "
-                      f"gridder {r['nvis'] / r['grid_s'] / 1e6:.3f} Mvis/s, degridder "
+            "sample": f"every {every}th subgrid of the plan ({r['nsub']} subgrids, {r['nvis']} visibilities), median "
+                      f"of 3 runs: gridder {r['nvis'] / r['grid_s'] / 1e6:.3f} Mvis/s, degridder "
                       f"{r['nvis'] / r['degrid_s'] / 1e6:.3f} Mvis/s; oracle FP32 restatement of the reference "
-                      f"OpenMP CPU path (same loop nest and OpenMP structure), -O3 -ffp-contract=off",
+                      f"OpenMP CPU path (same loop nest and OpenMP structure), -O3 -ffp-contract=off, "
+                      f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}",
+            "host_cpu": f"{model}, {ncpu} logical CPUs available",
         }
+        ref = reference_as_shipped(nthreads)
+        if ref is not None:
+            cpu["reference_as_shipped"] = ref
 
     out = {
         "metric": "Mvis/s gridder+degridder, 1 MI355X: FP32 vs MFMA mixed at rel-err<=5e-5",

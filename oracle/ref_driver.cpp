@@ -47,8 +47,12 @@ namespace gridding_benchmark
     }
 } // namespace gridding_benchmark
 
+// out_dir "-" times the reference launchers only: nothing is written, one line with their compute times (the
+// benchmark_result.compute_time of gridding/inc/common_gridding.h:5-12, microseconds) goes to stdout.
 static void dump(const std::string &dir, const char *name, const void *data, size_t bytes)
 {
+    if (dir == "-")
+        return;
     const std::string path = dir + "/" + name;
     FILE *f = std::fopen(path.c_str(), "wb");
     if (!f || std::fwrite(data, 1, bytes, f) != bytes)
@@ -119,13 +123,15 @@ int main(int argc, char **argv)
     c.subgrid_size = std::stoul(argv[6]);
     c.grid_size = std::stoul(argv[7]);
 
+    double grid_us = 0.0, degrid_us = 0.0;
     // Gridder: inputs as launch_gridding builds them, output subgrids.
     {
         Inputs in(c);
         c.operation = gb::operation_type::gridding;
-        gb::gridding_benchmark_launcher<hardware_type::cpu>::launch(c, in.uvw, in.visibilities, in.baselines,
-                                                                     in.aterms, in.frequencies, in.wavenumbers,
-                                                                     in.spheroidal, in.subgrids, in.metadata);
+        grid_us = gb::gridding_benchmark_launcher<hardware_type::cpu>::launch(
+                      c, in.uvw, in.visibilities, in.baselines, in.aterms, in.frequencies, in.wavenumbers,
+                      in.spheroidal, in.subgrids, in.metadata)
+                      .compute_time;
         dump(dir, "uvw.bin", in.uvw.data(), in.uvw.size() * sizeof(in.uvw[0]));
         dump(dir, "frequencies.bin", in.frequencies.data(), in.frequencies.size() * sizeof(float));
         dump(dir, "wavenumbers.bin", in.wavenumbers.data(), in.wavenumbers.size() * sizeof(float));
@@ -144,11 +150,14 @@ int main(int argc, char **argv)
         gb::initialize_subgrids(in.subgrids);
         dump(dir, "degrid_subgrids.bin", in.subgrids.data(), in.subgrids.size() * sizeof(in.subgrids[0]));
         c.operation = gb::operation_type::degridding;
-        gb::degridding_benchmark_launcher<hardware_type::cpu>::launch(c, in.uvw, in.visibilities, in.baselines,
-                                                                       in.aterms, in.frequencies, in.wavenumbers,
-                                                                       in.spheroidal, in.subgrids, in.metadata);
+        degrid_us = gb::degridding_benchmark_launcher<hardware_type::cpu>::launch(
+                        c, in.uvw, in.visibilities, in.baselines, in.aterms, in.frequencies, in.wavenumbers,
+                        in.spheroidal, in.subgrids, in.metadata)
+                        .compute_time;
         dump(dir, "degrid_visibilities.bin", in.visibilities.data(),
              in.visibilities.size() * sizeof(in.visibilities[0]));
     }
+    if (dir == "-")
+        std::printf("{\"gridder_us\": %.3f, \"degridder_us\": %.3f}\n", grid_us, degrid_us);
     return 0;
 }
