@@ -374,6 +374,18 @@ def run_batched_config(args):
         f, sc, v = flop_model_selected(c["metadata"], h_off, int(c["subgrid_size"]))
         flops, sincos, nvis = flops + f, sincos + sc, nvis + v
     torch.cuda.synchronize()
+    # The plan again on the device (idgx_plan_segments): bit-identical to the host planner, and its time.
+    from idgx import plan_segments_device
+    planner = {"host_s": 0.0, "device_s": 0.0, "bit_identical": True}
+    for c in bo["classes"]:
+        t0 = time.perf_counter()
+        md_d, ch_d = plan_segments_device(bo["uvw"], c["frequencies64"], *c["stations"], *c["segments"],
+                                          subgrid_size=int(c["subgrid_size"]), grid_size=cfg["grid_size"],
+                                          image_size=cfg["image_size"], nr_times=cfg["n_times"])
+        planner["device_s"] += time.perf_counter() - t0
+        planner["host_s"] += c["host_plan_s"]
+        planner["bit_identical"] &= md_d.tobytes() == c["metadata"].tobytes() and \
+            ch_d.tobytes() == c["channels"].tobytes()
     n_sub = sum(int(c["nr_subgrids"]) for c in bo["classes"])
     window = n_bl * cfg["n_times"] * C
     print(f"bench: {args.config}: {n_sub} subgrids in {len(bo['classes'])} classes "
@@ -473,6 +485,7 @@ def run_batched_config(args):
         },
         "speedup_vs_plain_fp32": None,
         "forms": forms,
+        "planner": dict(planner, note="device time includes uploading the segments and downloading the plan"),
         "gflops_per_pass": flops / 1e9,
         "gsincos_per_pass": sincos / 1e9,
         "roofline": {

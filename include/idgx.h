@@ -201,6 +201,42 @@ int idgx_wstack_to_image(const idgx_plan *plan, const void *d_layers, int z_min,
 int idgx_wstack_from_image(const idgx_plan *plan, const void *d_image, int z_min, int nr_layers, void *d_layers,
                            void *stream);
 
+/* ---- Device-side planner (no reference counterpart: the reference draws random metadata,
+ * gridding/src/gridding_initialization.cpp:158-182) -------------------------------------------------------------
+ *
+ * Turns time segments of baselines into subgrid metadata and channel windows on the GPU.  Segment i covers rows
+ * seg_baseline[i] * nr_times + [seg_t0[i], seg_t1[i]) of uvw (float [rows][3], metres); a segment must not cross an
+ * A-term slot boundary and must not be empty.  Its channels are grouped greedily: a group closes when adding the
+ * next channel would make the uv footprint of its samples (metres x frequency / c x image_size grid cells) wider
+ * than subgrid_size - support cells in u or v.  Each group becomes a subgrid centred on its footprint, dropped if it
+ * would leave the grid; with w_step_in_lambda > 0, its w-layer is z = floor(middle of the segment's w range x the
+ * group's centre frequency / c / w_step).  Arithmetic is FP64 without contraction, so the plan is bit-identical
+ * to the package's host planner.  Subgrids are written in segment order, each segment's groups in channel order
+ * (sorted by first row and first channel when the segments are sorted by baseline and t0). */
+typedef struct
+{
+    int32_t subgrid_size;
+    int32_t grid_size;
+    int32_t support;        /* cells of the subgrid kept free of samples (taper support) */
+    int32_t nr_times;       /* rows per baseline */
+    int32_t nr_channels;
+    int32_t aterm_interval; /* timesteps per A-term slot; 0 = one slot */
+    double image_size;
+    double w_step_in_lambda; /* 0 = no w-layers */
+} idgx_planner;
+
+/* Bytes of device workspace idgx_plan_segments needs for nr_segments segments. */
+size_t idgx_plan_workspace_bytes(int64_t nr_segments);
+
+/* Writes at most `capacity` subgrids to d_metadata / d_channels and returns their total number in *h_nr_subgrids
+ * (call again with a larger capacity when it exceeds `capacity`).  d_frequencies are the channel frequencies in Hz
+ * (double [nr_channels]); d_station1 / d_station2 the stations of each baseline.  Synchronises `stream`. */
+int idgx_plan_segments(const idgx_planner *planner, const float *d_uvw, const double *d_frequencies,
+                       const uint32_t *d_station1, const uint32_t *d_station2, const int32_t *d_seg_baseline,
+                       const int32_t *d_seg_t0, const int32_t *d_seg_t1, int64_t nr_segments, void *d_workspace,
+                       idgx_metadata *d_metadata, idgx_channels *d_channels, int64_t capacity,
+                       int64_t *h_nr_subgrids, void *stream);
+
 /* ---- Batched execution: one persistent launch per direction over a whole plan ---------------------------------
  *
  * A batch holds up to 4 subgrid classes (one subgrid size each: mixed sizes run in the same launch).  Its

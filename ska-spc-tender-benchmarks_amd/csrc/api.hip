@@ -332,6 +332,37 @@ extern "C" int idgx_wstack_from_image(const idgx_plan *plan, const void *d_image
                                          plan->w_step_in_lambda, false, static_cast<hipStream_t>(stream)));
 }
 
+extern "C" size_t idgx_plan_workspace_bytes(int64_t nr_segments)
+{
+    return nr_segments < 0 ? 0 : idgx::plan_workspace_bytes(nr_segments);
+}
+
+extern "C" int idgx_plan_segments(const idgx_planner *pl, const float *d_uvw, const double *d_frequencies,
+                                  const uint32_t *d_station1, const uint32_t *d_station2,
+                                  const int32_t *d_seg_baseline, const int32_t *d_seg_t0, const int32_t *d_seg_t1,
+                                  int64_t nr_segments, void *d_workspace, idgx_metadata *d_metadata,
+                                  idgx_channels *d_channels, int64_t capacity, int64_t *h_nr_subgrids, void *stream)
+{
+    if (!pl || !h_nr_subgrids || nr_segments < 0 || capacity < 0 || pl->subgrid_size < 2 || pl->grid_size < 1 ||
+        pl->nr_times < 1 || pl->nr_channels < 1 || pl->aterm_interval < 0 || pl->support < 0 ||
+        pl->support >= pl->subgrid_size || !(pl->image_size > 0) || !(pl->w_step_in_lambda >= 0))
+        return IDGX_ERR_INVALID_ARGUMENT;
+    if (nr_segments > 0 && (!d_uvw || !d_frequencies || !d_station1 || !d_station2 || !d_seg_baseline || !d_seg_t0 ||
+                            !d_seg_t1 || !d_workspace || (capacity > 0 && (!d_metadata || !d_channels))))
+        return IDGX_ERR_INVALID_ARGUMENT;
+    long long count = 0;
+    if (nr_segments == 0)
+    {
+        *h_nr_subgrids = 0;
+        return IDGX_OK;
+    }
+    const int rc = to_status(idgx::launch_plan(*pl, d_uvw, d_frequencies, d_station1, d_station2, d_seg_baseline,
+                                               d_seg_t0, d_seg_t1, nr_segments, d_workspace, d_metadata, d_channels,
+                                               capacity, &count, static_cast<hipStream_t>(stream)));
+    *h_nr_subgrids = count;
+    return rc;
+}
+
 extern "C" const char *idgx_strerror(int status)
 {
     switch (status)

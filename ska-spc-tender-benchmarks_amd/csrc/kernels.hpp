@@ -110,6 +110,12 @@ hipError_t launch_adder(const idgx_metadata *md, const int32_t *order, const int
 hipError_t launch_splitter(const idgx_metadata *md, long long nr_subgrids, int S, int G, int z_min, int nr_layers,
                            const void *grid, void *subgrids, hipStream_t st);
 
+size_t plan_workspace_bytes(long long nr_segments);
+hipError_t launch_plan(const idgx_planner &pl, const float *uvw, const double *frequencies, const uint32_t *station1,
+                       const uint32_t *station2, const int32_t *seg_bl, const int32_t *seg_t0, const int32_t *seg_t1,
+                       long long nr_segments, void *workspace, idgx_metadata *metadata, idgx_channels *channels,
+                       long long capacity, long long *h_count, hipStream_t st);
+
 bool grid_fft_supported(int G);
 hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward, hipStream_t st);
 hipError_t launch_wstack(const void *src, void *dst, int G, int z_min, int nr_layers, float image_size,

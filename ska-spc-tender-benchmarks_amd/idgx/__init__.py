@@ -78,6 +78,14 @@ FFT_FORWARD = 0
 FFT_BACKWARD = 1
 
 
+class Planner(ctypes.Structure):
+    """idgx_planner (include/idgx.h)."""
+
+    _fields_ = [("subgrid_size", ctypes.c_int32), ("grid_size", ctypes.c_int32), ("support", ctypes.c_int32),
+                ("nr_times", ctypes.c_int32), ("nr_channels", ctypes.c_int32), ("aterm_interval", ctypes.c_int32),
+                ("image_size", ctypes.c_double), ("w_step_in_lambda", ctypes.c_double)]
+
+
 class AdderPlan(ctypes.Structure):
     """idgx_adder_plan (include/idgx.h)."""
 
@@ -138,6 +146,11 @@ def lib():
                                                     ctypes.POINTER(AdderPlan), vp, vp]
         L.idgx_split_subgrids_layered.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
         L.idgx_grid_fft.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp]
+        L.idgx_plan_workspace_bytes.argtypes = [ctypes.c_int64]
+        L.idgx_plan_workspace_bytes.restype = ctypes.c_size_t
+        L.idgx_plan_segments.argtypes = [ctypes.POINTER(Planner)] + [vp] * 7 + [ctypes.c_int64, vp, vp, vp,
+                                                                                ctypes.c_int64,
+                                                                                ctypes.POINTER(ctypes.c_int64), vp]
         L.idgx_wstack_to_image.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.idgx_wstack_from_image.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.idgx_split_subgrids.argtypes = [ctypes.POINTER(Plan)] + [vp] * 4
@@ -300,6 +313,39 @@ def wstack_to_image(plan, d_layers, z_min, nr_layers, d_image, stream=None):
 def wstack_from_image(plan, d_image, z_min, nr_layers, d_layers, stream=None):
     _check(lib().idgx_wstack_from_image(ctypes.byref(plan), _dptr(d_image), int(z_min), int(nr_layers),
                                         _dptr(d_layers), _stream(stream)), "idgx_wstack_from_image")
+
+
+def plan_segments_device(uvw, frequencies, station1, station2, seg_bl, seg_t0, seg_t1, *, subgrid_size, grid_size,
+                         image_size, nr_times, support=8, aterm_interval=0, w_step=0.0, device="cuda", stream=None):
+    """idgx_plan_segments: the subgrid plan computed on the GPU from host arrays (uvw float32 [rows][3] in metres,
+    frequencies float64 [C]); returns host (metadata, channels) arrays, bit-identical to
+    observation.plan_segments for segments sorted by (baseline, t0)."""
+    import torch
+
+    def up(a, dt):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=dt).reshape(-1)).to(device)
+
+    nseg = int(np.asarray(seg_bl).size)
+    pl = Planner(subgrid_size=int(subgrid_size), grid_size=int(grid_size), support=int(support),
+                 nr_times=int(nr_times), nr_channels=int(np.asarray(frequencies).size),
+                 aterm_interval=int(aterm_interval or 0), image_size=float(image_size),
+                 w_step_in_lambda=float(w_step or 0.0))
+    d = dict(uvw=up(uvw, np.float32), f=up(frequencies, np.float64),
+             s1=up(np.asarray(station1, np.uint32).view(np.int32), np.int32),
+             s2=up(np.asarray(station2, np.uint32).view(np.int32), np.int32),
+             bl=up(seg_bl, np.int32), t0=up(seg_t0, np.int32), t1=up(seg_t1, np.int32))
+    ws = torch.zeros(max(1, int(lib().idgx_plan_workspace_bytes(nseg))), dtype=torch.uint8, device=device)
+    cap = nseg * pl.nr_channels
+    md = torch.zeros(max(1, cap) * METADATA_DTYPE.itemsize, dtype=torch.uint8, device=device)
+    ch = torch.zeros(max(1, cap) * CHANNELS_DTYPE.itemsize, dtype=torch.uint8, device=device)
+    n = ctypes.c_int64(0)
+    _check(lib().idgx_plan_segments(ctypes.byref(pl), _dptr(d["uvw"]), _dptr(d["f"]), _dptr(d["s1"]), _dptr(d["s2"]),
+                                    _dptr(d["bl"]), _dptr(d["t0"]), _dptr(d["t1"]), nseg, _dptr(ws), _dptr(md),
+                                    _dptr(ch), cap, ctypes.byref(n), _stream(stream)), "idgx_plan_segments")
+    k = int(n.value)
+    mdh = md.cpu().numpy().view(METADATA_DTYPE)[:k].copy()
+    chh = ch.cpu().numpy().view(CHANNELS_DTYPE)[:k].copy()
+    return mdh, chh
 
 
 def _check(rc, what):

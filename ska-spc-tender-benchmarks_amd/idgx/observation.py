@@ -20,6 +20,7 @@ footprint; groups whose subgrid would leave the grid are dropped (their visibili
 counted in the throughput).
 """
 import math
+import time
 
 import numpy as np
 
@@ -141,8 +142,9 @@ def plan_segments(uvw_m, frequencies, s1, s2, seg_bl, seg_t0, seg_t1, *, subgrid
     out as row = baseline * n_times + t); a segment must not cross an A-term slot boundary.  Its channels are
     grouped greedily so that the uv footprint of every (timestep, channel) sample of a group spans at most
     S - support grid cells in u and in v; each group becomes one subgrid centred on its footprint, dropped if it
-    would leave the grid.  With w_step (wavelengths), the subgrid's w-layer z = floor(w / w_step) puts its mean w
-    (over its timesteps, at its centre frequency) within w_step / 2 of its phase reference (z + 0.5) w_step.
+    would leave the grid.  With w_step (wavelengths), the subgrid's w-layer z = floor(w / w_step) puts the middle of
+    its w range (over its timesteps, at its centre frequency) within w_step / 2 of its phase reference
+    (z + 0.5) w_step.
     Returns (metadata, channels) sorted by (first row, first channel)."""
     n_bl, n_times, _ = uvw_m.shape
     S, G = subgrid_size, grid_size
@@ -161,7 +163,7 @@ def plan_segments(uvw_m, frequencies, s1, s2, seg_bl, seg_t0, seg_t1, *, subgrid
 
     umin, umax = seg_reduce(np.minimum, 0), seg_reduce(np.maximum, 0)
     vmin, vmax = seg_reduce(np.minimum, 1), seg_reduce(np.maximum, 1)
-    wmean = seg_reduce(np.add, 2) / np.maximum(seg_t1 - seg_t0, 1) if w_step else None
+    wmin, wmax = (seg_reduce(np.minimum, 2), seg_reduce(np.maximum, 2)) if w_step else (None, None)
 
     def footprint(i, a, z):
         # grid-cell extent of the samples of segments i with channels in [a, z]
@@ -207,7 +209,8 @@ def plan_segments(uvw_m, frequencies, s1, s2, seg_bl, seg_t0, seg_t1, *, subgrid
     md["z"] = 0
     if w_step:
         f_c = 0.5 * (frequencies[ca] + frequencies[cb])
-        md["z"] = np.floor(wmean[seg] * f_c / SPEED_OF_LIGHT / w_step).astype(np.int64)
+        w_mid = 0.5 * (wmin[seg] + wmax[seg])
+        md["z"] = np.floor(w_mid * f_c / SPEED_OF_LIGHT / w_step).astype(np.int64)
     ch = np.zeros(seg.size, CHANNELS_DTYPE)
     ch["channel_begin"] = ca
     ch["nr_channels"] = cb - ca + 1
@@ -360,9 +363,12 @@ def make_batched_observation(name=None, *, visibilities=True, flags=True, **over
     classes = []
     for k, S in enumerate(cfg["subgrid_sizes"]):
         bl, t0, t1 = ragged_time_segments(seg_rng, np.sort(terciles[k]), n_times, *cfg["t_range"])
+        t_plan = time.perf_counter()
         md, ch = plan_segments(uvw64, freqs, s1, s2, bl, t0, t1, subgrid_size=S, grid_size=cfg["grid_size"],
                                image_size=cfg["image_size"])
+        t_plan = time.perf_counter() - t_plan
         c = dict(shared, config=cfg, metadata=md, channels=ch, nr_subgrids=md.size, subgrid_size=S,
+                 segments=(bl, t0, t1), host_plan_s=t_plan, frequencies64=freqs, stations=(s1, s2),
                  spheroidal=prolate_spheroidal_taper(S), aterms=identity_aterms(1, cfg["n_stations"], S),
                  nr_aterm_slots=1, max_nr_timesteps=int(md["nr_timesteps"].max()) if md.size else 0)
         if fl is not None:
