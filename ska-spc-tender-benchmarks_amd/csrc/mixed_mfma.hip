@@ -895,8 +895,10 @@ __device__ __forceinline__ bool next_item(const QueueArgs &q, int &cls, int &s, 
     return true;
 }
 
+// Persistent kernels ask for 4 (gridder) and 5 (degridder) resident workgroups per CU: the register budget this
+// leaves (128 / 96 VGPRs) spills a few bytes outside the inner loops and measured 2 % faster than 3 / 4 waves.
 template <int DT, bool SPLIT, int PT, int VB, bool PM>
-__global__ __launch_bounds__(kThreads) void gridder_mfma_queue(QueueArgs q)
+__global__ __launch_bounds__(kThreads, 4) void gridder_mfma_queue(QueueArgs q)
 {
     __shared__ GridderShared<PT, VB> sm;
     int cls, s, blk;
@@ -908,7 +910,7 @@ __global__ __launch_bounds__(kThreads) void gridder_mfma_queue(QueueArgs q)
 }
 
 template <int DT, bool SPLIT, int VT, int PB, bool PM>
-__global__ __launch_bounds__(kThreads) void degridder_mfma_queue(QueueArgs q)
+__global__ __launch_bounds__(kThreads, 5) void degridder_mfma_queue(QueueArgs q)
 {
     __shared__ DegridderShared<VT, PB> sm;
     int cls, s, blk;
