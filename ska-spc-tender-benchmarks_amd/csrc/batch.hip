@@ -278,9 +278,9 @@ extern "C" int idgx_batch_create(const idgx_batch_class *classes, int nr_classes
     }
     int rc = to_status(e);
     if (rc == IDGX_OK)
-        rc = upload_queue(b, true, 0);
+        rc = upload_queue(b, true, idgx::mixed_default_shape(true));
     if (rc == IDGX_OK)
-        rc = upload_queue(b, false, 0);
+        rc = upload_queue(b, false, idgx::mixed_default_shape(false));
     if (rc != IDGX_OK)
     {
         idgx_batch_destroy(b);

@@ -91,6 +91,7 @@ hipError_t launch_degrid_mixed(const KernelArgs &a, int precision, bool phase_mf
 // Persistent work-queue forms (the queue items are built on the host, see batch.cpp).
 int mixed_shape_count();
 MixedShape mixed_shape(int i);
+int mixed_default_shape(bool gridder); // shape of the one-shot kernels
 int grid_items_per_subgrid(int shape, int S);            // gridder items of one subgrid
 long long grid_item_pixels(int shape, int S, int item);  // pixels gridder item `item` of a subgrid writes
 int degrid_visibilities_per_item(int shape);             // visibilities one degridder item covers

@@ -861,7 +861,7 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
 // and persistent workgroups fed from the device work queue.
 // ---------------------------------------------------------------------------------------------------------------
 template <int DT, bool SPLIT, int PT, int VB, bool PM>
-__global__ __launch_bounds__(kThreads) void gridder_mfma(KernelArgs a)
+__global__ __launch_bounds__(kThreads, 4) void gridder_mfma(KernelArgs a)
 {
     __shared__ GridderShared<PT, VB> sm;
     const int pblk = blockIdx.x / a.p.nr_subgrids;
@@ -869,7 +869,7 @@ __global__ __launch_bounds__(kThreads) void gridder_mfma(KernelArgs a)
 }
 
 template <int DT, bool SPLIT, int VT, int PB, bool PM>
-__global__ __launch_bounds__(kThreads) void degridder_mfma(KernelArgs a)
+__global__ __launch_bounds__(kThreads, 5) void degridder_mfma(KernelArgs a)
 {
     __shared__ DegridderShared<VT, PB> sm;
     const int vblk = blockIdx.x / a.p.nr_subgrids;
@@ -990,7 +990,7 @@ template <int DT, bool SPLIT> __global__ __launch_bounds__(kThreads) void degrid
 
 // ---- launchers ------------------------------------------------------------------------------------------------
 
-constexpr int kDefaultPT = 4, kDefaultVB = 256, kDefaultVT = 4, kDefaultPB = 256;
+constexpr int kDefaultPT = 8, kDefaultVB = 256, kDefaultVT = 4, kDefaultPB = 256;
 
 template <int DT, bool SPLIT> hipError_t grid_mfma(const KernelArgs &a, bool pm, hipStream_t st)
 {
@@ -1124,6 +1124,20 @@ hipError_t launch_degrid_mixed(const KernelArgs &a, int precision, bool pm, hipS
 int mixed_shape_count()
 {
     return kNumShapes;
+}
+
+// The one-shot kernels' tile shapes, as persistent-kernel shapes (the batch starts from them).
+constexpr int kDefaultGridShape = 2, kDefaultDegridShape = 0;
+static_assert(kShapes[kDefaultGridShape].tiles == kDefaultPT && kShapes[kDefaultGridShape].batch == kDefaultVB &&
+                  kShapes[kDefaultGridShape].phase_mfma == 0,
+              "default gridder shape");
+static_assert(kShapes[kDefaultDegridShape].tiles == kDefaultVT && kShapes[kDefaultDegridShape].batch == kDefaultPB &&
+                  kShapes[kDefaultDegridShape].phase_mfma == 0,
+              "default degridder shape");
+
+int mixed_default_shape(bool gridder)
+{
+    return gridder ? kDefaultGridShape : kDefaultDegridShape;
 }
 
 MixedShape mixed_shape(int i)
