@@ -860,8 +860,24 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
 // Kernels: one-shot grids (workgroup b = item (b % n, b / n), so a subgrid's workgroups spread over the XCDs)
 // and persistent workgroups fed from the device work queue.
 // ---------------------------------------------------------------------------------------------------------------
+// Resident workgroups per CU the kernels ask for: 4 (gridder) and 5 (degridder), which leaves 128 / 96 VGPRs (a
+// few bytes spill outside the inner loops; measured 2 % faster than 3 / 4), fewer where the shape's LDS or its
+// accumulators do not allow it.
+constexpr int kLdsBytes = 160 * 1024;
+template <int PT, int VB> constexpr int gridder_min_blocks()
+{
+    constexpr int fit = kLdsBytes / static_cast<int>(sizeof(GridderShared<PT, VB>));
+    return fit < 4 ? fit : 4;
+}
+template <int VT, int PB> constexpr int degridder_min_blocks()
+{
+    constexpr int fit = kLdsBytes / static_cast<int>(sizeof(DegridderShared<VT, PB>));
+    constexpr int regs = VT >= 8 ? 3 : 5;
+    return fit < regs ? fit : regs;
+}
+
 template <int DT, bool SPLIT, int PT, int VB, bool PM>
-__global__ __launch_bounds__(kThreads, 4) void gridder_mfma(KernelArgs a)
+__global__ __launch_bounds__(kThreads, (gridder_min_blocks<PT, VB>())) void gridder_mfma(KernelArgs a)
 {
     __shared__ GridderShared<PT, VB> sm;
     const int pblk = blockIdx.x / a.p.nr_subgrids;
@@ -869,7 +885,7 @@ __global__ __launch_bounds__(kThreads, 4) void gridder_mfma(KernelArgs a)
 }
 
 template <int DT, bool SPLIT, int VT, int PB, bool PM>
-__global__ __launch_bounds__(kThreads, 5) void degridder_mfma(KernelArgs a)
+__global__ __launch_bounds__(kThreads, (degridder_min_blocks<VT, PB>())) void degridder_mfma(KernelArgs a)
 {
     __shared__ DegridderShared<VT, PB> sm;
     const int vblk = blockIdx.x / a.p.nr_subgrids;
@@ -895,10 +911,8 @@ __device__ __forceinline__ bool next_item(const QueueArgs &q, int &cls, int &s, 
     return true;
 }
 
-// Persistent kernels ask for 4 (gridder) and 5 (degridder) resident workgroups per CU: the register budget this
-// leaves (128 / 96 VGPRs) spills a few bytes outside the inner loops and measured 2 % faster than 3 / 4 waves.
 template <int DT, bool SPLIT, int PT, int VB, bool PM>
-__global__ __launch_bounds__(kThreads, 4) void gridder_mfma_queue(QueueArgs q)
+__global__ __launch_bounds__(kThreads, (gridder_min_blocks<PT, VB>())) void gridder_mfma_queue(QueueArgs q)
 {
     __shared__ GridderShared<PT, VB> sm;
     int cls, s, blk;
@@ -910,7 +924,7 @@ __global__ __launch_bounds__(kThreads, 4) void gridder_mfma_queue(QueueArgs q)
 }
 
 template <int DT, bool SPLIT, int VT, int PB, bool PM>
-__global__ __launch_bounds__(kThreads, 5) void degridder_mfma_queue(QueueArgs q)
+__global__ __launch_bounds__(kThreads, (degridder_min_blocks<VT, PB>())) void degridder_mfma_queue(QueueArgs q)
 {
     __shared__ DegridderShared<VT, PB> sm;
     int cls, s, blk;
@@ -948,6 +962,53 @@ template <int DT, bool SPLIT> __global__ __launch_bounds__(kThreads) void degrid
     if (!subgrid_in_bounds(a.p, m, row0, c0, nc) || nc == 0)
         return; // uniform
     const float2 *sub = reinterpret_cast<const float2 *>(a.subgrids) + static_cast<size_t>(s) * kCorr * P;
+    uint4 *dst = const_cast<uint4 *>(a.prepared) + static_cast<size_t>(s) * nslots * 2;
+    constexpr int kCached = 4; // slots per thread held in registers between the max and the split (S <= 32)
+    if (nslots <= kCached * kThreads)
+    {
+        // One pass over the subgrid: pixels stay in registers while the workgroup finds the FP16 scale.
+        cfloat px[kCached][4];
+        float vmax = 0.F;
+#pragma unroll
+        for (int k = 0; k < kCached; ++k)
+        {
+            const int q = tid + k * kThreads;
+            const int sy = q / Sp, sx = q - sy * Sp;
+#pragma unroll
+            for (int pol = 0; pol < 4; ++pol)
+                px[k][pol] = {0.F, 0.F};
+            if (q < nslots && sx < S)
+            {
+                degrid_pixel(a, m, sub, sy * S + sx, px[k]);
+#pragma unroll
+                for (int pol = 0; pol < 4; ++pol)
+                    vmax = fmaxf(vmax, fmaxf(fabsf(px[k][pol].re), fabsf(px[k][pol].im)));
+            }
+        }
+        float inv1 = 1.F;
+        if constexpr (DT == kFP16)
+        {
+            const int e = scale_exponent(block_max(vmax, maxbits));
+            if (tid == 0)
+                const_cast<int *>(a.prepared_scale)[s] = e;
+            inv1 = ldexpf(1.F, -e);
+        }
+#pragma unroll
+        for (int k = 0; k < kCached; ++k)
+        {
+            const int q = tid + k * kThreads;
+            if (q >= nslots)
+                break;
+            uint32_t hi[4], lo[4];
+#pragma unroll
+            for (int pol = 0; pol < 4; ++pol)
+                px[k][pol] = {px[k][pol].re * inv1, px[k][pol].im * inv1};
+            split_item<DT, SPLIT>(px[k], hi, lo);
+            dst[2 * q] = make_uint4(hi[0], lo[0], hi[1], lo[1]);
+            dst[2 * q + 1] = make_uint4(hi[2], lo[2], hi[3], lo[3]);
+        }
+        return;
+    }
     float inv = 1.F;
     if constexpr (DT == kFP16)
     {
@@ -969,7 +1030,6 @@ template <int DT, bool SPLIT> __global__ __launch_bounds__(kThreads) void degrid
             const_cast<int *>(a.prepared_scale)[s] = e;
         inv = ldexpf(1.F, -e);
     }
-    uint4 *dst = const_cast<uint4 *>(a.prepared) + static_cast<size_t>(s) * nslots * 2;
     for (int q = tid; q < nslots; q += kThreads)
     {
         const int sy = q / Sp, sx = q - sy * Sp;
