@@ -24,11 +24,18 @@
 // and D[4*(l/16) .. +3][l%16].  So a lane evaluates phasors of one row (pixel, or visibility) against four
 // k-pairs, packs (cos, sin) pairs straight into A registers, and reads its 16-byte B fragment from LDS.
 //
-// The phases themselves come from the matrix core too: phi = qu x' + qv y' + dw n is a rank-3 product, one
-// v_mfma_f32_16x16x4_f32 per 16 x 16 tile (FP32 operands, FP32 result).  Its D layout, lane l holding
-// D[4*(l/16)+j][l%16], is exactly the A layout above when D's rows are the k-pairs and its columns the rows of
-// the contraction, so the four phases a lane needs arrive in its own registers.  Per pair the VALU then only
-// evaluates v_sin/v_cos and the hi/lo split.
+// Phases.  By default the VALU forms phi = qu x' + qv y' + dw n with two FMAs per pair: the gridder's pixel
+// mapping keeps a lane's column x' fixed across its tiles (strips of 16 / 8 / 4 / 2 columns), and the
+// degridder's pixel slots run in rows padded to a multiple of 4 so the four pixels of a lane group share y', so
+// one of the three terms is formed once per visibility.  The PM shapes take the phases from the matrix core
+// instead (phi is a rank-3 product: one v_mfma_f32_16x16x4_f32 per 16 x 16 tile, whose D layout, lane l holding
+// D[4*(l/16)+j][l%16], is exactly the A layout above); that frees two VALU ops per pair but measured slower, so
+// the autotuner rarely picks it.
+//
+// hi/lo split (FP16): hi = v_cvt_pk_f16_f32 of (cos, sin); lo = the exact residual from v_fma_mix_f32 (reading
+// the half straight from the packed register), packed again: 4 VALU ops per pair.  FP16 B operands are scaled
+// by a power of two so that their largest value is ~2^14 (per LDS batch, or per subgrid for prepared degridder
+// pixels), which keeps any data range in range; the accumulators are rescaled exactly when the scale grows.
 //
 // Workgroup numbering spreads the workgroups of one subgrid over the XCDs (block b = sub_block * n + s): subgrid
 // costs are uneven, and grouping a subgrid's workgroups on one XCD (for L2 reuse) measured 1.2-2x slower because
