@@ -127,7 +127,11 @@ def kernel_key(precision, path, direction, shape=None):
     dt = 0 if precision.startswith("bf16") else 1
     split = "true" if precision.endswith("x3") else "false"
     if path == "one-shot":
-        return f"{direction}_mfma<{dt}, {split}, 4, 256, false>"
+        return f"{direction}_mfma<{dt}, {split}, {8 if direction == 'gridder' else 4}, 256, false>"
+    if isinstance(shape, list):  # one shape per class: one kernel only when they agree
+        if len(set(shape)) != 1:
+            return None
+        shape = shape[0]
     tiles, batch, pm = shape
     return f"{direction}_mfma_queue<{dt}, {split}, {tiles}, {batch}, {'true' if pm else 'false'}>"
 
@@ -136,7 +140,7 @@ def measured_traffic(config, key):
     """HBM bytes per launch of kernel `key` from the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE,
     scaled by the copy-kernel calibration) for this workload, or None when it was not profiled."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{config}.json")
-    if not os.path.exists(path):
+    if key is None or not os.path.exists(path):
         return None
     with open(path) as f:
         k = json.load(f).get("kernels", {}).get(key)
@@ -173,14 +177,17 @@ class Batched:
         self.precision, self.path = precision, "batched"
         self.db = DeviceBatch(classes, precision=precision, visibilities_on_device=d_vis)
         tg, td = self.db.batch.autotune(self.db.data, repeats=2, stream=stream)
-        gs, ds = self.db.batch.shape()
         shapes = Batch.shapes()
-        self.shapes = {"gridder": shapes[gs], "degridder": shapes[ds]}
+        per_class = [self.db.batch.class_shape(k) for k in range(len(classes))]
+        self.shapes = {"gridder": [shapes[g] for g, _ in per_class], "degridder": [shapes[d] for _, d in per_class]}
         items = self.db.batch.items()
-        self.info = {"gridder_shape": {"tiles_per_wave": shapes[gs][0], "lds_visibilities": shapes[gs][1],
-                                       "phase_mfma": shapes[gs][2]},
-                     "degridder_shape": {"tiles_per_wave": shapes[ds][0], "lds_pixels": shapes[ds][1],
-                                         "phase_mfma": shapes[ds][2]},
+        self.info = {"shapes_per_class": [
+                         {"subgrid_size": int(c["subgrid_size"]),
+                          "gridder": {"tiles_per_wave": shapes[g][0], "lds_visibilities": shapes[g][1],
+                                      "phase_mfma": shapes[g][2]},
+                          "degridder": {"tiles_per_wave": shapes[d][0], "lds_pixels": shapes[d][1],
+                                        "phase_mfma": shapes[d][2]}}
+                         for c, (g, d) in zip(classes, per_class)],
                      "queue_items": {"gridder": items[0], "degridder": items[1]},
                      "persistent_workgroups": {"gridder": items[2], "degridder": items[3]},
                      "autotune_ms": {"gridder": tg, "degridder": td}}
@@ -432,6 +439,9 @@ def run_batched_config(args):
     achieved = flops / t_dom / 1e12
     dom_key = kernel_key(head["precision"], "batched", dominant, runner.shapes[dominant])
     traffic = measured_traffic(args.config, dom_key)
+    if dom_key is None:  # one launch per class with different shapes
+        dom_key = " + ".join(kernel_key(head["precision"], "batched", dominant, sh)
+                             for sh in runner.shapes[dominant])
 
     cpu = None
     if not args.no_cpu:

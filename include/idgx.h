@@ -237,16 +237,17 @@ int idgx_plan_segments(const idgx_planner *planner, const float *d_uvw, const do
                        idgx_metadata *d_metadata, idgx_channels *d_channels, int64_t capacity,
                        int64_t *h_nr_subgrids, void *stream);
 
-/* ---- Batched execution: one persistent launch per direction over a whole plan ---------------------------------
+/* ---- Batched execution: persistent launches over a whole plan ------------------------------------------------
  *
- * A batch holds up to 4 subgrid classes (one subgrid size each: mixed sizes run in the same launch).  Its
- * work queue of (class, subgrid, block) items is built on the host at creation, sorted heaviest first, and
- * consumed by persistent workgroups through one device atomic counter.  Optional per-class visibility selections
- * (flagging, ragged timesteps) restrict each subgrid to the listed entries: subgrid s uses
- * vis_index[vis_offset[s] .. vis_offset[s+1]), each entry (t << 16) | c relative to the subgrid's first row and
- * channel window.  Creation allocates the queue memory; the execution entry points allocate nothing, and
- * idgx_batch_capture records them into a hipGraph (no memset nodes) for idgx_batch_launch.  Batches run the
- * mixed-precision forms (precision != IDGX_PLAIN_FP32).  A batch is used from one host thread at a time. */
+ * A batch holds up to 4 subgrid classes (one subgrid size each).  Its work queues of (class, subgrid, block)
+ * items are built on the host at creation, one per class and direction, sorted heaviest first, and consumed by
+ * persistent workgroups (one launch per class, each class with its own tile shape) through device atomic
+ * counters.  Optional per-class visibility selections (flagging, ragged timesteps) restrict each subgrid to the
+ * listed entries: subgrid s uses vis_index[vis_offset[s] .. vis_offset[s+1]), each entry (t << 16) | c relative to
+ * the subgrid's first row and channel window.  Creation allocates the queue memory; the execution entry points
+ * allocate nothing, and idgx_batch_capture records them into a hipGraph (no memset nodes) for idgx_batch_launch.
+ * Batches run the mixed-precision forms (precision != IDGX_PLAIN_FP32).  A batch is used from one host thread at a
+ * time. */
 
 typedef struct idgx_batch idgx_batch;
 
@@ -292,15 +293,18 @@ void idgx_batch_destroy(idgx_batch *batch);
  * core). */
 int idgx_batch_shape_count(void);
 int idgx_batch_shape_info(int shape, int *tiles_per_wave, int *lds_batch, int *phase_mfma);
+/* set_shape sets every class's shapes; get_shape reports class 0's. */
 int idgx_batch_set_shape(idgx_batch *batch, int gridder_shape, int degridder_shape);
 int idgx_batch_get_shape(const idgx_batch *batch, int *gridder_shape, int *degridder_shape);
+int idgx_batch_set_class_shape(idgx_batch *batch, int cls, int gridder_shape, int degridder_shape);
+int idgx_batch_get_class_shape(const idgx_batch *batch, int cls, int *gridder_shape, int *degridder_shape);
 
-/* Times every shape `repeats` times per direction on the given data (overwriting its outputs), keeps the fastest
- * and reports its average time (ms). */
+/* Times every shape `repeats` times per class and direction on the given data (overwriting its outputs), keeps
+ * each class's fastest and reports the sum over classes of their average times (ms). */
 int idgx_batch_autotune(idgx_batch *batch, const idgx_batch_data *data, int repeats, void *stream,
                         float *best_gridder_ms, float *best_degridder_ms);
 
-/* One persistent gridder / degridder launch over every subgrid of every class. */
+/* Persistent gridder / degridder launches (one per class) over every subgrid of every class. */
 int idgx_batch_grid(idgx_batch *batch, const idgx_batch_data *data, void *stream);
 int idgx_batch_degrid(idgx_batch *batch, const idgx_batch_data *data, void *stream);
 
@@ -309,7 +313,7 @@ int idgx_batch_degrid(idgx_batch *batch, const idgx_batch_data *data, void *stre
 int idgx_batch_capture(idgx_batch *batch, const idgx_batch_data *data, int stages, void *stream);
 int idgx_batch_launch(idgx_batch *batch, void *stream);
 
-/* Queue sizes and resident workgroups of the current shapes. */
+/* Queue sizes (all classes) and resident workgroups (the largest launch) of the current shapes. */
 int idgx_batch_items(const idgx_batch *batch, int *gridder_items, int *degridder_items, int *gridder_workgroups,
                      int *degridder_workgroups);
 

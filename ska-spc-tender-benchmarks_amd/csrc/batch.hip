@@ -1,13 +1,17 @@
-// Batched execution of a whole plan: every subgrid of up to four subgrid classes (one subgrid size each) runs in
-// one persistent gridder (or degridder) launch fed from a device work queue, optionally captured in a hipGraph,
-// with the tile shape chosen by an autotuner that times the compiled shapes on the real data.
+// Batched execution of a whole plan: the subgrids of up to four subgrid classes (one subgrid size each) run in
+// persistent gridder (or degridder) launches fed from device work queues, one launch per class so that each class
+// has its own tile shape, optionally captured in a hipGraph, with the shapes chosen by an autotuner that times
+// the compiled shapes on the real data, class by class.  (The tile that suits a subgrid size is the one that
+// covers its pixels with the fewest idle lanes: S = 24 and 48 prefer fewer tiles per wave than S = 32.  Separate
+// launches per class measured no slower than one launch over the mixed queue.)
 //
 // Work queue.  An item is (class, subgrid, block): a block of pixels for the gridder, a chunk of visibilities
 // for the degridder.  Items are built on the host from the plan (metadata, channel windows and visibility
 // selections give every subgrid's visibility count V_s), costed as visibilities x pixels, and sorted heaviest
-// first (longest-processing-time order), so the uneven tail (flagging, ragged timesteps, mixed sizes) finishes
-// together.  Persistent workgroups (as many as fit on the chip) take items through one atomic counter, which a
-// one-thread kernel clears before every launch (no memset nodes, so the sequence can be captured).
+// first (longest-processing-time order), so the uneven tail (flagging, ragged timesteps) finishes together.
+// Persistent workgroups (as many as fit on the chip) take items through an atomic counter per class and
+// direction, which a one-workgroup kernel clears before the launches (no memset nodes, so the sequence can be
+// captured).
 //
 // Degridder pixels: before the persistent degridder, one workgroup per subgrid applies taper and A-terms and
 // splits the pixels into the packed words the degridder stages (32 B per pixel slot), so the work items copy
@@ -31,13 +35,13 @@ struct idgx_batch
     idgx_plan plans[idgx::kMaxClasses];
     std::vector<int64_t> vis_count[idgx::kMaxClasses]; // V_s per subgrid
     int precision = IDGX_MIXED_BF16X3;
-    int grid_shape = 0, degrid_shape = 0;
-    int grid_wg = 0, degrid_wg = 0;
-    int nr_grid_items = 0, nr_degrid_items = 0;
-    size_t grid_capacity = 0, degrid_capacity = 0;
-    uint2 *d_grid_items = nullptr;
-    uint2 *d_degrid_items = nullptr;
-    int *d_counters = nullptr; // [0] gridder, [1] degridder
+    // Per direction (0 gridder, 1 degridder) and class: tile shape, resident workgroups, queue slice.
+    int shape[2][idgx::kMaxClasses] = {};
+    int workgroups[2][idgx::kMaxClasses] = {};
+    int nr_items[2][idgx::kMaxClasses] = {};
+    size_t offset[2][idgx::kMaxClasses] = {}, capacity[2][idgx::kMaxClasses] = {};
+    uint2 *d_items[2] = {};
+    int *d_counters = nullptr; // [direction][class]
     // Degridder pixels prepared once per subgrid and pass (per class): split words, FP16 scales, slot n table.
     uint4 *d_prepared[idgx::kMaxClasses] = {};
     int *d_prepared_scale[idgx::kMaxClasses] = {};
@@ -76,67 +80,79 @@ struct Item
     double cost;
 };
 
-std::vector<Item> build_items(const idgx_batch *b, bool gridder, int shape)
+// Work items of class k with tile shape `shape`, heaviest first.
+std::vector<Item> build_items(const idgx_batch *b, bool gridder, int k, int shape)
 {
     std::vector<Item> items;
     const long long per = idgx::degrid_visibilities_per_item(shape);
-    for (int k = 0; k < b->nr_classes; ++k)
+    const int S = b->plans[k].subgrid_size;
+    const long long P = static_cast<long long>(S) * S;
+    const int gitems = idgx::grid_items_per_subgrid(shape, S);
+    std::vector<double> gpix(gitems);
+    for (int i = 0; i < gitems; ++i)
+        gpix[i] = static_cast<double>(idgx::grid_item_pixels(shape, S, i));
+    for (int s = 0; s < b->plans[k].nr_subgrids; ++s)
     {
-        const int S = b->plans[k].subgrid_size;
-        const long long P = static_cast<long long>(S) * S;
-        const int gitems = idgx::grid_items_per_subgrid(shape, S);
-        std::vector<double> gpix(gitems);
-        for (int i = 0; i < gitems; ++i)
-            gpix[i] = static_cast<double>(idgx::grid_item_pixels(shape, S, i));
-        for (int s = 0; s < b->plans[k].nr_subgrids; ++s)
+        const long long V = b->vis_count[k][s];
+        const uint32_t tag = (static_cast<uint32_t>(k) << 28) | static_cast<uint32_t>(s);
+        if (gridder)
         {
-            const long long V = b->vis_count[k][s];
-            const uint32_t tag = (static_cast<uint32_t>(k) << 28) | static_cast<uint32_t>(s);
-            if (gridder)
-            {
-                // Every pixel block is written (zeros for an empty subgrid); the epilogue costs ~ V of 1.
-                for (int blk = 0; blk < gitems; ++blk)
-                    items.push_back({{tag, static_cast<uint32_t>(blk)}, gpix[blk] * static_cast<double>(V + 1)});
-            }
-            else
-            {
-                for (long long v0 = 0, blk = 0; v0 < V; v0 += per, ++blk)
-                    items.push_back({{tag, static_cast<uint32_t>(blk)},
-                                     static_cast<double>(std::min(per, V - v0)) * static_cast<double>(P)});
-            }
+            // Every pixel block is written (zeros for an empty subgrid); the epilogue costs ~ V of 1.
+            for (int blk = 0; blk < gitems; ++blk)
+                items.push_back({{tag, static_cast<uint32_t>(blk)}, gpix[blk] * static_cast<double>(V + 1)});
+        }
+        else
+        {
+            for (long long v0 = 0, blk = 0; v0 < V; v0 += per, ++blk)
+                items.push_back({{tag, static_cast<uint32_t>(blk)},
+                                 static_cast<double>(std::min(per, V - v0)) * static_cast<double>(P)});
         }
     }
     std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
     return items;
 }
 
-size_t max_items(const idgx_batch *b, bool gridder)
+// Queue slice of class k: room for its items under any shape (the counts build_items gives).
+size_t max_items(const idgx_batch *b, bool gridder, int k)
 {
     size_t m = 0;
     for (int sh = 0; sh < idgx::mixed_shape_count(); ++sh)
-        m = std::max(m, build_items(b, gridder, sh).size());
+    {
+        size_t n = 0;
+        if (gridder)
+            n = static_cast<size_t>(b->plans[k].nr_subgrids) *
+                static_cast<size_t>(idgx::grid_items_per_subgrid(sh, b->plans[k].subgrid_size));
+        else
+        {
+            const long long per = idgx::degrid_visibilities_per_item(sh);
+            for (const int64_t V : b->vis_count[k])
+                n += static_cast<size_t>((V + per - 1) / per);
+        }
+        m = std::max(m, n);
+    }
     return m;
 }
 
-int upload_queue(idgx_batch *b, bool gridder, int shape)
+int upload_queue(idgx_batch *b, bool gridder, int k, int shape)
 {
-    const std::vector<Item> items = build_items(b, gridder, shape);
+    const int dir = gridder ? 0 : 1;
+    const std::vector<Item> items = build_items(b, gridder, k, shape);
+    if (items.size() > b->capacity[dir][k] || items.size() > 0x7fffffffULL)
+        return IDGX_ERR_INVALID_ARGUMENT;
     std::vector<uint2> flat(items.size());
     for (size_t i = 0; i < items.size(); ++i)
         flat[i] = items[i].v;
-    uint2 *dst = gridder ? b->d_grid_items : b->d_degrid_items;
-    if (flat.size() > (gridder ? b->grid_capacity : b->degrid_capacity))
-        return IDGX_ERR_INVALID_ARGUMENT;
     if (!flat.empty())
     {
-        const hipError_t e = hipMemcpy(dst, flat.data(), flat.size() * sizeof(uint2), hipMemcpyHostToDevice);
+        const hipError_t e = hipMemcpy(b->d_items[dir] + b->offset[dir][k], flat.data(), flat.size() * sizeof(uint2),
+                                       hipMemcpyHostToDevice);
         if (e != hipSuccess)
             return to_status(e);
     }
-    (gridder ? b->nr_grid_items : b->nr_degrid_items) = static_cast<int>(flat.size());
-    (gridder ? b->grid_shape : b->degrid_shape) = shape;
-    (gridder ? b->grid_wg : b->degrid_wg) = idgx::queue_workgroups(b->precision, gridder, shape);
-    return IDGX_OK;
+    b->nr_items[dir][k] = static_cast<int>(flat.size());
+    b->shape[dir][k] = shape;
+    b->workgroups[dir][k] = idgx::queue_workgroups(b->precision, gridder, shape);
+    return b->workgroups[dir][k] > 0 ? IDGX_OK : IDGX_ERR_DEVICE;
 }
 
 int make_queue_args(const idgx_batch *b, const idgx_batch_data *d, bool gridder, QueueArgs &q)
@@ -182,29 +198,32 @@ int make_queue_args(const idgx_batch *b, const idgx_batch_data *d, bool gridder,
             a.slot_n = b->d_slot_n[k];
         }
     }
-    q.items = gridder ? b->d_grid_items : b->d_degrid_items;
-    q.nr_items = gridder ? b->nr_grid_items : b->nr_degrid_items;
-    q.counter = b->d_counters + (gridder ? 0 : 1);
     return IDGX_OK;
 }
 
-int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st)
+// One persistent launch per class (each with its own tile shape), or only class `only` when it is >= 0.
+int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st, int only = -1)
 {
+    const int dir = gridder ? 0 : 1;
     QueueArgs q;
     int rc = make_queue_args(b, d, gridder, q);
     if (rc != IDGX_OK)
         return rc;
-    if (q.nr_items == 0)
-        return IDGX_OK;
-    hipError_t e = hipSuccess;
-    for (int k = 0; k < b->nr_classes && !gridder && e == hipSuccess; ++k)
-        e = idgx::launch_degrid_prepare(q.cls[k], b->precision, st);
-    if (e == hipSuccess)
-        e = idgx::launch_reset_counter(q.counter, st);
-    if (e != hipSuccess)
-        return to_status(e);
-    e = gridder ? idgx::launch_grid_mixed_queue(q, b->precision, b->grid_shape, b->grid_wg, st)
-                : idgx::launch_degrid_mixed_queue(q, b->precision, b->degrid_shape, b->degrid_wg, st);
+    hipError_t e = idgx::launch_reset_counters(b->d_counters + dir * idgx::kMaxClasses, idgx::kMaxClasses, st);
+    for (int k = 0; k < b->nr_classes && e == hipSuccess; ++k)
+    {
+        if ((only >= 0 && k != only) || b->nr_items[dir][k] == 0)
+            continue;
+        if (!gridder)
+            e = idgx::launch_degrid_prepare(q.cls[k], b->precision, st);
+        if (e != hipSuccess)
+            break;
+        q.items = b->d_items[dir] + b->offset[dir][k];
+        q.nr_items = b->nr_items[dir][k];
+        q.counter = b->d_counters + dir * idgx::kMaxClasses + k;
+        e = gridder ? idgx::launch_grid_mixed_queue(q, b->precision, b->shape[0][k], b->workgroups[0][k], st)
+                    : idgx::launch_degrid_mixed_queue(q, b->precision, b->shape[1][k], b->workgroups[1][k], st);
+    }
     return to_status(e);
 }
 } // namespace
@@ -259,13 +278,20 @@ extern "C" int idgx_batch_create(const idgx_batch_class *classes, int nr_classes
             b->vis_count[k][s] = v;
         }
     }
-    b->grid_capacity = std::max<size_t>(1, max_items(b, true));
-    b->degrid_capacity = std::max<size_t>(1, max_items(b, false));
-    hipError_t e = hipMalloc(&b->d_grid_items, b->grid_capacity * sizeof(uint2));
+    hipError_t e = hipSuccess;
+    for (int dir = 0; dir < 2 && e == hipSuccess; ++dir)
+    {
+        size_t total = 0;
+        for (int k = 0; k < nr_classes; ++k)
+        {
+            b->offset[dir][k] = total;
+            b->capacity[dir][k] = max_items(b, dir == 0, k);
+            total += b->capacity[dir][k];
+        }
+        e = hipMalloc(&b->d_items[dir], std::max<size_t>(1, total) * sizeof(uint2));
+    }
     if (e == hipSuccess)
-        e = hipMalloc(&b->d_degrid_items, b->degrid_capacity * sizeof(uint2));
-    if (e == hipSuccess)
-        e = hipMalloc(&b->d_counters, 2 * sizeof(int));
+        e = hipMalloc(&b->d_counters, 2 * idgx::kMaxClasses * sizeof(int));
     for (int k = 0; k < nr_classes && e == hipSuccess; ++k)
     {
         const int S = b->plans[k].subgrid_size;
@@ -277,10 +303,13 @@ extern "C" int idgx_batch_create(const idgx_batch_class *classes, int nr_classes
             e = hipMalloc(&b->d_slot_n[k], static_cast<size_t>(S) * idgx::degrid_row_stride(S) * sizeof(float));
     }
     int rc = to_status(e);
-    if (rc == IDGX_OK)
-        rc = upload_queue(b, true, idgx::mixed_default_shape(true));
-    if (rc == IDGX_OK)
-        rc = upload_queue(b, false, idgx::mixed_default_shape(false));
+    for (int k = 0; k < nr_classes && rc == IDGX_OK; ++k)
+    {
+        const int S = b->plans[k].subgrid_size;
+        rc = upload_queue(b, true, k, idgx::mixed_default_shape(true, S));
+        if (rc == IDGX_OK)
+            rc = upload_queue(b, false, k, idgx::mixed_default_shape(false, S));
+    }
     if (rc != IDGX_OK)
     {
         idgx_batch_destroy(b);
@@ -309,29 +338,44 @@ extern "C" int idgx_batch_shape_info(int shape, int *tiles_per_wave, int *lds_ba
     return IDGX_OK;
 }
 
-extern "C" int idgx_batch_set_shape(idgx_batch *b, int gridder_shape, int degridder_shape)
+extern "C" int idgx_batch_set_class_shape(idgx_batch *b, int cls, int gridder_shape, int degridder_shape)
 {
-    if (!b || gridder_shape < 0 || gridder_shape >= idgx::mixed_shape_count() || degridder_shape < 0 ||
-        degridder_shape >= idgx::mixed_shape_count())
+    if (!b || cls < 0 || cls >= b->nr_classes || gridder_shape < 0 || gridder_shape >= idgx::mixed_shape_count() ||
+        degridder_shape < 0 || degridder_shape >= idgx::mixed_shape_count())
         return IDGX_ERR_INVALID_ARGUMENT;
     drop_graph(b);
     int rc = IDGX_OK;
-    if (gridder_shape != b->grid_shape)
-        rc = upload_queue(b, true, gridder_shape);
-    if (rc == IDGX_OK && degridder_shape != b->degrid_shape)
-        rc = upload_queue(b, false, degridder_shape);
+    if (gridder_shape != b->shape[0][cls])
+        rc = upload_queue(b, true, cls, gridder_shape);
+    if (rc == IDGX_OK && degridder_shape != b->shape[1][cls])
+        rc = upload_queue(b, false, cls, degridder_shape);
     return rc;
+}
+
+extern "C" int idgx_batch_set_shape(idgx_batch *b, int gridder_shape, int degridder_shape)
+{
+    if (!b)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    int rc = IDGX_OK;
+    for (int k = 0; k < b->nr_classes && rc == IDGX_OK; ++k)
+        rc = idgx_batch_set_class_shape(b, k, gridder_shape, degridder_shape);
+    return rc;
+}
+
+extern "C" int idgx_batch_get_class_shape(const idgx_batch *b, int cls, int *gridder_shape, int *degridder_shape)
+{
+    if (!b || cls < 0 || cls >= b->nr_classes)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    if (gridder_shape)
+        *gridder_shape = b->shape[0][cls];
+    if (degridder_shape)
+        *degridder_shape = b->shape[1][cls];
+    return IDGX_OK;
 }
 
 extern "C" int idgx_batch_get_shape(const idgx_batch *b, int *gridder_shape, int *degridder_shape)
 {
-    if (!b)
-        return IDGX_ERR_INVALID_ARGUMENT;
-    if (gridder_shape)
-        *gridder_shape = b->grid_shape;
-    if (degridder_shape)
-        *degridder_shape = b->degrid_shape;
-    return IDGX_OK;
+    return idgx_batch_get_class_shape(b, 0, gridder_shape, degridder_shape);
 }
 
 extern "C" int idgx_batch_grid(idgx_batch *b, const idgx_batch_data *data, void *stream)
@@ -364,41 +408,47 @@ extern "C" int idgx_batch_autotune(idgx_batch *b, const idgx_batch_data *data, i
         return IDGX_ERR_DEVICE;
     }
     int rc = IDGX_OK;
+    float total[2] = {0.F, 0.F};
+    // Each class is tuned on its own launch: the launches of a direction run one after another.
     for (int dir = 0; dir < 2 && rc == IDGX_OK; ++dir)
     {
         const bool gridder = dir == 0;
-        float best = -1.F;
-        int best_shape = 0;
-        for (int sh = 0; sh < idgx::mixed_shape_count() && rc == IDGX_OK; ++sh)
+        for (int k = 0; k < b->nr_classes && rc == IDGX_OK; ++k)
         {
-            rc = upload_queue(b, gridder, sh);
-            if (rc != IDGX_OK)
-                break;
-            rc = run(b, data, gridder, st); // warm-up
-            if (rc != IDGX_OK)
-                break;
-            if (hipEventRecord(e0, st) != hipSuccess)
-                rc = IDGX_ERR_DEVICE;
-            for (int r = 0; r < repeats && rc == IDGX_OK; ++r)
-                rc = run(b, data, gridder, st);
-            float ms = 0.F;
-            if (rc == IDGX_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                                  hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                rc = IDGX_ERR_DEVICE;
-            ms /= repeats;
-            if (rc == IDGX_OK && (best < 0.F || ms < best))
+            float best = -1.F;
+            int best_shape = b->shape[dir][k];
+            for (int sh = 0; sh < idgx::mixed_shape_count() && rc == IDGX_OK; ++sh)
             {
-                best = ms;
-                best_shape = sh;
+                rc = upload_queue(b, gridder, k, sh);
+                if (rc != IDGX_OK)
+                    break;
+                rc = run(b, data, gridder, st, k); // warm-up
+                if (rc != IDGX_OK)
+                    break;
+                if (hipEventRecord(e0, st) != hipSuccess)
+                    rc = IDGX_ERR_DEVICE;
+                for (int r = 0; r < repeats && rc == IDGX_OK; ++r)
+                    rc = run(b, data, gridder, st, k);
+                float ms = 0.F;
+                if (rc == IDGX_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                      hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                    rc = IDGX_ERR_DEVICE;
+                ms /= repeats;
+                if (rc == IDGX_OK && (best < 0.F || ms < best))
+                {
+                    best = ms;
+                    best_shape = sh;
+                }
             }
+            if (rc == IDGX_OK)
+                rc = upload_queue(b, gridder, k, best_shape);
+            total[dir] += std::max(best, 0.F);
         }
-        if (rc == IDGX_OK)
-            rc = upload_queue(b, gridder, best_shape);
-        if (gridder && best_grid_ms)
-            *best_grid_ms = best;
-        if (!gridder && best_degrid_ms)
-            *best_degrid_ms = best;
     }
+    if (best_grid_ms)
+        *best_grid_ms = total[0];
+    if (best_degrid_ms)
+        *best_degrid_ms = total[1];
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return rc;
@@ -453,14 +503,21 @@ extern "C" int idgx_batch_items(const idgx_batch *b, int *gridder_items, int *de
 {
     if (!b)
         return IDGX_ERR_INVALID_ARGUMENT;
+    int items[2] = {0, 0}, wg[2] = {0, 0};
+    for (int dir = 0; dir < 2; ++dir)
+        for (int k = 0; k < b->nr_classes; ++k)
+        {
+            items[dir] += b->nr_items[dir][k];
+            wg[dir] = std::max(wg[dir], b->workgroups[dir][k]);
+        }
     if (gridder_items)
-        *gridder_items = b->nr_grid_items;
+        *gridder_items = items[0];
     if (degridder_items)
-        *degridder_items = b->nr_degrid_items;
+        *degridder_items = items[1];
     if (gridder_workgroups)
-        *gridder_workgroups = b->grid_wg;
+        *gridder_workgroups = wg[0];
     if (degridder_workgroups)
-        *degridder_workgroups = b->degrid_wg;
+        *degridder_workgroups = wg[1];
     return IDGX_OK;
 }
 
@@ -469,10 +526,9 @@ extern "C" void idgx_batch_destroy(idgx_batch *b)
     if (!b)
         return;
     drop_graph(b);
-    if (b->d_grid_items)
-        (void)hipFree(b->d_grid_items);
-    if (b->d_degrid_items)
-        (void)hipFree(b->d_degrid_items);
+    for (int dir = 0; dir < 2; ++dir)
+        if (b->d_items[dir])
+            (void)hipFree(b->d_items[dir]);
     if (b->d_counters)
         (void)hipFree(b->d_counters);
     for (int k = 0; k < idgx::kMaxClasses; ++k)

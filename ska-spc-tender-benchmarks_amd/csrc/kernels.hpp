@@ -35,8 +35,8 @@ struct KernelArgs
     const float *slot_n;
 };
 
-// Persistent-kernel work: up to kMaxClasses subgrid classes (one subgrid size each) share one launch; the queue
-// holds (class << 28 | subgrid, block) items, heaviest first; workgroups take items with an atomic counter.
+// Persistent-kernel work over up to kMaxClasses subgrid classes (one subgrid size each): the queue holds
+// (class << 28 | subgrid, block) items, heaviest first; workgroups take items with an atomic counter.
 constexpr int kMaxClasses = 4;
 struct QueueArgs
 {
@@ -91,13 +91,13 @@ hipError_t launch_degrid_mixed(const KernelArgs &a, int precision, bool phase_mf
 // Persistent work-queue forms (the queue items are built on the host, see batch.cpp).
 int mixed_shape_count();
 MixedShape mixed_shape(int i);
-int mixed_default_shape(bool gridder); // shape of the one-shot kernels
+int mixed_default_shape(bool gridder, int S); // shape the one-shot kernels use for subgrid size S
 int grid_items_per_subgrid(int shape, int S);            // gridder items of one subgrid
 long long grid_item_pixels(int shape, int S, int item);  // pixels gridder item `item` of a subgrid writes
 int degrid_visibilities_per_item(int shape);             // visibilities one degridder item covers
 hipError_t launch_grid_mixed_queue(const QueueArgs &q, int precision, int shape, int workgroups, hipStream_t st);
 hipError_t launch_degrid_mixed_queue(const QueueArgs &q, int precision, int shape, int workgroups, hipStream_t st);
-hipError_t launch_reset_counter(int *counter, hipStream_t st);
+hipError_t launch_reset_counters(int *counters, int n, hipStream_t st); // counters[0 .. n) = 0
 // Degridder pixel preparation for a.prepared / a.prepared_scale / a.slot_n (writes through those pointers).
 size_t degrid_prepared_bytes(int S, long long nr_subgrids); // bytes of a.prepared
 hipError_t launch_degrid_prepare(const KernelArgs &a, int precision, hipStream_t st);

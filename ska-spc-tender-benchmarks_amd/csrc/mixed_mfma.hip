@@ -656,6 +656,7 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
 #pragma unroll
     for (int t = 0; t < VT; ++t)
         acc[t] = (f32x4){0.F, 0.F, 0.F, 0.F};
+    const bool wave_live = v0 + wave * VT * 16 < V; // wave-uniform
     int scale_exp = -126; // FP16: current power-of-two scale of the staged B operand
     if (DT == kFP16 && a.prepared)
         scale_exp = a.prepared_scale[s];
@@ -780,7 +781,8 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
             }
         }
         __syncthreads();
-        const int nblk = (np + 15) >> 4;
+        // A wave whose visibilities all lie past V (the subgrid's last chunk) only stages and waits.
+        const int nblk = wave_live ? (np + 15) >> 4 : 0;
         for (int b = 0; b < nblk; ++b)
         {
             const u32x4 bf = sm.b[b * 64 + lane];
@@ -879,7 +881,7 @@ template <int PT, int VB> constexpr int gridder_min_blocks()
 template <int VT, int PB> constexpr int degridder_min_blocks()
 {
     constexpr int fit = kLdsBytes / static_cast<int>(sizeof(DegridderShared<VT, PB>));
-    constexpr int regs = VT >= 8 ? 3 : 5;
+    constexpr int regs = VT >= 8 ? 3 : VT >= 6 ? 4 : 5;
     return fit < regs ? fit : regs;
 }
 
@@ -939,9 +941,10 @@ __global__ __launch_bounds__(kThreads, (degridder_min_blocks<VT, PB>())) void de
         degridder_item<DT, SPLIT, VT, PB, PM>(q.cls[cls], s, blk, sm);
 }
 
-__global__ void reset_counter_kernel(int *counter)
+__global__ void reset_counters_kernel(int *counters, int n)
 {
-    *counter = 0;
+    if (static_cast<int>(threadIdx.x) < n)
+        counters[threadIdx.x] = 0;
 }
 
 // One workgroup per subgrid: the taper x A-term-applied pixels of every slot, split into the packed words the
@@ -1059,17 +1062,64 @@ template <int DT, bool SPLIT> __global__ __launch_bounds__(kThreads) void degrid
 
 constexpr int kDefaultPT = 8, kDefaultVB = 256, kDefaultVT = 4, kDefaultPB = 256;
 
-template <int DT, bool SPLIT> hipError_t grid_mfma(const KernelArgs &a, bool pm, hipStream_t st)
+// Gridder tiles per wave for subgrid size S.  A wave with more tiles spreads each visibility's staged work over
+// more pixels (issue efficiency relative to 8 tiles, measured on sizes the tiles cover exactly: 6 -> 0.965,
+// 4 -> 0.89, 3 -> 0.845, 2 -> 0.78), but the last row block and the last item of a subgrid can leave tiles and
+// waves idle (coverage = S^2 / pixels of the items).  The choice maximises coverage x efficiency: S = 32 and 64
+// take 8, S = 48 takes 6, S = 24 takes 3, S = 16 takes 4.
+constexpr int kGridTileChoices[] = {8, 6, 4, 3, 2};
+constexpr double kGridTileEfficiency[] = {1.0, 0.965, 0.89, 0.845, 0.78};
+
+double grid_coverage(int S, int tiles)
 {
-    const GridUnits gu = grid_units(a.p.S, kDefaultPT);
+    const GridUnits gu = grid_units(S, tiles);
+    const long long items = (gu.units + kWaves - 1) / kWaves;
+    return static_cast<double>(S) * S / static_cast<double>(items * kWaves * tiles * 16);
+}
+
+int grid_tiles_for(int S)
+{
+    int best = kGridTileChoices[0];
+    double best_score = -1.0;
+    for (int i = 0; i < static_cast<int>(sizeof(kGridTileChoices) / sizeof(int)); ++i)
+    {
+        const double score = grid_coverage(S, kGridTileChoices[i]) * kGridTileEfficiency[i];
+        if (score > best_score + 1e-9) // ties keep the larger tile count
+        {
+            best_score = score;
+            best = kGridTileChoices[i];
+        }
+    }
+    return best;
+}
+
+template <int DT, bool SPLIT, int PT, bool PM> hipError_t grid_mfma_tiles(const KernelArgs &a, hipStream_t st)
+{
+    const GridUnits gu = grid_units(a.p.S, PT);
     const long long blocks = static_cast<long long>(a.p.nr_subgrids) * ((gu.units + kWaves - 1) / kWaves);
     if (blocks > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
-    if (pm)
-        gridder_mfma<DT, SPLIT, kDefaultPT, kDefaultVB, true><<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(a);
-    else
-        gridder_mfma<DT, SPLIT, kDefaultPT, kDefaultVB, false><<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(a);
+    gridder_mfma<DT, SPLIT, PT, kDefaultVB, PM><<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(a);
     return hipGetLastError();
+}
+
+template <int DT, bool SPLIT> hipError_t grid_mfma(const KernelArgs &a, bool pm, hipStream_t st)
+{
+    if (pm)
+        return grid_mfma_tiles<DT, SPLIT, kDefaultPT, true>(a, st);
+    switch (grid_tiles_for(a.p.S))
+    {
+    case 8:
+        return grid_mfma_tiles<DT, SPLIT, 8, false>(a, st);
+    case 6:
+        return grid_mfma_tiles<DT, SPLIT, 6, false>(a, st);
+    case 4:
+        return grid_mfma_tiles<DT, SPLIT, 4, false>(a, st);
+    case 3:
+        return grid_mfma_tiles<DT, SPLIT, 3, false>(a, st);
+    default:
+        return grid_mfma_tiles<DT, SPLIT, 2, false>(a, st);
+    }
 }
 
 template <int DT, bool SPLIT> hipError_t degrid_mfma(const KernelArgs &a, bool pm, hipStream_t st)
@@ -1089,8 +1139,8 @@ template <int DT, bool SPLIT> hipError_t degrid_mfma(const KernelArgs &a, bool p
 
 // Tile shapes the persistent kernels are built for (the autotuner picks among them): {tiles per wave, LDS batch,
 // phases on the matrix core}.
-constexpr MixedShape kShapes[] = {{4, 256, 0}, {2, 256, 0}, {8, 256, 0}, {4, 128, 0},
-                                  {4, 512, 0}, {4, 256, 1}, {8, 256, 1}, {2, 256, 1}};
+constexpr MixedShape kShapes[] = {{4, 256, 0}, {2, 256, 0}, {8, 256, 0}, {4, 128, 0}, {4, 512, 0},
+                                  {4, 256, 1}, {8, 256, 1}, {2, 256, 1}, {3, 256, 0}, {6, 256, 0}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 template <int DT, bool SPLIT, int I> void *queue_fn(bool gridder)
@@ -1121,6 +1171,10 @@ template <int DT, bool SPLIT> void *queue_kernel(bool gridder, int shape)
         return queue_fn<DT, SPLIT, 6>(gridder);
     case 7:
         return queue_fn<DT, SPLIT, 7>(gridder);
+    case 8:
+        return queue_fn<DT, SPLIT, 8>(gridder);
+    case 9:
+        return queue_fn<DT, SPLIT, 9>(gridder);
     default:
         return nullptr;
     }
@@ -1193,7 +1247,7 @@ int mixed_shape_count()
     return kNumShapes;
 }
 
-// The one-shot kernels' tile shapes, as persistent-kernel shapes (the batch starts from them).
+// The one-shot kernels' tile shapes for S = 32, as persistent-kernel shapes.
 constexpr int kDefaultGridShape = 2, kDefaultDegridShape = 0;
 static_assert(kShapes[kDefaultGridShape].tiles == kDefaultPT && kShapes[kDefaultGridShape].batch == kDefaultVB &&
                   kShapes[kDefaultGridShape].phase_mfma == 0,
@@ -1202,9 +1256,15 @@ static_assert(kShapes[kDefaultDegridShape].tiles == kDefaultVT && kShapes[kDefau
                   kShapes[kDefaultDegridShape].phase_mfma == 0,
               "default degridder shape");
 
-int mixed_default_shape(bool gridder)
+int mixed_default_shape(bool gridder, int S)
 {
-    return gridder ? kDefaultGridShape : kDefaultDegridShape;
+    if (!gridder)
+        return kDefaultDegridShape;
+    const int tiles = grid_tiles_for(S);
+    for (int i = 0; i < kNumShapes; ++i)
+        if (kShapes[i].tiles == tiles && kShapes[i].batch == kDefaultVB && !kShapes[i].phase_mfma)
+            return i;
+    return kDefaultGridShape;
 }
 
 MixedShape mixed_shape(int i)
@@ -1273,9 +1333,11 @@ hipError_t launch_degrid_prepare(const KernelArgs &a, int precision, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_reset_counter(int *counter, hipStream_t st)
+hipError_t launch_reset_counters(int *counters, int n, hipStream_t st)
 {
-    reset_counter_kernel<<<1, 1, 0, st>>>(counter);
+    if (n < 1 || n > 64)
+        return hipErrorInvalidValue;
+    reset_counters_kernel<<<1, 64, 0, st>>>(counters, n);
     return hipGetLastError();
 }
 

@@ -159,6 +159,9 @@ def lib():
         L.idgx_batch_destroy.restype = None
         L.idgx_batch_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.idgx_batch_get_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.idgx_batch_set_class_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.idgx_batch_get_class_shape.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.POINTER(ctypes.c_int)]
         L.idgx_batch_shape_info.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
         L.idgx_batch_autotune.argtypes = [vp, ctypes.POINTER(BatchData), ctypes.c_int, vp,
                                           ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
@@ -373,6 +376,7 @@ class Batch:
         h = ctypes.c_void_p()
         _check(lib().idgx_batch_create(arr, len(classes), ctypes.byref(h)), "idgx_batch_create")
         self.handle = h
+        self.nr_classes = len(classes)
 
     def close(self):
         if self.handle:
@@ -410,6 +414,16 @@ class Batch:
     def shape(self):
         g, d = ctypes.c_int(), ctypes.c_int()
         _check(lib().idgx_batch_get_shape(self.handle, ctypes.byref(g), ctypes.byref(d)), "idgx_batch_get_shape")
+        return g.value, d.value
+
+    def set_class_shape(self, cls, gridder_shape, degridder_shape):
+        _check(lib().idgx_batch_set_class_shape(self.handle, cls, gridder_shape, degridder_shape),
+               "idgx_batch_set_class_shape")
+
+    def class_shape(self, cls):
+        g, d = ctypes.c_int(), ctypes.c_int()
+        _check(lib().idgx_batch_get_class_shape(self.handle, cls, ctypes.byref(g), ctypes.byref(d)),
+               "idgx_batch_get_class_shape")
         return g.value, d.value
 
     @staticmethod

@@ -1,5 +1,6 @@
 """Batched execution (idgx_batch_*): persistent work-queue kernels over whole plans, visibility selections
-(flagging), mixed subgrid sizes in one launch, hipGraph capture and the autotuner."""
+(flagging), mixed subgrid sizes in one batch (a launch and a tile shape per class), hipGraph capture and the
+autotuner."""
 import numpy as np
 import pytest
 
@@ -83,8 +84,8 @@ def test_batch_with_flag_selection_vs_fp64():
     assert ev <= 5e-5
 
 
-def test_mixed_subgrid_sizes_in_one_launch():
-    w = observation.make_observation("small")
+def _mixed_size_classes(w):
+    """Three classes (S = 24, 32, 48) from one observation, sharing the visibility rows."""
     classes = []
     for k, S in enumerate((24, 32, 48)):
         idx = np.arange(k, w["nr_subgrids"], 3)
@@ -105,6 +106,12 @@ def test_mixed_subgrid_sizes_in_one_launch():
     uvw, vis = w["uvw"][rows], w["visibilities"][rows]
     for c in shared:
         c.update(uvw=uvw, visibilities=vis, nr_rows=rows.size)
+    return shared
+
+
+def test_mixed_subgrid_sizes_in_one_batch():
+    w = observation.make_observation("small")
+    shared = _mixed_size_classes(w)
     db = DeviceBatch(shared, precision="bf16x3")
     db.grid()
     for k, c in enumerate(shared):
@@ -164,21 +171,60 @@ def test_autotuner_picks_a_shape_and_keeps_results():
 
 @pytest.mark.parametrize("precision", ["fp16x3", "bf16x3"])
 def test_every_tile_shape_gives_the_same_results(precision):
-    """All compiled tile shapes (tiles per wave, LDS depth, phases on the matrix core) agree with the default one
-    to rounding, on a plan with mixed subgrid sizes' edge cases (S = 24 rows not a multiple of the tile)."""
+    """All compiled tile shapes (tiles per wave, LDS depth, phases on the matrix core) agree with the default ones
+    to rounding, for subgrid sizes whose rows are not a multiple of the tile (S = 24, 48) and for S = 32."""
     w = observation.make_observation("small")
-    db = DeviceBatch([w], precision=precision)
+    db = DeviceBatch(_mixed_size_classes(w), precision=precision)
     db.grid()
     db.degrid()
-    sub0, vis0 = db.subgrids(), db.visibilities_out()
+    sub0 = [db.subgrids(k) for k in range(3)]
+    vis0 = db.visibilities_out()
     for k in range(len(Batch.shapes())):
         db.batch.set_shape(k, k)
-        db.classes[0]["d_subgrids"].zero_()
+        for c in db.classes:
+            c["d_subgrids"].zero_()
         db.d_vis_out.zero_()
         db.grid()
-        sub = db.subgrids()
+        eg = max(rel_l2(db.subgrids(j), sub0[j]) for j in range(3))
         db.degrid()
-        vis = db.visibilities_out()
-        eg, ev = rel_l2(sub, sub0), rel_l2(vis, vis0)
+        ev = rel_l2(db.visibilities_out(), vis0)
         print(f"{precision} shape {Batch.shapes()[k]}: gridder {eg:.2e}, degridder {ev:.2e}")
         assert eg < 2e-6 and ev < 2e-6
+
+
+def test_classes_run_with_their_own_tile_shapes():
+    """Each class keeps its own shapes (set one by one, or by the autotuner) and the results do not depend on
+    them beyond rounding."""
+    w = observation.make_observation("small")
+    db = DeviceBatch(_mixed_size_classes(w), precision="fp16x3")
+    db.grid()
+    db.degrid()
+    sub0 = [db.subgrids(k) for k in range(3)]
+    vis0 = db.visibilities_out()
+    shapes = Batch.shapes()
+    picks = [(shapes.index((3, 256, 0)), shapes.index((6, 256, 0))),
+             (shapes.index((8, 256, 0)), shapes.index((4, 256, 0))),
+             (shapes.index((6, 256, 0)), shapes.index((2, 256, 0)))]
+    for k, (g, d) in enumerate(picks):
+        db.batch.set_class_shape(k, g, d)
+    assert [db.batch.class_shape(k) for k in range(3)] == picks
+    assert db.batch.shape() == picks[0]
+    for c in db.classes:
+        c["d_subgrids"].zero_()
+    db.d_vis_out.zero_()
+    db.grid()
+    db.degrid()
+    for k in range(3):
+        assert rel_l2(db.subgrids(k), sub0[k]) < 2e-6, f"class {k}"
+    assert rel_l2(db.visibilities_out(), vis0) < 2e-6
+    tg, td = db.batch.autotune(db.data, repeats=1)
+    tuned = [db.batch.class_shape(k) for k in range(3)]
+    print("autotuned per class:", [(shapes[g], shapes[d]) for g, d in tuned], f"{tg:.3f} / {td:.3f} ms")
+    assert tg > 0 and td > 0
+    db.grid()
+    db.degrid()
+    for k in range(3):
+        assert rel_l2(db.subgrids(k), sub0[k]) < 2e-6, f"class {k}"
+    assert rel_l2(db.visibilities_out(), vis0) < 2e-6
+    with pytest.raises(Exception):
+        db.batch.set_class_shape(3, 0, 0)

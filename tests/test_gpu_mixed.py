@@ -63,6 +63,26 @@ def test_small_config_mixed_vs_fp64(small, precision):
     assert egs <= bar and eds <= bar
 
 
+@pytest.mark.parametrize("S", [8, 16, 20, 24, 40, 48])
+def test_one_shot_tile_choice_per_subgrid_size_vs_fp64(S):
+    """The one-shot gridder picks its tiles per wave from the subgrid size (S = 8, 20: 2; 16: 4; 24: 3; 40, 48: 6,
+    with strips 8, 16, 4, 8, 8, 16 columns wide); every choice meets the split-form bar against FP64."""
+    w = observation.make_observation("small", subgrid_size=S, timesteps_per_subgrid=64)
+    w = observation.subset(w, np.arange(0, w["nr_subgrids"], 2))
+    rng = np.random.default_rng(S)
+    sub = (rng.standard_normal((w["nr_subgrids"], 4, S, S)) +
+           1j * rng.standard_normal((w["nr_subgrids"], 4, S, S))).astype(np.complex64)
+    dw = DeviceWorkload(w)
+    plan = dw.plan(precision="fp16x3")
+    dw.grid(plan)
+    egs = per_subgrid_rel_l2(dw.subgrids(), oracle_grid(w, fp64=True)).max()
+    dw.d_subgrids.copy_(upload(sub))
+    dw.degrid(plan)
+    eds = _vis_errors(w, dw.visibilities_out(), oracle_degrid(w, sub, fp64=True)).max()
+    print(f"S={S} ({w['nr_subgrids']} subgrids) fp16x3 one-shot: gridder worst {egs:.2e}, degridder worst {eds:.2e}")
+    assert egs <= SPLIT_BAR and eds <= SPLIT_BAR
+
+
 @pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
 def test_ska_low_subset_mixed_meets_5e5_where_fp32_does_not(precision):
     full = observation.make_observation("ska_low", visibilities=False)
