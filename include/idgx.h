@@ -187,8 +187,11 @@ int idgx_split_subgrids_layered(const idgx_plan *plan, const idgx_metadata *d_me
  * on d_grid (complex64 [nr_layers][4][G][G]), in each dimension:
  *   FORWARD  (image -> uv):  G[K] = sum_X I[X] exp(+2 pi i (K - G/2)(X - G/2) / G)
  *   BACKWARD (uv -> image):  I[X] = sum_K G[K] exp(-2 pi i (K - G/2)(X - G/2) / G)
- * unnormalised (BACKWARD(FORWARD(I)) = G^2 I), the signs of idgx_subgrid_fft.  G must be a multiple of 4.  The DFT
- * is rocFFT (plans cached per device, G and batch; the first call for a size allocates them). */
+ * unnormalised (BACKWARD(FORWARD(I)) = G^2 I), the signs of idgx_subgrid_fft.  G must be a multiple of 4.  For
+ * G = 1024 .. 16384 (powers of two) the transform is this library's three-pass kernel set; the first call for a G on
+ * a device allocates a plane-sized scratch (G^2 x 8 B) kept for the life of the process.  Other sizes use rocFFT
+ * between two sign-flip passes (plans cached per device, G and batch; the first call for a size allocates them).
+ * Calls that overlap on one device share the scratch, so issue them on one stream. */
 int idgx_grid_fft(const idgx_plan *plan, void *d_grid, int nr_layers, int direction, void *stream);
 
 /* W-stacking in the image domain, with w_z = (z + 0.5) w_step_in_lambda for layer z - z_min and

@@ -7,8 +7,22 @@
 //     FORWARD  (image -> uv):  G[K] = sum_X I[X] exp(+2 pi i (K - G/2)(X - G/2) / G)
 //     BACKWARD (uv -> image):  I[X] = sum_K G[K] exp(-2 pi i (K - G/2)(X - G/2) / G)
 // (the signs of the subgrid FFT, unnormalised).  Since (K - G/2)(X - G/2) = K X - (G/2)(K + X) + G^2/4, they are a
-// plain DFT between two checkerboard sign flips (-1)^(x + y) when G is a multiple of 4; the DFT itself is rocFFT
-// (hipFFT C2C, batched over correlations and layers).
+// plain DFT between two checkerboard sign flips (-1)^(x + y) when G is a multiple of 4.
+//
+// G = 1024 .. 16384 (powers of two): three passes over each plane, the sign flips folded into the first loads and
+// the last stores, each pass a batched Stockham FFT (radix-16 stages in registers, exchanges through LDS, then one
+// radix-2/4/8 stage), twiddles from the hardware sine and cosine of exact fractions of a turn.  Columns are split as G = N1 x N2 (n = N2 n1 + n2,
+// k = k1 + N1 k2):
+//   pass 1 (in place): for each n2, the N1-point transforms over n1 of rows N2 n1 + n2, times W_G^(n2 k1), back to
+//           rows N2 k1 + n2;
+//   pass 2 (to a plane-sized scratch): for each k1, the N2-point transforms over n2 of rows N2 k1 + n2, to rows
+//           k1 + N1 k2;
+//   pass 3 (scratch -> grid): the G-point transform of each row, the whole row in LDS.
+// The column passes take tiles of 64 columns (one wave reads a 512-byte row segment), so every pass reads and
+// writes whole cache lines: 3 x (read + write) of the grid, against 2 checkerboard passes plus the library
+// transform otherwise.  All three kernels are persistent (as many workgroups as are resident): each loads its next
+// tile or row into registers while it transforms the current one, and its LDS barriers do not wait for those
+// loads.  Other sizes use rocFFT (hipFFT C2C, batched over correlations and layers) between two checkerboard passes.
 //
 // W-stacking.  A subgrid with w-layer z is gridded relative to w_z = (z + 0.5) w_step (in wavelengths), so layer
 // z of the dirty image carries exp(+2 pi i w_z n(l, m)) relative to the true one, n = 1 - sqrt(1 - l^2 - m^2)
@@ -16,6 +30,7 @@
 // after multiplying each by exp(-2 pi i w_z n); from_image is its adjoint.  The phases are reduced in FP64.
 #include <hipfft/hipfft.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -113,6 +128,459 @@ __global__ __launch_bounds__(kThreads) void wstack_from_image_kernel(const float
     }
 }
 
+// ---- Stockham passes ------------------------------------------------------------------------------------------
+constexpr int kColTile = 64; // columns per column-pass tile
+
+__device__ __forceinline__ float2 f2add(float2 a, float2 b)
+{
+    return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 f2sub(float2 a, float2 b)
+{
+    return make_float2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ float2 f2mul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// cos and sin of 2 pi m / 16.
+__device__ constexpr float cos16(int m)
+{
+    constexpr float t[16] = {1.F,
+                             0.92387953251128674F,
+                             0.70710678118654752F,
+                             0.38268343236508977F,
+                             0.F,
+                             -0.38268343236508977F,
+                             -0.70710678118654752F,
+                             -0.92387953251128674F,
+                             -1.F,
+                             -0.92387953251128674F,
+                             -0.70710678118654752F,
+                             -0.38268343236508977F,
+                             0.F,
+                             0.38268343236508977F,
+                             0.70710678118654752F,
+                             0.92387953251128674F};
+    return t[m & 15];
+}
+__device__ constexpr float sin16(int m)
+{
+    return cos16(m + 12);
+}
+
+// In-register DFT of length R (a power of two <= 16), natural order in and out:
+// v[k] <- sum_n v[n] exp(SG 2 pi i n k / R), by radix-2 decimation in time.
+template <int R, int SG> __device__ __forceinline__ void dft(float2 *v)
+{
+    if constexpr (R > 1)
+    {
+        float2 e[R / 2], o[R / 2];
+#pragma unroll
+        for (int i = 0; i < R / 2; ++i)
+        {
+            e[i] = v[2 * i];
+            o[i] = v[2 * i + 1];
+        }
+        dft<R / 2, SG>(e);
+        dft<R / 2, SG>(o);
+#pragma unroll
+        for (int k = 0; k < R / 2; ++k)
+        {
+            float2 t = o[k];
+            if (4 * k == R)
+                t = make_float2(-SG * t.y, SG * t.x);
+            else if (k != 0)
+                t = f2mul(make_float2(cos16(16 * k / R), SG * sin16(16 * k / R)), t);
+            v[k] = f2add(e[k], t);
+            v[k + R / 2] = f2sub(e[k], t);
+        }
+    }
+}
+
+enum FftPass
+{
+    kPassCol1 = 0,
+    kPassCol2 = 1
+};
+
+struct FftArgs
+{
+    float2 *grid;    // one plane, G x G
+    float2 *scratch; // plane-sized
+    int G, N1, N2;
+    int log_g, log_tiles; // log2(G), log2(G / kColTile)
+};
+
+constexpr int ilog2(int n)
+{
+    return n <= 1 ? 0 : 1 + ilog2(n / 2);
+}
+
+// W_G^q = exp(SG 2 pi i q / G) for 0 <= q < G (G a power of two) from v_cos / v_sin on the exact fraction q / G of a
+// turn (absolute error <= 1.1e-7, measured on the device): no table and no memory access inside the stages.
+template <int SG> __device__ __forceinline__ float2 twiddle(int q, int log_g)
+{
+    const float turns = __builtin_ldexpf(static_cast<float>(q), -log_g);
+    return make_float2(__builtin_amdgcn_cosf(turns), SG * __builtin_amdgcn_sinf(turns));
+}
+
+// LDS exchanges.  Slots hold one float: the exchange sends the real parts, then the imaginary parts, through the
+// same array, which halves the LDS a workgroup holds.  Column tiles interleave their kColTile transforms (a wave
+// touches one row, so no two lanes share a bank); rows pad one slot per 16 so the radix-16 scatter is
+// conflict-free.
+//
+// LDS-only barrier: this wave's LDS accesses are complete, then the workgroup meets.  Unlike __syncthreads() it
+// does not wait for outstanding global loads, so the next row's prefetch stays in flight across the exchanges.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Row pass, persistent: each workgroup transforms rows blockIdx.x, blockIdx.x + gridDim.x, ... of one plane
+// (scratch -> grid), L = G points per row, one radix-16 butterfly per thread and stage.  The next row is loaded into
+// registers while the current one is transformed.  Twiddles W_L^q = exp(SG 2 pi i q / L) come from v_cos / v_sin
+// on the exact fraction q / L of a turn (absolute error <= 1.1e-7, measured), formed where they are used, so no
+// stage waits on memory and none holds a table in registers.
+template <int L, int SG> __global__ __launch_bounds__(L / 16) void fft_row_kernel(FftArgs A)
+{
+    constexpr int kLog = ilog2(L);
+    constexpr int kStages16 = kLog / 4;
+    constexpr int kRl = 1 << (kLog % 4);
+    constexpr int kQ = L / 16;
+    static_assert(kStages16 >= 2, "L = 16^a x {1, 2, 4, 8}, a >= 2");
+    // LDS slot of element i is i + i / 16.  Every access below is a per-thread base plus a compile-time offset:
+    // for the elements j + t kQ (kQ a multiple of 16) the slot is slot(j) + t (kQ + kQ / 16).
+    constexpr int kQs = kQ + kQ / 16;
+    constexpr int kRs = L / kRl + L / (16 * kRl);
+    __shared__ float buf[L + L / 16];
+
+    float2 nxt[16];
+    int y = blockIdx.x;
+    if (y < L)
+    {
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            nxt[r] = A.scratch[static_cast<size_t>(y) * L + threadIdx.x + r * kQ];
+    }
+    for (; y < L; y += gridDim.x)
+    {
+        // An opaque copy of the thread index per row: what derives from it (twiddles, slots) is formed where it is
+        // used instead of being hoisted out of the row loop and held in registers for every row.
+        int j0 = threadIdx.x;
+        asm volatile("" : "+v"(j0));
+        auto tw = [&](int qq) { return twiddle<SG>(qq & (L - 1), kLog); };
+        float2 v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            v[r] = nxt[r];
+        const int yn = y + gridDim.x;
+        if (yn < L)
+        {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                nxt[r] = A.scratch[static_cast<size_t>(yn) * L + j0 + r * kQ];
+        }
+        int Ns = 1;
+#pragma unroll
+        for (int st = 0; st < kStages16; ++st)
+        {
+            if (st > 0)
+            {
+                const int m = (j0 & (Ns - 1)) * (L / (16 * Ns));
+#pragma unroll
+                for (int r = 1; r < 16; ++r)
+                    v[r] = f2mul(v[r], tw(m * r));
+            }
+            dft<16, SG>(v);
+            if (st == kStages16 - 1 && kRl == 1)
+            {
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                {
+                    const int k = j0 + r * Ns; // k odd <=> j0 odd
+                    A.grid[static_cast<size_t>(y) * L + k] = (k & 1) ? make_float2(-v[r].x, -v[r].y) : v[r];
+                }
+            }
+            else
+            {
+                // Output r of butterfly j0 goes to element d + r Ns: slot wb + r (Ns + Ns / 16), or 17 j0 + r when
+                // Ns = 1.  The next stage reads elements j0 + t kQ (slot rb + t kQs), or for the radix-kRl stage
+                // j0 + b kQ + r (L / kRl) (slot rb + b kQs + r kRs).
+                const int d = (j0 / Ns) * Ns * 16 + (j0 & (Ns - 1));
+                const int wb = d + (d >> 4), ws = Ns == 1 ? 1 : Ns + Ns / 16;
+                const int rb = j0 + (j0 >> 4);
+                const bool next16 = st + 1 < kStages16;
+                auto rslot = [&](int t) { return next16 ? rb + t * kQs : rb + (t / kRl) * kQs + (t % kRl) * kRs; };
+                float re[16];
+                lds_barrier(); // the previous exchange's reads are done
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    buf[wb + r * ws] = v[r].x;
+                lds_barrier();
+#pragma unroll
+                for (int t = 0; t < 16; ++t)
+                    re[t] = buf[rslot(t)];
+                lds_barrier();
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    buf[wb + r * ws] = v[r].y;
+                lds_barrier();
+#pragma unroll
+                for (int t = 0; t < 16; ++t)
+                    v[t] = make_float2(re[t], buf[rslot(t)]);
+            }
+            Ns *= 16;
+        }
+        if constexpr (kRl > 1)
+        {
+#pragma unroll
+            for (int b = 0; b < 16 / kRl; ++b)
+            {
+                const int j = j0 + b * kQ;
+                float2 u[kRl];
+#pragma unroll
+                for (int r = 0; r < kRl; ++r)
+                    u[r] = v[b * kRl + r];
+#pragma unroll
+                for (int r = 1; r < kRl; ++r)
+                    u[r] = f2mul(u[r], tw(j * r));
+                dft<kRl, SG>(u);
+#pragma unroll
+                for (int r = 0; r < kRl; ++r)
+                {
+                    const int k = j + r * Ns;
+                    A.grid[static_cast<size_t>(y) * L + k] = (k & 1) ? make_float2(-u[r].x, -u[r].y) : u[r];
+                }
+            }
+        }
+    }
+}
+
+// Column passes, persistent: tiles of kColTile columns x one transform set (C1: rows N2 n1 + n2 for one n2; C2:
+// rows N2 k1 + n2 for one k1), tile index = x tile + (G / kColTile) x (n2 or k1), taken blockIdx.x, blockIdx.x +
+// gridDim.x, ...  The next tile is loaded into registers while the current one is transformed.  L = 16 kRl:
+// a radix-16 stage from global memory, one LDS exchange, a radix-kRl stage to global memory.
+template <int PASS, int L, int SG>
+__global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, int nr_tiles)
+{
+    constexpr int kLog = ilog2(L);
+    constexpr int kRl = 1 << (kLog % 4);
+    constexpr int kQ = L / 16;
+    static_assert(kLog / 4 == 1 && kRl > 1, "L = 32, 64 or 128");
+    __shared__ float buf[L * kColTile];
+    const int c = threadIdx.x & (kColTile - 1);
+    const int G = A.G;
+    const int tmask = (1 << A.log_tiles) - 1;
+    auto element = [&](int tile, int i) {
+        const int x = (tile & tmask) * kColTile + c, q = tile >> A.log_tiles;
+        const int y = PASS == kPassCol1 ? A.N2 * i + q : A.N2 * q + i;
+        return static_cast<size_t>(y) * G + x;
+    };
+
+    float2 nxt[16];
+    int tile = blockIdx.x;
+    if (tile < nr_tiles)
+    {
+        const int j = threadIdx.x / kColTile;
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            nxt[r] = A.grid[element(tile, j + r * kQ)];
+    }
+    for (; tile < nr_tiles; tile += gridDim.x)
+    {
+        int j0 = threadIdx.x / kColTile; // opaque per tile: keeps twiddles and slots out of registers across tiles
+        asm volatile("" : "+v"(j0));
+        const int x = (tile & tmask) * kColTile + c, q = tile >> A.log_tiles;
+        float2 v[16];
+        // C1 applies the input sign (-1)^(x + y), y = N2 n1 + n2 with N2 even: (-1)^(x + n2) for the whole thread.
+        const float sg_in = PASS == kPassCol1 && ((x + q) & 1) ? -1.F : 1.F;
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            v[r] = make_float2(sg_in * nxt[r].x, sg_in * nxt[r].y);
+        const int tn = tile + gridDim.x;
+        if (tn < nr_tiles)
+        {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                nxt[r] = A.grid[element(tn, j0 + r * kQ)];
+        }
+        dft<16, SG>(v);
+        // Output r of butterfly j0 is element 16 j0 + r; the radix-kRl stage takes elements j0 + b kQ + r (L / kRl).
+        auto rslot = [&](int t) { return (j0 + (t / kRl) * kQ + (t % kRl) * (L / kRl)) * kColTile + c; };
+        float re[16];
+        lds_barrier(); // the previous tile's reads are done
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            buf[(16 * j0 + r) * kColTile + c] = v[r].x;
+        lds_barrier();
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+            re[t] = buf[rslot(t)];
+        lds_barrier();
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            buf[(16 * j0 + r) * kColTile + c] = v[r].y;
+        lds_barrier();
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+            v[t] = make_float2(re[t], buf[rslot(t)]);
+#pragma unroll
+        for (int b = 0; b < 16 / kRl; ++b)
+        {
+            const int j = j0 + b * kQ;
+            float2 u[kRl];
+#pragma unroll
+            for (int r = 0; r < kRl; ++r)
+                u[r] = v[b * kRl + r];
+#pragma unroll
+            for (int r = 1; r < kRl; ++r)
+                u[r] = f2mul(u[r], twiddle<SG>(j * r, kLog));
+            dft<kRl, SG>(u);
+#pragma unroll
+            for (int r = 0; r < kRl; ++r)
+            {
+                const int k = j + r * (L / kRl);
+                if constexpr (PASS == kPassCol1)
+                    A.grid[static_cast<size_t>(A.N2 * k + q) * G + x] = f2mul(u[r], twiddle<SG>(q * k, A.log_g));
+                else
+                {
+                    const int row = q + A.N1 * k;
+                    A.scratch[static_cast<size_t>(row) * G + x] =
+                        (row & 1) ? make_float2(-u[r].x, -u[r].y) : u[r];
+                }
+            }
+        }
+    }
+}
+
+// Persistent grid: as many workgroups as are resident at once (at most one per tile or row).
+hipError_t resident_blocks(const void *fn, int threads, int work, int &blocks)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0);
+    blocks = std::max(1, std::min(work, cus * std::max(per_cu, 1)));
+    return e;
+}
+
+template <int PASS, int L, int SG> hipError_t launch_cols(const FftArgs &A, int nr_tiles, hipStream_t st)
+{
+    constexpr int threads = kColTile * L / 16;
+    int blocks = 0;
+    const hipError_t e = resident_blocks(reinterpret_cast<const void *>(&fft_col_kernel<PASS, L, SG>), threads,
+                                         nr_tiles, blocks);
+    if (e != hipSuccess)
+        return e;
+    fft_col_kernel<PASS, L, SG><<<blocks, threads, 0, st>>>(A, nr_tiles);
+    return hipGetLastError();
+}
+
+template <int PASS, int SG> hipError_t launch_col_pass(const FftArgs &A, int L, int nr_tiles, hipStream_t st)
+{
+    switch (L)
+    {
+    case 32:
+        return launch_cols<PASS, 32, SG>(A, nr_tiles, st);
+    case 64:
+        return launch_cols<PASS, 64, SG>(A, nr_tiles, st);
+    case 128:
+        return launch_cols<PASS, 128, SG>(A, nr_tiles, st);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+template <int L, int SG> hipError_t launch_rows(const FftArgs &A, hipStream_t st)
+{
+    int blocks = 0;
+    const hipError_t e =
+        resident_blocks(reinterpret_cast<const void *>(&fft_row_kernel<L, SG>), L / 16, L, blocks);
+    if (e != hipSuccess)
+        return e;
+    fft_row_kernel<L, SG><<<blocks, L / 16, 0, st>>>(A);
+    return hipGetLastError();
+}
+
+template <int SG> hipError_t launch_row_pass(const FftArgs &A, hipStream_t st)
+{
+    switch (A.G)
+    {
+    case 1024:
+        return launch_rows<1024, SG>(A, st);
+    case 2048:
+        return launch_rows<2048, SG>(A, st);
+    case 4096:
+        return launch_rows<4096, SG>(A, st);
+    case 8192:
+        return launch_rows<8192, SG>(A, st);
+    case 16384:
+        return launch_rows<16384, SG>(A, st);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+bool stockham_size(int G)
+{
+    return G >= 1024 && G <= 16384 && (G & (G - 1)) == 0;
+}
+
+// Scratch plane per (device, G), created on first use and kept for the life of the process.
+std::mutex g_scratch_mutex;
+std::map<std::pair<int, int>, float2 *> g_scratch;
+
+hipError_t fft_scratch(int G, float2 *&out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    const auto key = std::make_pair(dev, G);
+    auto it = g_scratch.find(key);
+    if (it != g_scratch.end())
+    {
+        out = it->second;
+        return hipSuccess;
+    }
+    float2 *p = nullptr;
+    e = hipMalloc(&p, static_cast<size_t>(G) * G * sizeof(float2));
+    if (e != hipSuccess)
+        return e;
+    g_scratch[key] = p;
+    out = p;
+    return hipSuccess;
+}
+
+template <int SG> hipError_t stockham_grid_fft(float2 *grid, int G, long long planes, hipStream_t st)
+{
+    FftArgs A;
+    hipError_t e = fft_scratch(G, A.scratch);
+    if (e != hipSuccess)
+        return e;
+    A.G = G;
+    A.log_g = ilog2(G);
+    A.N1 = 1 << (ilog2(G) / 2);
+    A.N2 = G / A.N1;
+    A.log_tiles = ilog2(G / kColTile);
+    const int tiles = G / kColTile;
+    for (long long p = 0; p < planes && e == hipSuccess; ++p)
+    {
+        A.grid = grid + static_cast<size_t>(p) * G * G;
+        e = launch_col_pass<kPassCol1, SG>(A, A.N1, tiles * A.N2, st);
+        if (e == hipSuccess)
+            e = launch_col_pass<kPassCol2, SG>(A, A.N2, tiles * A.N1, st);
+        if (e == hipSuccess)
+            e = launch_row_pass<SG>(A, st);
+    }
+    return e;
+}
+
 // hipFFT plans per (device, G, batch), created on first use and kept for the life of the process.
 std::mutex g_plan_mutex;
 std::map<std::tuple<int, int, long long>, hipfftHandle> g_plans;
@@ -162,6 +630,9 @@ hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward,
     const long long planes = nr_layers * kCorr;
     if (planes > 0x7fffffffLL || static_cast<long long>(G) * G > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
+    if (stockham_size(G))
+        return forward ? stockham_grid_fft<1>(static_cast<float2 *>(grid), G, planes, st)
+                       : stockham_grid_fft<-1>(static_cast<float2 *>(grid), G, planes, st);
     hipfftHandle plan;
     hipError_t e = fft_plan(G, planes, plan);
     if (e != hipSuccess)

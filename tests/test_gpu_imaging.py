@@ -7,7 +7,7 @@ import pytest
 
 import oracle
 from helpers import rel_l2
-from idgx import FFT_BACKWARD, FFT_FORWARD, METADATA_DTYPE, CHANNELS_DTYPE, observation
+from idgx import FFT_BACKWARD, FFT_FORWARD, METADATA_DTYPE, CHANNELS_DTYPE, grid_fft, observation
 from idgx.device import DeviceWorkload, download, upload
 
 pytestmark = pytest.mark.gpu
@@ -50,6 +50,75 @@ def test_grid_fft_matches_the_centred_dft_and_round_trips():
     e_b = max(rel_l2(bwd[l, p], _centred(x[l, p].astype(np.complex128), -1)) for l in range(L) for p in range(4))
     print(f"grid FFT vs FP64 DFT: forward {e_f:.2e}, backward {e_b:.2e}, round trip {rel_l2(rt, x):.2e}")
     assert e_f < 1e-6 and e_b < 1e-6 and rel_l2(rt, x) < 1e-6
+
+
+def _centred_np(x, sign):
+    """The centred transform in FP64 through numpy's FFT between the same sign flips (G % 4 == 0)."""
+    G = x.shape[-1]
+    cc = np.outer((-1.0) ** np.arange(G), (-1.0) ** np.arange(G))
+    y = np.fft.ifft2(x * cc) * (G * G) if sign > 0 else np.fft.fft2(x * cc)
+    return y * cc
+
+
+def _fft_plan(G):
+    md = np.zeros(1, METADATA_DTYPE)
+    md["nr_timesteps"] = 1
+    md["station2"] = 1
+    return DeviceWorkload(_grid_workload(G, md)).plan()
+
+
+@pytest.mark.parametrize("G", [1024, 2048, 4096, 8192])
+def test_three_pass_grid_fft_matches_numpy_fp64(G):
+    """G = 1024 .. 16384 run the hand-written three-pass transform (column passes N1 x N2 = 32x32, 32x64, 64x64,
+    64x128): every correlation plane transformed, planes 0 and 3 compared with numpy's FP64 FFT both ways, and the
+    round trip."""
+    rng = np.random.default_rng(G)
+    x = (rng.standard_normal((4, G, G), np.float32) + 1j * rng.standard_normal((4, G, G), np.float32)).astype(
+        np.complex64)
+    plan = _fft_plan(G)
+    d = upload(x)
+    grid_fft(plan, d, 1, FFT_FORWARD)
+    fwd = download(d, np.complex64, x.shape)
+    e_f = max(rel_l2(fwd[p], _centred_np(x[p].astype(np.complex128), +1)) for p in (0, 3))
+    grid_fft(plan, d, 1, FFT_BACKWARD)
+    e_rt = rel_l2(download(d, np.complex64, x.shape) / np.float32(G * G), x)
+    d2 = upload(x)
+    grid_fft(plan, d2, 1, FFT_BACKWARD)
+    bwd = download(d2, np.complex64, x.shape)
+    e_b = max(rel_l2(bwd[p], _centred_np(x[p].astype(np.complex128), -1)) for p in (0, 3))
+    print(f"G={G} three-pass grid FFT vs numpy FP64: forward {e_f:.2e}, backward {e_b:.2e}, round trip {e_rt:.2e}")
+    assert e_f < 1e-6 and e_b < 1e-6 and e_rt < 1e-6
+
+
+def test_three_pass_grid_fft_at_16384():
+    """The 16384^2 master grid (8.6 GB, the full-cycle configuration): four output rows of plane 0 against the
+    FP64 centred DFT (column sums by matrix-vector product, then numpy's row FFT), Parseval on every plane, and the
+    round trip, all on the device."""
+    import torch
+
+    G = 16384
+    torch.manual_seed(5)
+    x = torch.randn((4, G, G, 2), device="cuda", dtype=torch.float32)
+    x0 = x[0].cpu().numpy().view(np.complex64)[..., 0]
+    e_in = [float(torch.linalg.vector_norm(x[p], dtype=torch.float64)) ** 2 for p in range(4)]
+    d = x.clone()
+    grid_fft(_fft_plan(G), d, 1, FFT_FORWARD)
+    torch.cuda.synchronize()
+    c = (-1.0) ** np.arange(G)
+    worst_row = 0.0
+    for L in (0, 1, G // 2, 12345):
+        w = np.exp(2j * np.pi * (((L - G // 2) * (np.arange(G) - G // 2)) % G) / G)
+        v = w @ x0  # sum over rows Y of x[Y, X] exp(+2 pi i (L - G/2)(Y - G/2) / G)
+        ref = np.fft.ifft(v * c) * G * c
+        got = d[0, L].cpu().numpy().view(np.complex64)[..., 0]
+        worst_row = max(worst_row, rel_l2(got, ref))
+    e_out = [float(torch.linalg.vector_norm(d[p], dtype=torch.float64)) ** 2 for p in range(4)]
+    parseval = max(abs(e_out[p] / (e_in[p] * G * G) - 1.0) for p in range(4))
+    grid_fft(_fft_plan(G), d, 1, FFT_BACKWARD)
+    d /= G * G
+    rt = float(torch.linalg.vector_norm(d - x, dtype=torch.float64) / torch.linalg.vector_norm(x, dtype=torch.float64))
+    print(f"G=16384 three-pass grid FFT: worst row vs FP64 {worst_row:.2e}, Parseval {parseval:.2e}, round trip {rt:.2e}")
+    assert worst_row < 1e-6 and parseval < 1e-5 and rt < 1e-6
 
 
 def test_layered_adder_and_splitter_match_the_placement_oracle_per_layer():
