@@ -240,9 +240,9 @@ __device__ __forceinline__ void lds_barrier()
 
 // Row pass, persistent: each workgroup transforms rows blockIdx.x, blockIdx.x + gridDim.x, ... of one plane
 // (scratch -> grid), L = G points per row, one radix-16 butterfly per thread and stage.  The next row is loaded into
-// registers while the current one is transformed.  Twiddles W_L^q = exp(SG 2 pi i q / L) come from v_cos / v_sin
-// on the exact fraction q / L of a turn (absolute error <= 1.1e-7, measured), formed where they are used, so no
-// stage waits on memory and none holds a table in registers.
+// registers while the current one is transformed.  Twiddles W_L^q = exp(SG 2 pi i q / L) come from a two-level
+// table in LDS, formed where they are used, so no stage waits on global memory and none holds twiddles in
+// registers across rows.
 template <int L, int SG> __global__ __launch_bounds__(L / 16) void fft_row_kernel(FftArgs A)
 {
     constexpr int kLog = ilog2(L);
@@ -255,6 +255,21 @@ template <int L, int SG> __global__ __launch_bounds__(L / 16) void fft_row_kerne
     constexpr int kQs = kQ + kQ / 16;
     constexpr int kRs = L / kRl + L / (16 * kRl);
     __shared__ float buf[L + L / 16];
+    // Twiddles W_L^q = T_hi[q / 64] T_lo[q % 64]: two LDS reads and a complex product instead of a sine and a cosine
+    // at quarter rate.  The L / 64 + 64 entries are rounded from FP64 values once per workgroup.
+    __shared__ float2 t_hi[L / 64], t_lo[64];
+    for (int i = threadIdx.x; i < L / 64 + 64; i += kQ)
+    {
+        const int qq = i < L / 64 ? 64 * i : i - L / 64;
+        double sn, cs;
+        sincospi(2.0 * qq / L, &sn, &cs);
+        const float2 w = make_float2(static_cast<float>(cs), static_cast<float>(SG * sn));
+        if (i < L / 64)
+            t_hi[i] = w;
+        else
+            t_lo[i - L / 64] = w;
+    }
+    __syncthreads();
 
     float2 nxt[16];
     int y = blockIdx.x;
@@ -270,7 +285,10 @@ template <int L, int SG> __global__ __launch_bounds__(L / 16) void fft_row_kerne
         // used instead of being hoisted out of the row loop and held in registers for every row.
         int j0 = threadIdx.x;
         asm volatile("" : "+v"(j0));
-        auto tw = [&](int qq) { return twiddle<SG>(qq & (L - 1), kLog); };
+        auto tw = [&](int qq) {
+            qq &= L - 1;
+            return f2mul(t_hi[qq >> 6], t_lo[qq & 63]);
+        };
         float2 v[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r)

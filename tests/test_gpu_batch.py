@@ -152,6 +152,38 @@ def test_graph_replays_match_eager():
         assert float(x) == 1000.0
 
 
+def test_graph_of_a_mixed_size_batch_replays_its_per_class_launches():
+    """Three classes with different tile shapes: the captured graph holds the counter reset, the degridder pixel
+    preparation and one persistent launch per class and direction; every replay equals the eager run bit for bit."""
+    import torch
+
+    w = observation.make_observation("small")
+    db = DeviceBatch(_mixed_size_classes(w), precision="fp16x3")
+    shapes = Batch.shapes()
+    for k, (g, d) in enumerate([((3, 256, 0), (4, 256, 0)), ((8, 256, 0), (6, 256, 0)), ((6, 256, 0), (2, 256, 0))]):
+        db.batch.set_class_shape(k, shapes.index(g), shapes.index(d))
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        db.grid(stream)
+        db.degrid(stream)
+    stream.synchronize()
+    sub0 = [db.subgrids(k) for k in range(3)]
+    vis0 = db.visibilities_out()
+    db.batch.capture(db.data, STAGE_GRID | STAGE_DEGRID, stream)
+    for r in range(3):
+        for c in db.classes:
+            c["d_subgrids"].zero_()
+        db.d_vis_out.zero_()
+        torch.cuda.synchronize()
+        db.batch.launch(stream)
+        stream.synchronize()
+        for k in range(3):
+            assert db.subgrids(k).tobytes() == sub0[k].tobytes(), f"replay {r}, class {k}"
+        assert db.visibilities_out().tobytes() == vis0.tobytes(), f"replay {r}"
+        x = torch.ones(1000, device="cuda").sum()  # eager work between replays
+        assert float(x) == 1000.0
+
+
 def test_autotuner_picks_a_shape_and_keeps_results():
     w = observation.make_observation("small")
     db = DeviceBatch([w], precision="bf16x3")
