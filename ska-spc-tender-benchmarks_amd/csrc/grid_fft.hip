@@ -11,7 +11,8 @@
 //
 // G = 1024 .. 16384 (powers of two): three passes over each plane, the sign flips folded into the first loads and
 // the last stores, each pass a batched Stockham FFT (radix-16 stages in registers, exchanges through LDS, then one
-// radix-2/4/8 stage), twiddles from the hardware sine and cosine of exact fractions of a turn.  Columns are split as G = N1 x N2 (n = N2 n1 + n2,
+// radix-2/4/8 stage); twiddles from a two-level LDS table of FP64-rounded values (rows) or from the hardware sine
+// and cosine of exact fractions of a turn (columns).  Columns are split as G = N1 x N2 (n = N2 n1 + n2,
 // k = k1 + N1 k2):
 //   pass 1 (in place): for each n2, the N1-point transforms over n1 of rows N2 n1 + n2, times W_G^(n2 k1), back to
 //           rows N2 k1 + n2;
