@@ -230,14 +230,7 @@ template <int SG> __device__ __forceinline__ float2 twiddle(int q, int log_g)
 // LDS exchanges.  Slots hold one float: the exchange sends the real parts, then the imaginary parts, through the
 // same array, which halves the LDS a workgroup holds.  Column tiles interleave their kColTile transforms (a wave
 // touches one row, so no two lanes share a bank); rows pad one slot per 16 so the radix-16 scatter is
-// conflict-free.
-//
-// LDS-only barrier: this wave's LDS accesses are complete, then the workgroup meets.  Unlike __syncthreads() it
-// does not wait for outstanding global loads, so the next row's prefetch stays in flight across the exchanges.
-__device__ __forceinline__ void lds_barrier()
-{
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
+// conflict-free.  The exchanges use lds_barrier(), so a prefetch of the next tile stays in flight across them.
 
 // Row pass, persistent: each workgroup transforms rows blockIdx.x, blockIdx.x + gridDim.x, ... of one plane
 // (scratch -> grid), L = G points per row, one radix-16 butterfly per thread and stage.  The next row is loaded into

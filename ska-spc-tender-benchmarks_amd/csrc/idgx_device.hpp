@@ -114,6 +114,14 @@ template <int MODE> __device__ __forceinline__ void sincos_rad(float x, float &s
     }
 }
 
+// LDS-only workgroup barrier: this wave's LDS accesses are complete, then the workgroup meets.  Unlike
+// __syncthreads() it does not wait for outstanding global loads, so a persistent kernel's prefetch of its next
+// item stays in flight across the barriers of the current one (global memory is not exchanged through it).
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // ---- 2x2 Jones algebra, row-major {xx, xy, yx, yy} -------------------------------------------------------------
 
 __device__ __forceinline__ cfloat cmul(cfloat a, cfloat b)
