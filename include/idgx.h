@@ -194,6 +194,21 @@ int idgx_split_subgrids_layered(const idgx_plan *plan, const idgx_metadata *d_me
  * Calls that overlap on one device share the scratch, so issue them on one stream. */
 int idgx_grid_fft(const idgx_plan *plan, void *d_grid, int nr_layers, int direction, void *stream);
 
+/* Plain batched 2-D DFT, the transform of the reference's FFT benchmark (heFFTe fft2d, fft/inc/fft_benchmark.h:65-99,
+ * run by fft/src/fft_benchmark.cpp:260-330): for b < batch,
+ *   d_out[b][ky][kx] = sum_{y < ny, x < nx} d_in[b][y][x] exp(sign 2 pi i (ky y / ny + kx x / nx)),
+ * unnormalised, sign = -1 (heFFTe forward) or +1 (backward); complex64 planes, or complex128 with
+ * precision = IDGX_FFT_FP64.  d_in == d_out transforms in place.  FP32 with nx and ny powers of two in
+ * 1024 .. 16384 runs this library's three-pass kernels (the first call for a plane size on a device allocates a
+ * plane-sized scratch, kept until the process ends); other shapes and FP64 run rocFFT (the first call for a shape
+ * creates its plan).  Calls that overlap on one device share the scratch, so issue them on one stream. */
+enum
+{
+    IDGX_FFT_FP32 = 0,
+    IDGX_FFT_FP64 = 1
+};
+int idgx_fft2d(const void *d_in, void *d_out, int nx, int ny, int64_t batch, int sign, int precision, void *stream);
+
 /* W-stacking in the image domain, with w_z = (z + 0.5) w_step_in_lambda for layer z - z_min and
  * n = 1 - sqrt(1 - l^2 - m^2) at image pixel (X, Y):
  *   to_image:   d_image[pol][Y][X] = sum_z d_layers[z - z_min][pol][Y][X] exp(-2 pi i w_z n)

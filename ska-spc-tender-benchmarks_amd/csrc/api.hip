@@ -308,6 +308,20 @@ extern "C" int idgx_grid_fft(const idgx_plan *plan, void *d_grid, int nr_layers,
                                            static_cast<hipStream_t>(stream)));
 }
 
+extern "C" int idgx_fft2d(const void *d_in, void *d_out, int nx, int ny, int64_t batch, int sign, int precision,
+                          void *stream)
+{
+    if (nx < 1 || ny < 1 || batch < 1 || (sign != 1 && sign != -1) ||
+        (precision != IDGX_FFT_FP32 && precision != IDGX_FFT_FP64))
+        return IDGX_ERR_INVALID_ARGUMENT;
+    if (!idgx::fft2d_supported(nx, ny) || batch > 0x7fffffffLL)
+        return IDGX_ERR_UNSUPPORTED;
+    if (!d_in || !d_out)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    return to_status(idgx::launch_fft2d(d_in, d_out, nx, ny, batch, sign, precision == IDGX_FFT_FP64,
+                                        static_cast<hipStream_t>(stream)));
+}
+
 extern "C" int idgx_wstack_to_image(const idgx_plan *plan, const void *d_layers, int z_min, int nr_layers,
                                     void *d_image, void *stream)
 {

@@ -208,10 +208,15 @@ enum FftPass
 
 struct FftArgs
 {
-    float2 *grid;    // one plane, G x G
-    float2 *scratch; // plane-sized
-    int G, N1, N2;
-    int log_g, log_tiles; // log2(G), log2(G / kColTile)
+    const float2 *src; // nplanes planes of gy rows of gx: column pass 1 input (the same planes as grid when in place)
+    float2 *grid;      // column pass 1 output, column pass 2 input, row pass output
+    float2 *scratch;   // nplanes planes
+    int gx, gy;        // row length, column length
+    int N1, N2;        // gy = N1 x N2
+    int log_gy, log_tiles; // log2(gy), log2(gx / kColTile)
+    int log_n1, log_n2;
+    int nplanes;       // planes of this launch
+    int centred;       // 1: the (-1)^(x + y) sign flips of the centred transform, 0: a plain DFT
 };
 
 constexpr int ilog2(int n)
@@ -265,15 +270,17 @@ template <int L, int SG> __global__ __launch_bounds__(L / 16) void fft_row_kerne
     }
     __syncthreads();
 
+    const bool flip = A.centred != 0;
+    const int rows = A.nplanes * A.gy; // planes are contiguous: global row y is row y % gy of plane y / gy
     float2 nxt[16];
     int y = blockIdx.x;
-    if (y < L)
+    if (y < rows)
     {
 #pragma unroll
         for (int r = 0; r < 16; ++r)
             nxt[r] = A.scratch[static_cast<size_t>(y) * L + threadIdx.x + r * kQ];
     }
-    for (; y < L; y += gridDim.x)
+    for (; y < rows; y += gridDim.x)
     {
         // An opaque copy of the thread index per row: what derives from it (twiddles, slots) is formed where it is
         // used instead of being hoisted out of the row loop and held in registers for every row.
@@ -288,7 +295,7 @@ template <int L, int SG> __global__ __launch_bounds__(L / 16) void fft_row_kerne
         for (int r = 0; r < 16; ++r)
             v[r] = nxt[r];
         const int yn = y + gridDim.x;
-        if (yn < L)
+        if (yn < rows)
         {
 #pragma unroll
             for (int r = 0; r < 16; ++r)
@@ -312,7 +319,8 @@ template <int L, int SG> __global__ __launch_bounds__(L / 16) void fft_row_kerne
                 for (int r = 0; r < 16; ++r)
                 {
                     const int k = j0 + r * Ns; // k odd <=> j0 odd
-                    A.grid[static_cast<size_t>(y) * L + k] = (k & 1) ? make_float2(-v[r].x, -v[r].y) : v[r];
+                    A.grid[static_cast<size_t>(y) * L + k] =
+                        (flip && (k & 1)) ? make_float2(-v[r].x, -v[r].y) : v[r];
                 }
             }
             else
@@ -363,7 +371,8 @@ template <int L, int SG> __global__ __launch_bounds__(L / 16) void fft_row_kerne
                 for (int r = 0; r < kRl; ++r)
                 {
                     const int k = j + r * Ns;
-                    A.grid[static_cast<size_t>(y) * L + k] = (k & 1) ? make_float2(-u[r].x, -u[r].y) : u[r];
+                    A.grid[static_cast<size_t>(y) * L + k] =
+                        (flip && (k & 1)) ? make_float2(-u[r].x, -u[r].y) : u[r];
                 }
             }
         }
@@ -383,12 +392,17 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
     static_assert(kLog / 4 == 1 && kRl > 1, "L = 32, 64 or 128");
     __shared__ float buf[L * kColTile];
     const int c = threadIdx.x & (kColTile - 1);
-    const int G = A.G;
+    const int G = A.gx; // row stride
     const int tmask = (1 << A.log_tiles) - 1;
+    // Tile index = plane x (tiles per plane) + (n2 or k1) x (G / kColTile) + column tile.
+    const int log_tpp = A.log_tiles + (PASS == kPassCol1 ? A.log_n2 : A.log_n1);
+    const size_t plane_elems = static_cast<size_t>(G) * A.gy;
+    const float2 *in = PASS == kPassCol1 ? A.src : A.grid;
     auto element = [&](int tile, int i) {
-        const int x = (tile & tmask) * kColTile + c, q = tile >> A.log_tiles;
+        const int pl = tile >> log_tpp, t = tile & ((1 << log_tpp) - 1);
+        const int x = (t & tmask) * kColTile + c, q = t >> A.log_tiles;
         const int y = PASS == kPassCol1 ? A.N2 * i + q : A.N2 * q + i;
-        return static_cast<size_t>(y) * G + x;
+        return pl * plane_elems + static_cast<size_t>(y) * G + x;
     };
 
     float2 nxt[16];
@@ -398,16 +412,18 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
         const int j = threadIdx.x / kColTile;
 #pragma unroll
         for (int r = 0; r < 16; ++r)
-            nxt[r] = A.grid[element(tile, j + r * kQ)];
+            nxt[r] = in[element(tile, j + r * kQ)];
     }
     for (; tile < nr_tiles; tile += gridDim.x)
     {
         int j0 = threadIdx.x / kColTile; // opaque per tile: keeps twiddles and slots out of registers across tiles
         asm volatile("" : "+v"(j0));
-        const int x = (tile & tmask) * kColTile + c, q = tile >> A.log_tiles;
+        const int pl = tile >> log_tpp, t = tile & ((1 << log_tpp) - 1);
+        const int x = (t & tmask) * kColTile + c, q = t >> A.log_tiles;
+        const size_t base = pl * plane_elems;
         float2 v[16];
         // C1 applies the input sign (-1)^(x + y), y = N2 n1 + n2 with N2 even: (-1)^(x + n2) for the whole thread.
-        const float sg_in = PASS == kPassCol1 && ((x + q) & 1) ? -1.F : 1.F;
+        const float sg_in = PASS == kPassCol1 && A.centred && ((x + q) & 1) ? -1.F : 1.F;
 #pragma unroll
         for (int r = 0; r < 16; ++r)
             v[r] = make_float2(sg_in * nxt[r].x, sg_in * nxt[r].y);
@@ -416,7 +432,7 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
         {
 #pragma unroll
             for (int r = 0; r < 16; ++r)
-                nxt[r] = A.grid[element(tn, j0 + r * kQ)];
+                nxt[r] = in[element(tn, j0 + r * kQ)];
         }
         dft<16, SG>(v);
         // Output r of butterfly j0 is element 16 j0 + r; the radix-kRl stage takes elements j0 + b kQ + r (L / kRl).
@@ -455,12 +471,13 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
             {
                 const int k = j + r * (L / kRl);
                 if constexpr (PASS == kPassCol1)
-                    A.grid[static_cast<size_t>(A.N2 * k + q) * G + x] = f2mul(u[r], twiddle<SG>(q * k, A.log_g));
+                    A.grid[base + static_cast<size_t>(A.N2 * k + q) * G + x] =
+                        f2mul(u[r], twiddle<SG>(q * k, A.log_gy));
                 else
                 {
                     const int row = q + A.N1 * k;
-                    A.scratch[static_cast<size_t>(row) * G + x] =
-                        (row & 1) ? make_float2(-u[r].x, -u[r].y) : u[r];
+                    A.scratch[base + static_cast<size_t>(row) * G + x] =
+                        (A.centred && (row & 1)) ? make_float2(-u[r].x, -u[r].y) : u[r];
                 }
             }
         }
@@ -511,7 +528,7 @@ template <int L, int SG> hipError_t launch_rows(const FftArgs &A, hipStream_t st
 {
     int blocks = 0;
     const hipError_t e =
-        resident_blocks(reinterpret_cast<const void *>(&fft_row_kernel<L, SG>), L / 16, L, blocks);
+        resident_blocks(reinterpret_cast<const void *>(&fft_row_kernel<L, SG>), L / 16, A.nplanes * A.gy, blocks);
     if (e != hipSuccess)
         return e;
     fft_row_kernel<L, SG><<<blocks, L / 16, 0, st>>>(A);
@@ -520,7 +537,7 @@ template <int L, int SG> hipError_t launch_rows(const FftArgs &A, hipStream_t st
 
 template <int SG> hipError_t launch_row_pass(const FftArgs &A, hipStream_t st)
 {
-    switch (A.G)
+    switch (A.gx)
     {
     case 1024:
         return launch_rows<1024, SG>(A, st);
@@ -542,18 +559,18 @@ bool stockham_size(int G)
     return G >= 1024 && G <= 16384 && (G & (G - 1)) == 0;
 }
 
-// Scratch plane per (device, G), created on first use and kept for the life of the process.
+// Scratch plane per (device, plane size), created on first use and kept for the life of the process.
 std::mutex g_scratch_mutex;
-std::map<std::pair<int, int>, float2 *> g_scratch;
+std::map<std::pair<int, long long>, float2 *> g_scratch;
 
-hipError_t fft_scratch(int G, float2 *&out)
+hipError_t fft_scratch(long long elements, float2 *&out)
 {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess)
         return e;
     std::lock_guard<std::mutex> lock(g_scratch_mutex);
-    const auto key = std::make_pair(dev, G);
+    const auto key = std::make_pair(dev, elements);
     auto it = g_scratch.find(key);
     if (it != g_scratch.end())
     {
@@ -561,7 +578,7 @@ hipError_t fft_scratch(int G, float2 *&out)
         return hipSuccess;
     }
     float2 *p = nullptr;
-    e = hipMalloc(&p, static_cast<size_t>(G) * G * sizeof(float2));
+    e = hipMalloc(&p, static_cast<size_t>(elements) * sizeof(float2));
     if (e != hipSuccess)
         return e;
     g_scratch[key] = p;
@@ -569,42 +586,59 @@ hipError_t fft_scratch(int G, float2 *&out)
     return hipSuccess;
 }
 
-template <int SG> hipError_t stockham_grid_fft(float2 *grid, int G, long long planes, hipStream_t st)
+// Planes of gy rows of gx points (both powers of two in 1024 .. 16384), src -> dst (in place when equal).  Each
+// launch takes a group of planes, so that small planes still fill the chip; the scratch holds one group, at most
+// kScratchBytes (or one plane when a plane is larger).  Kept small enough that the scratch written by column pass 2
+// is still in the 256 MB Infinity Cache when the row pass reads it (measured at G = 4096: one 128 MB plane per
+// group 0.57 ms per direction, two planes 0.64 ms).
+constexpr long long kScratchBytes = 64LL << 20;
+
+template <int SG>
+hipError_t stockham_fft2d(const float2 *src, float2 *dst, int gx, int gy, long long planes, bool centred,
+                          hipStream_t st)
 {
     FftArgs A;
-    hipError_t e = fft_scratch(G, A.scratch);
+    const long long plane = static_cast<long long>(gx) * gy;
+    const long long group = std::max(1LL, std::min(planes, kScratchBytes / (plane * static_cast<long long>(sizeof(float2)))));
+    hipError_t e = fft_scratch(group * plane, A.scratch);
     if (e != hipSuccess)
         return e;
-    A.G = G;
-    A.log_g = ilog2(G);
-    A.N1 = 1 << (ilog2(G) / 2);
-    A.N2 = G / A.N1;
-    A.log_tiles = ilog2(G / kColTile);
-    const int tiles = G / kColTile;
-    for (long long p = 0; p < planes && e == hipSuccess; ++p)
+    A.gx = gx;
+    A.gy = gy;
+    A.log_gy = ilog2(gy);
+    A.log_n1 = ilog2(gy) / 2;
+    A.log_n2 = ilog2(gy) - A.log_n1;
+    A.N1 = 1 << A.log_n1;
+    A.N2 = 1 << A.log_n2;
+    A.log_tiles = ilog2(gx / kColTile);
+    A.centred = centred ? 1 : 0;
+    const long long tiles = gx / kColTile;
+    for (long long p = 0; p < planes && e == hipSuccess; p += group)
     {
-        A.grid = grid + static_cast<size_t>(p) * G * G;
-        e = launch_col_pass<kPassCol1, SG>(A, A.N1, tiles * A.N2, st);
+        A.nplanes = static_cast<int>(std::min(group, planes - p));
+        A.src = src + static_cast<size_t>(p) * plane;
+        A.grid = dst + static_cast<size_t>(p) * plane;
+        e = launch_col_pass<kPassCol1, SG>(A, A.N1, static_cast<int>(A.nplanes * tiles * A.N2), st);
         if (e == hipSuccess)
-            e = launch_col_pass<kPassCol2, SG>(A, A.N2, tiles * A.N1, st);
+            e = launch_col_pass<kPassCol2, SG>(A, A.N2, static_cast<int>(A.nplanes * tiles * A.N1), st);
         if (e == hipSuccess)
             e = launch_row_pass<SG>(A, st);
     }
     return e;
 }
 
-// hipFFT plans per (device, G, batch), created on first use and kept for the life of the process.
+// hipFFT plans per (device, nx, ny, batch, type), created on first use and kept for the life of the process.
 std::mutex g_plan_mutex;
-std::map<std::tuple<int, int, long long>, hipfftHandle> g_plans;
+std::map<std::tuple<int, int, int, long long, int>, hipfftHandle> g_plans;
 
-hipError_t fft_plan(int G, long long batch, hipfftHandle &out)
+hipError_t fft_plan(int nx, int ny, long long batch, hipfftType type, hipfftHandle &out)
 {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess)
         return e;
     std::lock_guard<std::mutex> lock(g_plan_mutex);
-    const auto key = std::make_tuple(dev, G, batch);
+    const auto key = std::make_tuple(dev, nx, ny, batch, static_cast<int>(type));
     auto it = g_plans.find(key);
     if (it != g_plans.end())
     {
@@ -612,8 +646,8 @@ hipError_t fft_plan(int G, long long batch, hipfftHandle &out)
         return hipSuccess;
     }
     hipfftHandle h;
-    int n[2] = {G, G};
-    if (hipfftPlanMany(&h, 2, n, nullptr, 1, G * G, nullptr, 1, G * G, HIPFFT_C2C, static_cast<int>(batch)) !=
+    int n[2] = {ny, nx}; // rows, row length
+    if (hipfftPlanMany(&h, 2, n, nullptr, 1, nx * ny, nullptr, 1, nx * ny, type, static_cast<int>(batch)) !=
         HIPFFT_SUCCESS)
         return hipErrorUnknown;
     g_plans[key] = h;
@@ -642,11 +676,11 @@ hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward,
     const long long planes = nr_layers * kCorr;
     if (planes > 0x7fffffffLL || static_cast<long long>(G) * G > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
+    float2 *g = static_cast<float2 *>(grid);
     if (stockham_size(G))
-        return forward ? stockham_grid_fft<1>(static_cast<float2 *>(grid), G, planes, st)
-                       : stockham_grid_fft<-1>(static_cast<float2 *>(grid), G, planes, st);
+        return forward ? stockham_fft2d<1>(g, g, G, G, planes, true, st) : stockham_fft2d<-1>(g, g, G, G, planes, true, st);
     hipfftHandle plan;
-    hipError_t e = fft_plan(G, planes, plan);
+    hipError_t e = fft_plan(G, G, planes, HIPFFT_C2C, plan);
     if (e != hipSuccess)
         return e;
     e = checkerboard(grid, planes, G, st);
@@ -659,6 +693,40 @@ hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward,
     if (hipfftExecC2C(plan, d, d, forward ? HIPFFT_BACKWARD : HIPFFT_FORWARD) != HIPFFT_SUCCESS)
         return hipErrorUnknown;
     return checkerboard(grid, planes, G, st);
+}
+
+bool fft2d_supported(int nx, int ny)
+{
+    return nx >= 1 && ny >= 1 && static_cast<long long>(nx) * ny <= 0x7fffffffLL;
+}
+
+hipError_t launch_fft2d(const void *in, void *out, int nx, int ny, long long batch, int sign, bool fp64,
+                        hipStream_t st)
+{
+    if (batch < 1 || batch > 0x7fffffffLL || !fft2d_supported(nx, ny) || (sign != 1 && sign != -1))
+        return hipErrorInvalidValue;
+    if (!fp64 && stockham_size(nx) && stockham_size(ny))
+    {
+        const float2 *src = static_cast<const float2 *>(in);
+        float2 *dst = static_cast<float2 *>(out);
+        return sign > 0 ? stockham_fft2d<1>(src, dst, nx, ny, batch, false, st)
+                        : stockham_fft2d<-1>(src, dst, nx, ny, batch, false, st);
+    }
+    hipfftHandle plan;
+    hipError_t e = fft_plan(nx, ny, batch, fp64 ? HIPFFT_Z2Z : HIPFFT_C2C, plan);
+    if (e != hipSuccess)
+        return e;
+    if (hipfftSetStream(plan, st) != HIPFFT_SUCCESS)
+        return hipErrorUnknown;
+    const int dir = sign < 0 ? HIPFFT_FORWARD : HIPFFT_BACKWARD;
+    hipfftResult r;
+    if (fp64)
+        r = hipfftExecZ2Z(plan, static_cast<hipfftDoubleComplex *>(const_cast<void *>(in)),
+                          static_cast<hipfftDoubleComplex *>(out), dir);
+    else
+        r = hipfftExecC2C(plan, static_cast<hipfftComplex *>(const_cast<void *>(in)), static_cast<hipfftComplex *>(out),
+                          dir);
+    return r == HIPFFT_SUCCESS ? hipSuccess : hipErrorUnknown;
 }
 
 hipError_t launch_wstack(const void *src, void *dst, int G, int z_min, int nr_layers, float image_size,

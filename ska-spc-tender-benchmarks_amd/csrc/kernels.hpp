@@ -119,6 +119,11 @@ hipError_t launch_plan(const idgx_planner &pl, const float *uvw, const double *f
 
 bool grid_fft_supported(int G);
 hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward, hipStream_t st);
+bool fft2d_supported(int nx, int ny);
+// Plain 2-D DFT of `batch` planes of ny rows of nx points, out = sum in exp(sign 2 pi i (...)), unnormalised;
+// in == out is in place.
+hipError_t launch_fft2d(const void *in, void *out, int nx, int ny, long long batch, int sign, bool fp64,
+                        hipStream_t st);
 hipError_t launch_wstack(const void *src, void *dst, int G, int z_min, int nr_layers, float image_size,
                          float w_step, bool to_image, hipStream_t st);
 } // namespace idgx

@@ -146,6 +146,7 @@ def lib():
                                                     ctypes.POINTER(AdderPlan), vp, vp]
         L.idgx_split_subgrids_layered.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
         L.idgx_grid_fft.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp]
+        L.idgx_fft2d.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int, vp]
         L.idgx_plan_workspace_bytes.argtypes = [ctypes.c_int64]
         L.idgx_plan_workspace_bytes.restype = ctypes.c_size_t
         L.idgx_plan_segments.argtypes = [ctypes.POINTER(Planner)] + [vp] * 7 + [ctypes.c_int64, vp, vp, vp,
@@ -306,6 +307,16 @@ def grid_fft(plan, d_grid, nr_layers, direction, stream=None):
     """idgx_grid_fft: in-place centred 2-D transform of a master grid complex64 [nr_layers][4][G][G]."""
     _check(lib().idgx_grid_fft(ctypes.byref(plan), _dptr(d_grid), int(nr_layers), int(direction), _stream(stream)),
            "idgx_grid_fft")
+
+
+FFT_FP32 = 0
+FFT_FP64 = 1
+
+
+def fft2d(d_in, d_out, nx, ny, batch, sign, precision=FFT_FP32, stream=None):
+    """idgx_fft2d: plain batched 2-D DFT of complex64 (or complex128) planes [batch][ny][nx], unnormalised."""
+    _check(lib().idgx_fft2d(_dptr(d_in), _dptr(d_out), int(nx), int(ny), int(batch), int(sign), int(precision),
+                            _stream(stream)), "idgx_fft2d")
 
 
 def wstack_to_image(plan, d_layers, z_min, nr_layers, d_image, stream=None):
