@@ -120,6 +120,17 @@ def reference_as_shipped(nthreads):
                       f"{nvis / td / 1e6:.4f} Mvis/s"}
 
 
+# v_sin / v_cos issue at half rate on gfx950 (8 cycles per wave64, measured): 256 CUs x 4 SIMDs x 8 lanes per cycle.
+TRANS_PEAK_T_PER_S = 256 * 4 * 8 * 2.4e9 / 1e12
+
+
+def trans_rate(sincos_pairs, seconds):
+    """Transcendental rate of a pass: one sine and one cosine per (pixel, visibility) pair, against the half-rate
+    v_sin / v_cos peak."""
+    rate = 2.0 * sincos_pairs / seconds / 1e12
+    return {"achieved": rate, "peak": TRANS_PEAK_T_PER_S, "unit": "T/s", "frac": rate / TRANS_PEAK_T_PER_S}
+
+
 def kernel_key(precision, path, direction, shape=None):
     """Name of the kernel instantiation that runs a pass, as rocprofv3 reports it (the rows of the PMC summary)."""
     if precision == "fp32":
@@ -504,6 +515,7 @@ def run_batched_config(args):
             "traffic": None if traffic is None else traffic["bytes"], "traffic_detail": traffic,
             "flop_model": "P*(5*T + 34*V) + 134*P per subgrid, V = selected visibilities (FP32-equivalent "
                           "algorithmic flops), sincos separate",
+            "transcendental": trans_rate(sincos, t_dom),
         },
         "cpu_baseline": cpu,
     }
@@ -667,6 +679,7 @@ def main():
             "traffic_detail": traffic,
             "flop_model": "P*(5*T + 34*V) + 134*P per subgrid (FP32-equivalent algorithmic flops), sincos separate",
             "mfma_utilisation": (mfma_flops / t_dom / 1e12) / 2500.0 if mfma_flops else None,
+            "transcendental": trans_rate(sincos, t_dom),
             "algorithmic_bytes": bytes_dom,
             "hbm_gbs_algorithmic": bytes_dom / t_dom / 1e9,
         },
