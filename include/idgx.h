@@ -189,7 +189,8 @@ int idgx_split_subgrids_layered(const idgx_plan *plan, const idgx_metadata *d_me
  *   BACKWARD (uv -> image):  I[X] = sum_K G[K] exp(-2 pi i (K - G/2)(X - G/2) / G)
  * unnormalised (BACKWARD(FORWARD(I)) = G^2 I), the signs of idgx_subgrid_fft.  G must be a multiple of 4.  For
  * G = 1024 .. 16384 (powers of two) the transform is this library's three-pass kernel set; the first call for a G on
- * a device allocates a plane-sized scratch (G^2 x 8 B) kept for the life of the process.  Other sizes use rocFFT
+ * a device allocates a scratch of up to 64 MB, or one plane (G^2 x 8 B) when a plane is larger, kept for the life
+ * of the process.  Other sizes use rocFFT
  * between two sign-flip passes (plans cached per device, G and batch; the first call for a size allocates them).
  * Calls that overlap on one device share the scratch, so issue them on one stream. */
 int idgx_grid_fft(const idgx_plan *plan, void *d_grid, int nr_layers, int direction, void *stream);
@@ -200,7 +201,7 @@ int idgx_grid_fft(const idgx_plan *plan, void *d_grid, int nr_layers, int direct
  * unnormalised, sign = -1 (heFFTe forward) or +1 (backward); complex64 planes, or complex128 with
  * precision = IDGX_FFT_FP64.  d_in == d_out transforms in place.  FP32 with nx and ny powers of two in
  * 1024 .. 16384 runs this library's three-pass kernels (the first call for a plane size on a device allocates a
- * plane-sized scratch, kept until the process ends); other shapes and FP64 run rocFFT (the first call for a shape
+ * scratch of up to 64 MB, or one plane when a plane is larger, kept until the process ends); other shapes and FP64 run rocFFT (the first call for a shape
  * creates its plan).  Calls that overlap on one device share the scratch, so issue them on one stream. */
 enum
 {
