@@ -196,13 +196,14 @@ int idgx_split_subgrids_layered(const idgx_plan *plan, const idgx_metadata *d_me
 int idgx_grid_fft(const idgx_plan *plan, void *d_grid, int nr_layers, int direction, void *stream);
 
 /* Plain batched 2-D DFT, the transform of the reference's FFT benchmark (heFFTe fft2d, fft/inc/fft_benchmark.h:65-99,
- * run by fft/src/fft_benchmark.cpp:260-330): for b < batch,
+ * run by fft/src/fft_benchmark.cpp:240-345): for b < batch,
  *   d_out[b][ky][kx] = sum_{y < ny, x < nx} d_in[b][y][x] exp(sign 2 pi i (ky y / ny + kx x / nx)),
  * unnormalised, sign = -1 (heFFTe forward) or +1 (backward); complex64 planes, or complex128 with
  * precision = IDGX_FFT_FP64.  d_in == d_out transforms in place.  FP32 with nx and ny powers of two in
  * 1024 .. 16384 runs this library's three-pass kernels (the first call for a plane size on a device allocates a
- * scratch of up to 64 MB, or one plane when a plane is larger, kept until the process ends); other shapes and FP64 run rocFFT (the first call for a shape
- * creates its plan).  Calls that overlap on one device share the scratch, so issue them on one stream. */
+ * scratch of up to 64 MB, or one plane when a plane is larger, kept until the process ends); other shapes and
+ * FP64 run rocFFT (the first call for a shape creates its plan).  Calls that overlap on one device share the
+ * scratch, so issue them on one stream. */
 enum
 {
     IDGX_FFT_FP32 = 0,
