@@ -204,22 +204,35 @@ int build_adder_plan(const idgx_plan *plan, const idgx_metadata *md, int z_min, 
     ap->tile_size = T;
     ap->z_min = nr_layers > 1 ? z_min : 0;
     ap->nr_layers = nr_layers;
-    int tiles = 0;
-    for (int c = 0; c <= 4; ++c)
-        ap->color_begin[c] = 0;
-    long long prev = -1;
+    // Tiles (runs of equal keys): colour, first entry, count.
+    struct Tile
+    {
+        int color, first, count;
+    };
+    std::vector<Tile> runs;
     for (int i = 0; i < n; ++i)
     {
-        order[i] = keys[i].second;
-        if (keys[i].first != prev)
-        {
-            const int color = static_cast<int>(keys[i].first / (L * ntiles_axis * ntiles_axis));
-            tile_start[tiles] = i;
-            for (int c = color + 1; c <= 4; ++c)
-                ap->color_begin[c] = tiles + 1;
-            ++tiles;
-            prev = keys[i].first;
-        }
+        if (i == 0 || keys[i].first != keys[i - 1].first)
+            runs.push_back({static_cast<int>(keys[i].first / (L * ntiles_axis * ntiles_axis)), i, 0});
+        ++runs.back().count;
+    }
+    // Within a colour, heaviest tile first: a tile's workgroups walk its subgrids one after another, so the crowded
+    // tiles near the uv centre start at the head of the launch instead of finishing it.  The order of the subgrids
+    // inside a tile, and so every cell's summation order, is unchanged.
+    std::stable_sort(runs.begin(), runs.end(), [](const Tile &a, const Tile &b) {
+        return a.color != b.color ? a.color < b.color : a.count > b.count;
+    });
+    for (int c = 0; c <= 4; ++c)
+        ap->color_begin[c] = 0;
+    int tiles = 0, pos = 0;
+    for (const Tile &t : runs)
+    {
+        tile_start[tiles] = pos;
+        for (int k = 0; k < t.count; ++k)
+            order[pos++] = keys[t.first + k].second;
+        for (int c = t.color + 1; c <= 4; ++c)
+            ap->color_begin[c] = tiles + 1;
+        ++tiles;
     }
     tile_start[tiles] = n;
     ap->nr_tiles = tiles;
