@@ -363,8 +363,10 @@ def class_sample(bo, k, idx, d_vis, d_flags):
 
 
 def run_batched_config(args):
-    """Batched uneven workload: every subgrid class in one persistent launch per direction, flags as selection
-    lists, all built in HBM; throughput counts the selected (unflagged) visibilities."""
+    """Batched uneven workload: one persistent launch per subgrid class and direction (each class with its
+    autotuned tile shape), then the single-launch form (one launch per direction over the merged queue of every
+    class, one autotuned shape); flags as selection lists, all built in HBM; throughput counts the selected
+    (unflagged) visibilities."""
     import torch
 
     from idgx import observation
@@ -442,6 +444,22 @@ def run_batched_config(args):
                                 "gridder_worst_subgrid": float(per_subgrid_rel_l2(subs, g64).max()),
                                 "degridder": rel_l2(vis_out[sel], v64[sel])}
             rec["rel_l2_vs_fp64"] = acc
+        if len(bo["classes"]) > 1:
+            # The same plan as one persistent launch per direction over the merged queue of every class.
+            from idgx import Batch
+            runner.db.batch.set_single_launch(*runner.db.batch.class_shape(0))
+            sg_ms, sd_ms = runner.db.batch.autotune(runner.db.data, repeats=2, stream=stream)
+            sg, sd = runner.db.batch.single_launch()
+            s_tg, s_td, _ = time_form(runner, args.steps, args.warmup, stream)
+            shapes = Batch.shapes()
+            rec["single_launch"] = {"gridder_ms": s_tg * 1e3, "degridder_ms": s_td * 1e3,
+                                    "round_trip_mvis_s": nvis / (s_tg + s_td) / 1e6,
+                                    "gridder_shape": shapes[sg], "degridder_shape": shapes[sd],
+                                    "autotune_ms": {"gridder": sg_ms, "degridder": sd_ms}}
+            print(f"bench: {prec} batched, single launch over all classes: gridder {s_tg * 1e3:.2f} ms, "
+                  f"degridder {s_td * 1e3:.2f} ms, {rec['single_launch']['round_trip_mvis_s']:.1f} Mvis/s",
+                  file=sys.stderr, flush=True)
+            runner.db.batch.set_single_launch(-1, -1)
         forms.append(rec)
         runners.append(runner)
     head, runner = forms[-1], runners[-1]

@@ -261,8 +261,9 @@ int idgx_plan_segments(const idgx_planner *planner, const float *d_uvw, const do
  *
  * A batch holds up to 4 subgrid classes (one subgrid size each).  Its work queues of (class, subgrid, block)
  * items are built on the host at creation, one per class and direction, sorted heaviest first, and consumed by
- * persistent workgroups (one launch per class, each class with its own tile shape) through device atomic
- * counters.  Optional per-class visibility selections (flagging, ragged timesteps) restrict each subgrid to the
+ * persistent workgroups through device atomic counters: by default one launch per class, each class with its own
+ * tile shape; in single-launch mode one launch per direction over a merged queue of every class's items, heaviest
+ * first across classes, under one tile shape.  Optional per-class visibility selections (flagging, ragged timesteps) restrict each subgrid to the
  * listed entries: subgrid s uses vis_index[vis_offset[s] .. vis_offset[s+1]), each entry (t << 16) | c relative to
  * the subgrid's first row and channel window.  Creation allocates the queue memory; the execution entry points
  * allocate nothing, and idgx_batch_capture records them into a hipGraph (no memset nodes) for idgx_batch_launch.
@@ -319,12 +320,22 @@ int idgx_batch_get_shape(const idgx_batch *batch, int *gridder_shape, int *degri
 int idgx_batch_set_class_shape(idgx_batch *batch, int cls, int gridder_shape, int degridder_shape);
 int idgx_batch_get_class_shape(const idgx_batch *batch, int cls, int *gridder_shape, int *degridder_shape);
 
+/* Single-launch mode per direction: one persistent launch over the merged queue of all classes with the given
+ * shape; -1 returns that direction to per-class launches (the default).  The per-class shapes are kept.  For a
+ * given shape the results equal the per-class launches' bit for bit (every item writes its own outputs).  get
+ * reports -1 for a direction in per-class mode. */
+int idgx_batch_set_single_launch(idgx_batch *batch, int gridder_shape, int degridder_shape);
+int idgx_batch_get_single_launch(const idgx_batch *batch, int *gridder_shape, int *degridder_shape);
+
 /* Times every shape `repeats` times per class and direction on the given data (overwriting its outputs), keeps
- * each class's fastest and reports the sum over classes of their average times (ms). */
+ * each class's fastest and reports the sum over classes of their average times (ms).  A direction in
+ * single-launch mode then also times every shape of its merged launch, keeps the fastest and reports that
+ * launch's time. */
 int idgx_batch_autotune(idgx_batch *batch, const idgx_batch_data *data, int repeats, void *stream,
                         float *best_gridder_ms, float *best_degridder_ms);
 
-/* Persistent gridder / degridder launches (one per class) over every subgrid of every class. */
+/* Persistent gridder / degridder launches (one per class, or one in single-launch mode) over every subgrid of every
+ * class. */
 int idgx_batch_grid(idgx_batch *batch, const idgx_batch_data *data, void *stream);
 int idgx_batch_degrid(idgx_batch *batch, const idgx_batch_data *data, void *stream);
 

@@ -1,9 +1,11 @@
 // Batched execution of a whole plan: the subgrids of up to four subgrid classes (one subgrid size each) run in
-// persistent gridder (or degridder) launches fed from device work queues, one launch per class so that each class
-// has its own tile shape, optionally captured in a hipGraph, with the shapes chosen by an autotuner that times
-// the compiled shapes on the real data, class by class.  (The tile that suits a subgrid size is the one that
-// covers its pixels with the fewest idle lanes: S = 24 and 48 prefer fewer tiles per wave than S = 32.  Separate
-// launches per class measured no slower than one launch over the mixed queue.)
+// persistent gridder (or degridder) launches fed from device work queues, optionally captured in a hipGraph, with
+// the tile shapes chosen by an autotuner that times the compiled shapes on the real data.  Two launch modes:
+// - per class (default): one launch per class, each with its own tile shape.  The tile that suits a subgrid size
+//   is the one that covers its pixels with the fewest idle lanes: S = 24 and 48 prefer fewer tiles per wave than
+//   S = 32.
+// - single launch: one launch per direction over one queue that holds the items of every class, heaviest first
+//   across classes, with one tile shape for all of them (the kernel reads each item's class from the item).
 //
 // Work queue.  An item is (class, subgrid, block): a block of pixels for the gridder, a chunk of visibilities
 // for the degridder.  Items are built on the host from the plan (metadata, channel windows and visibility
@@ -41,6 +43,12 @@ struct idgx_batch
     int nr_items[2][idgx::kMaxClasses] = {};
     size_t offset[2][idgx::kMaxClasses] = {}, capacity[2][idgx::kMaxClasses] = {};
     uint2 *d_items[2] = {};
+    // Single-launch mode per direction: shape (-1 = per-class launches), item count and resident workgroups of
+    // the merged queue, which lives after the class slices.
+    int single_shape[2] = {-1, -1};
+    int single_items[2] = {};
+    int single_workgroups[2] = {};
+    size_t single_offset[2] = {};
     int *d_counters = nullptr; // [direction][class]
     // Degridder pixels prepared once per subgrid and pass (per class): split words, FP16 scales, slot n table.
     uint4 *d_prepared[idgx::kMaxClasses] = {};
@@ -155,6 +163,38 @@ int upload_queue(idgx_batch *b, bool gridder, int k, int shape)
     return b->workgroups[dir][k] > 0 ? IDGX_OK : IDGX_ERR_DEVICE;
 }
 
+// Merged queue of every class under one shape, heaviest first across classes (stable: ties keep class order).
+int upload_single_queue(idgx_batch *b, bool gridder, int shape)
+{
+    const int dir = gridder ? 0 : 1;
+    std::vector<Item> items;
+    for (int k = 0; k < b->nr_classes; ++k)
+    {
+        const std::vector<Item> ki = build_items(b, gridder, k, shape);
+        items.insert(items.end(), ki.begin(), ki.end());
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
+    size_t room = 0;
+    for (int k = 0; k < b->nr_classes; ++k)
+        room += b->capacity[dir][k];
+    if (items.size() > room || items.size() > 0x7fffffffULL)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    std::vector<uint2> flat(items.size());
+    for (size_t i = 0; i < items.size(); ++i)
+        flat[i] = items[i].v;
+    if (!flat.empty())
+    {
+        const hipError_t e = hipMemcpy(b->d_items[dir] + b->single_offset[dir], flat.data(),
+                                       flat.size() * sizeof(uint2), hipMemcpyHostToDevice);
+        if (e != hipSuccess)
+            return to_status(e);
+    }
+    b->single_items[dir] = static_cast<int>(flat.size());
+    b->single_shape[dir] = shape;
+    b->single_workgroups[dir] = idgx::queue_workgroups(b->precision, gridder, shape);
+    return b->single_workgroups[dir] > 0 ? IDGX_OK : IDGX_ERR_DEVICE;
+}
+
 int make_queue_args(const idgx_batch *b, const idgx_batch_data *d, bool gridder, QueueArgs &q)
 {
     if (!d || !d->d_uvw || !d->d_wavenumbers || (gridder ? !d->d_visibilities : !d->d_visibilities_out))
@@ -201,7 +241,8 @@ int make_queue_args(const idgx_batch *b, const idgx_batch_data *d, bool gridder,
     return IDGX_OK;
 }
 
-// One persistent launch per class (each with its own tile shape), or only class `only` when it is >= 0.
+// One persistent launch per class (each with its own tile shape), or only class `only` when it is >= 0; in
+// single-launch mode (and `only` < 0) one launch over the merged queue.
 int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st, int only = -1)
 {
     const int dir = gridder ? 0 : 1;
@@ -210,6 +251,21 @@ int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st, i
     if (rc != IDGX_OK)
         return rc;
     hipError_t e = idgx::launch_reset_counters(b->d_counters + dir * idgx::kMaxClasses, idgx::kMaxClasses, st);
+    if (only < 0 && b->single_shape[dir] >= 0)
+    {
+        for (int k = 0; k < b->nr_classes && e == hipSuccess && !gridder; ++k)
+            if (b->nr_items[dir][k] > 0)
+                e = idgx::launch_degrid_prepare(q.cls[k], b->precision, st);
+        if (e != hipSuccess || b->single_items[dir] == 0)
+            return to_status(e);
+        q.items = b->d_items[dir] + b->single_offset[dir];
+        q.nr_items = b->single_items[dir];
+        q.counter = b->d_counters + dir * idgx::kMaxClasses;
+        const int sh = b->single_shape[dir], wg = b->single_workgroups[dir];
+        e = gridder ? idgx::launch_grid_mixed_queue(q, b->precision, sh, wg, st)
+                    : idgx::launch_degrid_mixed_queue(q, b->precision, sh, wg, st);
+        return to_status(e);
+    }
     for (int k = 0; k < b->nr_classes && e == hipSuccess; ++k)
     {
         if ((only >= 0 && k != only) || b->nr_items[dir][k] == 0)
@@ -288,7 +344,9 @@ extern "C" int idgx_batch_create(const idgx_batch_class *classes, int nr_classes
             b->capacity[dir][k] = max_items(b, dir == 0, k);
             total += b->capacity[dir][k];
         }
-        e = hipMalloc(&b->d_items[dir], std::max<size_t>(1, total) * sizeof(uint2));
+        // The merged queue of single-launch mode holds at most the sum of the class slices.
+        b->single_offset[dir] = total;
+        e = hipMalloc(&b->d_items[dir], std::max<size_t>(1, 2 * total) * sizeof(uint2));
     }
     if (e == hipSuccess)
         e = hipMalloc(&b->d_counters, 2 * idgx::kMaxClasses * sizeof(int));
@@ -378,6 +436,37 @@ extern "C" int idgx_batch_get_shape(const idgx_batch *b, int *gridder_shape, int
     return idgx_batch_get_class_shape(b, 0, gridder_shape, degridder_shape);
 }
 
+extern "C" int idgx_batch_set_single_launch(idgx_batch *b, int gridder_shape, int degridder_shape)
+{
+    if (!b || gridder_shape < -1 || gridder_shape >= idgx::mixed_shape_count() || degridder_shape < -1 ||
+        degridder_shape >= idgx::mixed_shape_count())
+        return IDGX_ERR_INVALID_ARGUMENT;
+    drop_graph(b);
+    const int want[2] = {gridder_shape, degridder_shape};
+    int rc = IDGX_OK;
+    for (int dir = 0; dir < 2 && rc == IDGX_OK; ++dir)
+    {
+        if (want[dir] < 0)
+            b->single_shape[dir] = -1;
+        else
+            rc = upload_single_queue(b, dir == 0, want[dir]);
+    }
+    if (rc != IDGX_OK)
+        b->single_shape[0] = b->single_shape[1] = -1;
+    return rc;
+}
+
+extern "C" int idgx_batch_get_single_launch(const idgx_batch *b, int *gridder_shape, int *degridder_shape)
+{
+    if (!b)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    if (gridder_shape)
+        *gridder_shape = b->single_shape[0];
+    if (degridder_shape)
+        *degridder_shape = b->single_shape[1];
+    return IDGX_OK;
+}
+
 extern "C" int idgx_batch_grid(idgx_batch *b, const idgx_batch_data *data, void *stream)
 {
     if (!b)
@@ -444,6 +533,34 @@ extern "C" int idgx_batch_autotune(idgx_batch *b, const idgx_batch_data *data, i
                 rc = upload_queue(b, gridder, k, best_shape);
             total[dir] += std::max(best, 0.F);
         }
+        if (rc != IDGX_OK || b->single_shape[dir] < 0)
+            continue;
+        // Single-launch mode: the one shape of the merged queue, timed on the whole direction.
+        float best = -1.F;
+        int best_shape = b->single_shape[dir];
+        for (int sh = 0; sh < idgx::mixed_shape_count() && rc == IDGX_OK; ++sh)
+        {
+            rc = upload_single_queue(b, gridder, sh);
+            if (rc == IDGX_OK)
+                rc = run(b, data, gridder, st); // warm-up
+            if (rc == IDGX_OK && hipEventRecord(e0, st) != hipSuccess)
+                rc = IDGX_ERR_DEVICE;
+            for (int r = 0; r < repeats && rc == IDGX_OK; ++r)
+                rc = run(b, data, gridder, st);
+            float ms = 0.F;
+            if (rc == IDGX_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                  hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = IDGX_ERR_DEVICE;
+            ms /= repeats;
+            if (rc == IDGX_OK && (best < 0.F || ms < best))
+            {
+                best = ms;
+                best_shape = sh;
+            }
+        }
+        if (rc == IDGX_OK)
+            rc = upload_single_queue(b, gridder, best_shape);
+        total[dir] = std::max(best, 0.F);
     }
     if (best_grid_ms)
         *best_grid_ms = total[0];
@@ -505,11 +622,19 @@ extern "C" int idgx_batch_items(const idgx_batch *b, int *gridder_items, int *de
         return IDGX_ERR_INVALID_ARGUMENT;
     int items[2] = {0, 0}, wg[2] = {0, 0};
     for (int dir = 0; dir < 2; ++dir)
+    {
+        if (b->single_shape[dir] >= 0)
+        {
+            items[dir] = b->single_items[dir];
+            wg[dir] = b->single_workgroups[dir];
+            continue;
+        }
         for (int k = 0; k < b->nr_classes; ++k)
         {
             items[dir] += b->nr_items[dir][k];
             wg[dir] = std::max(wg[dir], b->workgroups[dir][k]);
         }
+    }
     if (gridder_items)
         *gridder_items = items[0];
     if (degridder_items)

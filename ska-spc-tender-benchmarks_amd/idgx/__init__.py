@@ -163,6 +163,8 @@ def lib():
         L.idgx_batch_set_class_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.idgx_batch_get_class_shape.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                                  ctypes.POINTER(ctypes.c_int)]
+        L.idgx_batch_set_single_launch.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.idgx_batch_get_single_launch.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.idgx_batch_shape_info.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
         L.idgx_batch_autotune.argtypes = [vp, ctypes.POINTER(BatchData), ctypes.c_int, vp,
                                           ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
@@ -435,6 +437,17 @@ class Batch:
         g, d = ctypes.c_int(), ctypes.c_int()
         _check(lib().idgx_batch_get_class_shape(self.handle, cls, ctypes.byref(g), ctypes.byref(d)),
                "idgx_batch_get_class_shape")
+        return g.value, d.value
+
+    def set_single_launch(self, gridder_shape, degridder_shape):
+        """One launch per direction over the merged queue of all classes (shape -1: per-class launches)."""
+        _check(lib().idgx_batch_set_single_launch(self.handle, gridder_shape, degridder_shape),
+               "idgx_batch_set_single_launch")
+
+    def single_launch(self):
+        g, d = ctypes.c_int(), ctypes.c_int()
+        _check(lib().idgx_batch_get_single_launch(self.handle, ctypes.byref(g), ctypes.byref(d)),
+               "idgx_batch_get_single_launch")
         return g.value, d.value
 
     @staticmethod

@@ -1,6 +1,6 @@
 """Batched execution (idgx_batch_*): persistent work-queue kernels over whole plans, visibility selections
-(flagging), mixed subgrid sizes in one batch (a launch and a tile shape per class), hipGraph capture and the
-autotuner."""
+(flagging), mixed subgrid sizes in one batch (a launch and a tile shape per class, or one launch over the merged
+queue of all classes), hipGraph capture and the autotuner."""
 import numpy as np
 import pytest
 
@@ -260,3 +260,123 @@ def test_classes_run_with_their_own_tile_shapes():
     assert rel_l2(db.visibilities_out(), vis0) < 2e-6
     with pytest.raises(Exception):
         db.batch.set_class_shape(3, 0, 0)
+
+
+def _zero_outputs(db):
+    for c in db.classes:
+        c["d_subgrids"].zero_()
+    db.d_vis_out.zero_()
+
+
+@pytest.mark.parametrize("gshape,dshape", [((4, 256, 0), (4, 256, 0)), ((3, 256, 0), (8, 256, 1))])
+def test_single_launch_over_all_classes_equals_per_class_launches_bitwise(gshape, dshape):
+    """Single-launch mode: one persistent launch per direction over the merged queue of the S = 24/32/48 classes
+    gives the per-class launches' results bit for bit when every class uses the same shape (each item writes its
+    own outputs, so the order of the merged queue does not matter)."""
+    w = observation.make_observation("small")
+    db = DeviceBatch(_mixed_size_classes(w), precision="fp16x3")
+    shapes = Batch.shapes()
+    g, d = shapes.index(gshape), shapes.index(dshape)
+    db.batch.set_shape(g, d)
+    assert db.batch.single_launch() == (-1, -1)
+    items_per_class = db.batch.items()
+    db.grid()
+    db.degrid()
+    sub0 = [db.subgrids(k) for k in range(3)]
+    vis0 = db.visibilities_out()
+
+    db.batch.set_single_launch(g, d)
+    assert db.batch.single_launch() == (g, d)
+    gi, di, gw, dw = db.batch.items()
+    assert (gi, di) == items_per_class[:2] and gw > 0 and dw > 0
+    _zero_outputs(db)
+    db.grid()
+    db.degrid()
+    for k in range(3):
+        assert db.subgrids(k).tobytes() == sub0[k].tobytes(), f"class {k}"
+    assert db.visibilities_out().tobytes() == vis0.tobytes()
+
+    db.batch.set_single_launch(-1, -1)  # back to per-class launches; their queues were kept
+    assert db.batch.single_launch() == (-1, -1)
+    _zero_outputs(db)
+    db.grid()
+    db.degrid()
+    for k in range(3):
+        assert db.subgrids(k).tobytes() == sub0[k].tobytes(), f"class {k}"
+    assert db.visibilities_out().tobytes() == vis0.tobytes()
+
+
+def test_single_launch_matches_fp64_with_flagged_classes():
+    """The merged queue with visibility selections (flagging) on every class: gridder and degridder within the
+    mixed-form tolerance of the FP64 oracle, and flagged visibilities left unwritten."""
+    w = observation.make_observation("small")
+    classes = _mixed_size_classes(w)
+    rng = np.random.default_rng(5)
+    flags = rng.random((classes[0]["nr_rows"], w["wavenumbers"].shape[0])) < 0.3
+    for c in classes:
+        c["vis_index"], c["vis_offset"] = selection_lists(c, flags)
+    db = DeviceBatch(classes, precision="fp16x3")
+    db.batch.set_single_launch(Batch.shapes().index((4, 256, 0)), Batch.shapes().index((4, 256, 0)))
+    db.grid()
+    db.degrid()
+    vis = db.visibilities_out()
+    assert not np.any(vis[flags]), "flagged visibilities must not be written"
+    for k, c in enumerate(classes):
+        c0 = dict(c, visibilities=np.where(flags[..., None], 0, c["visibilities"]).astype(np.complex64))
+        got = db.subgrids(k)
+        e = per_subgrid_rel_l2(got, oracle_grid(c0, fp64=True)).max()
+        v64 = oracle_degrid(c0, got, fp64=True)
+        r0 = int(c["metadata"]["time_offset"][0])
+        r1 = r0 + int(c["metadata"]["nr_timesteps"].sum())
+        keep = ~flags[r0:r1]
+        ev = rel_l2(vis[r0:r1][keep], v64[r0:r1][keep])
+        print(f"class S={c['subgrid_size']}: gridder worst subgrid vs FP64 {e:.2e}, degridder {ev:.2e}")
+        assert e <= 5e-5 and ev <= 5e-5
+
+
+def test_graph_of_a_single_launch_batch_replays_match_eager():
+    import torch
+
+    w = observation.make_observation("small")
+    db = DeviceBatch(_mixed_size_classes(w), precision="fp16x3")
+    shapes = Batch.shapes()
+    db.batch.set_single_launch(shapes.index((4, 256, 0)), shapes.index((6, 256, 0)))
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        db.grid(stream)
+        db.degrid(stream)
+    stream.synchronize()
+    sub0 = [db.subgrids(k) for k in range(3)]
+    vis0 = db.visibilities_out()
+    db.batch.capture(db.data, STAGE_GRID | STAGE_DEGRID, stream)
+    for r in range(3):
+        _zero_outputs(db)
+        torch.cuda.synchronize()
+        db.batch.launch(stream)
+        stream.synchronize()
+        for k in range(3):
+            assert db.subgrids(k).tobytes() == sub0[k].tobytes(), f"replay {r}, class {k}"
+        assert db.visibilities_out().tobytes() == vis0.tobytes(), f"replay {r}"
+        x = torch.ones(1000, device="cuda").sum()  # eager work between replays
+        assert float(x) == 1000.0
+
+
+def test_autotuner_tunes_the_single_launch_shape():
+    w = observation.make_observation("small")
+    db = DeviceBatch(_mixed_size_classes(w), precision="fp16x3")
+    db.grid()
+    db.degrid()
+    sub0 = [db.subgrids(k) for k in range(3)]
+    vis0 = db.visibilities_out()
+    db.batch.set_single_launch(0, 0)
+    tg, td = db.batch.autotune(db.data, repeats=1)
+    g, d = db.batch.single_launch()
+    shapes = Batch.shapes()
+    print(f"single launch autotuned: gridder {shapes[g]} {tg:.3f} ms, degridder {shapes[d]} {td:.3f} ms")
+    assert 0 <= g < len(shapes) and 0 <= d < len(shapes) and tg > 0 and td > 0
+    _zero_outputs(db)
+    db.grid()
+    db.degrid()
+    for k in range(3):
+        assert rel_l2(db.subgrids(k), sub0[k]) < 2e-6, f"class {k}"
+    assert rel_l2(db.visibilities_out(), vis0) < 2e-6
