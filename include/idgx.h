@@ -123,6 +123,43 @@ int idgx_degrid(const idgx_plan *plan, const idgx_metadata *d_metadata, const id
                 const float *d_uvw, const float *d_wavenumbers, const void *d_subgrids, const float *d_spheroidal,
                 const void *d_aterms, void *d_visibilities, void *stream);
 
+/* ---- Host-buffer calls: the reference launcher's contract in one call --------------------------------------
+ *
+ * For callers that hold only host arrays (a cgo / ctypes / JNI binding; the C++ drop-in launcher is built on
+ * them).  Each call checks the plan against h_metadata, allocates the device buffers, runs `iterations` uploads,
+ * `iterations` kernel runs and `iterations` downloads of the result on a private stream, frees everything and
+ * fills `timing` (may be NULL).  Plain FP32 runs the one-shot kernels (idgx_grid / idgx_degrid); the mixed forms run
+ * as a one-class batch (idgx_batch_*) created outside the timed runs.  Host layouts are the reference's:
+ * uvw [nr_rows][3] float (metres), wavenumbers [nr_channels], visibilities [nr_rows][nr_channels][4] complex64,
+ * spheroidal [S][S], aterms [slots][stations][S][S][4] complex64, subgrids [nr_subgrids][4][S][S] complex64;
+ * h_channels (per-subgrid channel windows) may be NULL: every subgrid then covers all channels, as in the
+ * reference's plans.
+ * Replaces gridding_benchmark_launcher<gpu>::launch (gridding/src/gridding_benchmark.cu:248-361) and
+ * degridding_benchmark_launcher<gpu>::launch (gridding/src/degridding_benchmark.cu:205-318), whose interface is
+ * gridding/inc/gridding_benchmark.h:10-21 and gridding/inc/degridding_benchmark.h:18-28. */
+
+/* Per-iteration timings, as the reference's benchmark_result (gridding/inc/common_gridding.h:5-12). */
+typedef struct
+{
+    double in_transfer_time;  /* microseconds per upload of the inputs */
+    double out_transfer_time; /* microseconds per download of the result */
+    double in_bandwidth;      /* MB/s (bytes per microsecond) */
+    double out_bandwidth;     /* MB/s */
+    double compute_time;      /* microseconds per kernel run */
+} idgx_timing;
+
+/* Gridder from host buffers: h_subgrids receives the subgrids. */
+int idgx_host_grid(const idgx_plan *plan, const idgx_metadata *h_metadata, const idgx_channels *h_channels,
+                   const float *h_uvw, const float *h_wavenumbers, const void *h_visibilities,
+                   const float *h_spheroidal, const void *h_aterms, void *h_subgrids, int iterations,
+                   idgx_timing *timing);
+
+/* Degridder from host buffers: h_visibilities receives every row and channel (entries no subgrid covers are 0). */
+int idgx_host_degrid(const idgx_plan *plan, const idgx_metadata *h_metadata, const idgx_channels *h_channels,
+                     const float *h_uvw, const float *h_wavenumbers, const void *h_subgrids,
+                     const float *h_spheroidal, const void *h_aterms, void *h_visibilities, int iterations,
+                     idgx_timing *timing);
+
 /* ---- Stages after the gridder / before the degridder (no reference counterpart: the reference benchmark stops
  * at image-domain subgrids, gridding/src/gridding_benchmark.cpp:155-163) ------------------------------------- */
 

@@ -78,6 +78,13 @@ FFT_FORWARD = 0
 FFT_BACKWARD = 1
 
 
+class Timing(ctypes.Structure):
+    """idgx_timing: per-iteration microseconds and MB/s, as the reference's benchmark_result."""
+    _fields_ = [("in_transfer_time", ctypes.c_double), ("out_transfer_time", ctypes.c_double),
+                ("in_bandwidth", ctypes.c_double), ("out_bandwidth", ctypes.c_double),
+                ("compute_time", ctypes.c_double)]
+
+
 class Planner(ctypes.Structure):
     """idgx_planner (include/idgx.h)."""
 
@@ -138,6 +145,8 @@ def lib():
         vp = ctypes.c_void_p
         L.idgx_check_plan.argtypes = [ctypes.POINTER(Plan), vp, vp]
         L.idgx_grid.argtypes = [ctypes.POINTER(Plan)] + [vp] * 9
+        L.idgx_host_grid.argtypes = [ctypes.POINTER(Plan)] + [vp] * 8 + [ctypes.c_int, ctypes.POINTER(Timing)]
+        L.idgx_host_degrid.argtypes = [ctypes.POINTER(Plan)] + [vp] * 8 + [ctypes.c_int, ctypes.POINTER(Timing)]
         L.idgx_degrid.argtypes = [ctypes.POINTER(Plan)] + [vp] * 9
         L.idgx_subgrid_fft.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, vp]
         L.idgx_adder_plan_build.argtypes = [ctypes.POINTER(Plan), vp, ctypes.POINTER(AdderPlan), vp, vp]
@@ -249,6 +258,45 @@ def degrid(plan, d_metadata, d_channels, d_uvw, d_wavenumbers, d_subgrids, d_sph
                            _dptr(d_visibilities), _stream(stream))
     if rc != OK:
         raise IdgxError(rc, "idgx_degrid")
+
+
+def _host_inputs(w):
+    arrays = {k: np.ascontiguousarray(w[k]) for k in ("metadata", "uvw", "wavenumbers", "spheroidal", "aterms")}
+    arrays["channels"] = None if w.get("channels") is None else np.ascontiguousarray(w["channels"])
+    assert arrays["metadata"].dtype == METADATA_DTYPE
+    assert arrays["uvw"].dtype == np.float32 and arrays["wavenumbers"].dtype == np.float32
+    assert arrays["spheroidal"].dtype == np.float32 and arrays["aterms"].dtype == np.complex64
+    return arrays
+
+
+def host_grid(plan, w, iterations=1):
+    """idgx_host_grid: the gridder from host arrays of workload w (metadata, uvw, wavenumbers, visibilities,
+    spheroidal, aterms); returns (subgrids [n][4][S][S] complex64, Timing)."""
+    h = _host_inputs(w)
+    vis = np.ascontiguousarray(w["visibilities"], dtype=np.complex64)
+    S = plan.subgrid_size
+    out = np.zeros((max(plan.nr_subgrids, 1), 4, S, S), np.complex64)
+    t = Timing()
+    rc = lib().idgx_host_grid(ctypes.byref(plan), _hptr(h["metadata"]), _hptr(h["channels"]), _hptr(h["uvw"]),
+                              _hptr(h["wavenumbers"]), _hptr(vis), _hptr(h["spheroidal"]), _hptr(h["aterms"]), _hptr(out), int(iterations),
+                              ctypes.byref(t))
+    if rc != OK:
+        raise IdgxError(rc, "idgx_host_grid")
+    return out[: plan.nr_subgrids], t
+
+
+def host_degrid(plan, w, subgrids, iterations=1):
+    """idgx_host_degrid: the degridder from host arrays; returns (visibilities [rows][C][4] complex64, Timing)."""
+    h = _host_inputs(w)
+    sub = np.ascontiguousarray(subgrids, dtype=np.complex64)
+    out = np.zeros((plan.nr_rows, plan.nr_channels, 4), np.complex64)
+    t = Timing()
+    rc = lib().idgx_host_degrid(ctypes.byref(plan), _hptr(h["metadata"]), _hptr(h["channels"]), _hptr(h["uvw"]),
+                                _hptr(h["wavenumbers"]), _hptr(sub), _hptr(h["spheroidal"]), _hptr(h["aterms"]),
+                                _hptr(out), int(iterations), ctypes.byref(t))
+    if rc != OK:
+        raise IdgxError(rc, "idgx_host_degrid")
+    return out, t
 
 
 def subgrid_fft(plan, d_subgrids, direction, stream=None):

@@ -88,6 +88,28 @@ def test_grid_entry_points_reject_null_arguments_without_touching_the_device():
     assert idgx.strerror(idgx.ERR_INVALID_METADATA).startswith("subgrid metadata")
 
 
+def test_host_buffer_calls_validate_on_the_host_before_touching_the_device():
+    """idgx_host_grid / idgx_host_degrid reject null buffers, a zero iteration count and out-of-bounds metadata
+    with a status code, before any device call (this runs without a GPU)."""
+    w = observation.make_observation("small", visibilities=False)
+    p = _plan(w)
+    L = idgx.lib()
+    t = idgx.Timing()
+    assert L.idgx_host_grid(None, *([None] * 8), 1, ctypes.byref(t)) == idgx.ERR_INVALID_ARGUMENT
+    assert L.idgx_host_degrid(ctypes.byref(p), *([None] * 8), 1, None) == idgx.ERR_INVALID_ARGUMENT
+    md, ch = w["metadata"].copy(), w["channels"]
+    uvw, wn = np.ascontiguousarray(w["uvw"]), np.ascontiguousarray(w["wavenumbers"])
+    sph, at = np.ascontiguousarray(w["spheroidal"]), np.ascontiguousarray(w["aterms"])
+    vis = np.zeros((w["nr_rows"], wn.size, 4), np.complex64)
+    sub = np.zeros((w["nr_subgrids"], 4, w["subgrid_size"], w["subgrid_size"]), np.complex64)
+    hp = idgx._hptr
+    args = [hp(md), hp(ch), hp(uvw), hp(wn), hp(vis), hp(sph), hp(at), hp(sub)]
+    assert L.idgx_host_grid(ctypes.byref(p), *args, 0, None) == idgx.ERR_INVALID_ARGUMENT
+    md["station1"][5] = w["nr_stations"]
+    assert L.idgx_host_grid(ctypes.byref(p), *args, 1, None) == idgx.ERR_INVALID_METADATA
+    assert L.idgx_host_degrid(ctypes.byref(p), *args, 1, None) == idgx.ERR_INVALID_METADATA
+
+
 def test_planner_covers_observation_and_keeps_subgrids_on_grid():
     w = observation.make_observation("ska_low", visibilities=False)
     md, ch = w["metadata"], w["channels"]
