@@ -72,8 +72,7 @@ int run_host(bool gridder, const idgx_plan *plan, const idgx_metadata *h_metadat
              const float *h_uvw, const float *h_wavenumbers, const void *h_in, const float *h_spheroidal,
              const void *h_aterms, void *h_out, int iterations, idgx_timing *timing)
 {
-    if (!plan || !h_metadata || !h_uvw || !h_wavenumbers || !h_in || !h_spheroidal || !h_aterms || !h_out ||
-        iterations < 1)
+    if (!plan || iterations < 1)
         return IDGX_ERR_INVALID_ARGUMENT;
     int rc = idgx_check_plan(plan, h_metadata, h_channels);
     if (rc != IDGX_OK)
@@ -96,6 +95,13 @@ int run_host(bool gridder, const idgx_plan *plan, const idgx_metadata *h_metadat
                                                 h_spheroidal, h_aterms,      h_metadata,
                                                 gridder ? nullptr : h_in,   h_channels};
     const int out_buffer = gridder ? kSubgrids : kVisibilities;
+    // Every buffer the call moves must be given (an empty plan moves no metadata and no subgrids).
+    const bool ins_given = h_uvw && h_wavenumbers && h_spheroidal && h_aterms && (n == 0 || (h_metadata && h_in));
+    if (!ins_given || (bytes[out_buffer] > 0 && !h_out))
+        return IDGX_ERR_INVALID_ARGUMENT;
+    for (int i = 0; i < CallState::kBuffers; ++i)
+        if (bytes[i] == 0)
+            host_in[i] = nullptr;
 
     CallState cs;
     hipError_t e = hipStreamCreateWithFlags(&cs.stream, hipStreamNonBlocking);
