@@ -452,11 +452,14 @@ def run_batched_config(args):
             sg, sd = runner.db.batch.single_launch()
             s_tg, s_td, _ = time_form(runner, args.steps, args.warmup, stream)
             shapes = Batch.shapes()
+            g_name = "tiles per class " + "/".join(str(shapes[runner.db.batch.class_shape(k)[0]][0])
+                                                   for k in range(len(bo["classes"]))) if sg < 0 else shapes[sg]
             rec["single_launch"] = {"gridder_ms": s_tg * 1e3, "degridder_ms": s_td * 1e3,
                                     "round_trip_mvis_s": nvis / (s_tg + s_td) / 1e6,
-                                    "gridder_shape": shapes[sg], "degridder_shape": shapes[sd],
+                                    "gridder_shape": g_name, "degridder_shape": shapes[sd],
                                     "autotune_ms": {"gridder": sg_ms, "degridder": sd_ms}}
-            print(f"bench: {prec} batched, single launch over all classes: gridder {s_tg * 1e3:.2f} ms, "
+            print(f"bench: {prec} batched, single launch over all classes (gridder {g_name}, degridder "
+                  f"{shapes[sd]}): gridder {s_tg * 1e3:.2f} ms, "
                   f"degridder {s_td * 1e3:.2f} ms, {rec['single_launch']['round_trip_mvis_s']:.1f} Mvis/s",
                   file=sys.stderr, flush=True)
             runner.db.batch.set_single_launch(-1, -1)

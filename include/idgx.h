@@ -357,8 +357,16 @@ int idgx_batch_get_shape(const idgx_batch *batch, int *gridder_shape, int *degri
 int idgx_batch_set_class_shape(idgx_batch *batch, int cls, int gridder_shape, int degridder_shape);
 int idgx_batch_get_class_shape(const idgx_batch *batch, int cls, int *gridder_shape, int *degridder_shape);
 
+/* Gridder shape of a single launch in which every item takes the tiles per wave of its class's gridder shape
+ * (LDS batch 256, phases on the VALU). */
+enum
+{
+    IDGX_SHAPE_TILES_PER_CLASS = -2
+};
+
 /* Single-launch mode per direction: one persistent launch over the merged queue of all classes with the given
- * shape; -1 returns that direction to per-class launches (the default).  The per-class shapes are kept.  For a
+ * shape (for the gridder also IDGX_SHAPE_TILES_PER_CLASS); -1 returns that direction to per-class launches (the
+ * default).  The per-class shapes are kept.  For a
  * given shape the results equal the per-class launches' bit for bit (every item writes its own outputs).  get
  * reports -1 for a direction in per-class mode. */
 int idgx_batch_set_single_launch(idgx_batch *batch, int gridder_shape, int degridder_shape);

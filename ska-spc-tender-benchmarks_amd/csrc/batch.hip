@@ -5,7 +5,8 @@
 //   is the one that covers its pixels with the fewest idle lanes: S = 24 and 48 prefer fewer tiles per wave than
 //   S = 32.
 // - single launch: one launch per direction over one queue that holds the items of every class, heaviest first
-//   across classes, with one tile shape for all of them (the kernel reads each item's class from the item).
+//   across classes (the kernel reads each item's class from the item), with one tile shape for all of them or,
+//   for the gridder, each class's own tiles per wave inside the one launch (IDGX_SHAPE_TILES_PER_CLASS).
 //
 // Work queue.  An item is (class, subgrid, block): a block of pixels for the gridder, a chunk of visibilities
 // for the degridder.  Items are built on the host from the plan (metadata, channel windows and visibility
@@ -46,6 +47,7 @@ struct idgx_batch
     // Single-launch mode per direction: shape (-1 = per-class launches), item count and resident workgroups of
     // the merged queue, which lives after the class slices.
     int single_shape[2] = {-1, -1};
+    int single_tiles[idgx::kMaxClasses] = {}; // gridder tiles per class of a kShapeTilesPerClass merged queue
     int single_items[2] = {};
     int single_workgroups[2] = {};
     size_t single_offset[2] = {};
@@ -163,15 +165,19 @@ int upload_queue(idgx_batch *b, bool gridder, int k, int shape)
     return b->workgroups[dir][k] > 0 ? IDGX_OK : IDGX_ERR_DEVICE;
 }
 
-// Merged queue of every class under one shape, heaviest first across classes (stable: ties keep class order).
+// Merged queue of every class under one shape (kShapeTilesPerClass: each class's items cut by its own gridder
+// shape's tiles per wave), heaviest first across classes (stable: ties keep class order).
 int upload_single_queue(idgx_batch *b, bool gridder, int shape)
 {
     const int dir = gridder ? 0 : 1;
+    const bool per_class = shape == idgx::kShapeTilesPerClass;
     std::vector<Item> items;
     for (int k = 0; k < b->nr_classes; ++k)
     {
-        const std::vector<Item> ki = build_items(b, gridder, k, shape);
+        const std::vector<Item> ki = build_items(b, gridder, k, per_class ? b->shape[0][k] : shape);
         items.insert(items.end(), ki.begin(), ki.end());
+        if (per_class)
+            b->single_tiles[k] = idgx::mixed_shape(b->shape[0][k]).tiles;
     }
     std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
     size_t room = 0;
@@ -261,6 +267,8 @@ int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st, i
         q.items = b->d_items[dir] + b->single_offset[dir];
         q.nr_items = b->single_items[dir];
         q.counter = b->d_counters + dir * idgx::kMaxClasses;
+        for (int k = 0; k < idgx::kMaxClasses; ++k)
+            q.tiles[k] = b->single_tiles[k];
         const int sh = b->single_shape[dir], wg = b->single_workgroups[dir];
         e = gridder ? idgx::launch_grid_mixed_queue(q, b->precision, sh, wg, st)
                     : idgx::launch_degrid_mixed_queue(q, b->precision, sh, wg, st);
@@ -404,7 +412,12 @@ extern "C" int idgx_batch_set_class_shape(idgx_batch *b, int cls, int gridder_sh
     drop_graph(b);
     int rc = IDGX_OK;
     if (gridder_shape != b->shape[0][cls])
+    {
         rc = upload_queue(b, true, cls, gridder_shape);
+        // A single launch with per-class tiles follows the classes' gridder shapes.
+        if (rc == IDGX_OK && b->single_shape[0] == idgx::kShapeTilesPerClass)
+            rc = upload_single_queue(b, true, idgx::kShapeTilesPerClass);
+    }
     if (rc == IDGX_OK && degridder_shape != b->shape[1][cls])
         rc = upload_queue(b, false, cls, degridder_shape);
     return rc;
@@ -438,15 +451,15 @@ extern "C" int idgx_batch_get_shape(const idgx_batch *b, int *gridder_shape, int
 
 extern "C" int idgx_batch_set_single_launch(idgx_batch *b, int gridder_shape, int degridder_shape)
 {
-    if (!b || gridder_shape < -1 || gridder_shape >= idgx::mixed_shape_count() || degridder_shape < -1 ||
-        degridder_shape >= idgx::mixed_shape_count())
+    if (!b || gridder_shape < idgx::kShapeTilesPerClass || gridder_shape >= idgx::mixed_shape_count() ||
+        degridder_shape < -1 || degridder_shape >= idgx::mixed_shape_count())
         return IDGX_ERR_INVALID_ARGUMENT;
     drop_graph(b);
     const int want[2] = {gridder_shape, degridder_shape};
     int rc = IDGX_OK;
     for (int dir = 0; dir < 2 && rc == IDGX_OK; ++dir)
     {
-        if (want[dir] < 0)
+        if (want[dir] == -1)
             b->single_shape[dir] = -1;
         else
             rc = upload_single_queue(b, dir == 0, want[dir]);
@@ -535,11 +548,14 @@ extern "C" int idgx_batch_autotune(idgx_batch *b, const idgx_batch_data *data, i
         }
         if (rc != IDGX_OK || b->single_shape[dir] < 0)
             continue;
-        // Single-launch mode: the one shape of the merged queue, timed on the whole direction.
+        // Single-launch mode: the one shape of the merged queue (for the gridder also each class's tuned tiles in
+        // the one launch), timed on the whole direction.
         float best = -1.F;
         int best_shape = b->single_shape[dir];
-        for (int sh = 0; sh < idgx::mixed_shape_count() && rc == IDGX_OK; ++sh)
+        for (int sh = gridder ? idgx::kShapeTilesPerClass : 0; sh < idgx::mixed_shape_count() && rc == IDGX_OK; ++sh)
         {
+            if (sh == -1)
+                continue;
             rc = upload_single_queue(b, gridder, sh);
             if (rc == IDGX_OK)
                 rc = run(b, data, gridder, st); // warm-up

@@ -43,8 +43,13 @@ struct QueueArgs
     KernelArgs cls[kMaxClasses];
     const uint2 *items;
     int nr_items;
-    int *counter; // zeroed by a reset kernel before every launch
+    int *counter;           // zeroed by a reset kernel before every launch
+    int tiles[kMaxClasses]; // gridder tiles per wave of each class (shape kShapeTilesPerClass only)
 };
+
+// Gridder queue "shape" of a single launch over several classes in which each item takes its class's tiles per
+// wave (QueueArgs::tiles: 2, 3, 4, 6 or 8), with the default LDS batch and phases on the VALU.
+constexpr int kShapeTilesPerClass = IDGX_SHAPE_TILES_PER_CLASS;
 
 // Tile shape of the mixed kernels: gridder = tiles of 16 pixels per wave (PT) and visibilities staged per LDS
 // batch (VB); degridder = tiles of 16 visibilities per wave (VT) and pixels staged per LDS batch (PB).

@@ -63,6 +63,9 @@ enum DType
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
+// Default tile shape: gridder 8 tiles of 16 pixels per wave with 256 visibilities per LDS batch, degridder 4 tiles
+// of 16 visibilities per wave with 256 pixels per LDS batch.
+constexpr int kDefaultPT = 8, kDefaultVB = 256, kDefaultVT = 4, kDefaultPB = 256;
 
 // Round a pair of floats to two 16-bit values (RNE), x0 in the low half.
 template <int DT> __device__ __forceinline__ uint32_t pack2(float x0, float x1)
@@ -377,11 +380,13 @@ template <int VT, int PB> struct DegridderShared
 // ---------------------------------------------------------------------------------------------------------------
 // Gridder item: 4 waves x PT tiles x 16 pixels (block pblk) of subgrid s; visibilities staged VB at a time.
 // ---------------------------------------------------------------------------------------------------------------
-template <int DT, bool SPLIT, int PT, int VB, bool PM>
-__device__ __forceinline__ void gridder_item(const KernelArgs &a, int s, int pblk, GridderShared<PT, VB> &sm)
+template <int DT, bool SPLIT, int PT, int VB, bool PM, class Shared>
+__device__ __forceinline__ void gridder_item(const KernelArgs &a, int s, int pblk, Shared &sm)
 {
     constexpr int kBlocks = VB / 16;
     constexpr int kPix = kWaves * PT * 16;
+    static_assert(sizeof(sm.b) == sizeof(u32x4) * kBlocks * 64 && sizeof(sm.pix) >= sizeof(float) * kPix * 8,
+                  "LDS staging of the item's shape");
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int S = a.p.S, P = S * S;
@@ -932,6 +937,38 @@ __global__ __launch_bounds__(kThreads, (gridder_min_blocks<PT, VB>())) void grid
     }
 }
 
+// One launch over items of several subgrid classes, each item with its class's tiles per wave (q.tiles[class]),
+// LDS batch kDefaultVB, phases on the VALU.  The LDS holds the largest tile count's staging; every variant's
+// registers fit the same occupancy (4 waves per SIMD), so a class loses nothing to sharing the launch.
+template <int DT, bool SPLIT>
+__global__ __launch_bounds__(kThreads, (gridder_min_blocks<8, kDefaultVB>())) void gridder_mfma_queue_tiles(QueueArgs q)
+{
+    __shared__ GridderShared<8, kDefaultVB> sm;
+    int cls, s, blk;
+    while (next_item(q, cls, s, blk))
+    {
+        switch (q.tiles[cls]) // workgroup-uniform
+        {
+        case 2:
+            gridder_item<DT, SPLIT, 2, kDefaultVB, false>(q.cls[cls], s, blk, sm);
+            break;
+        case 3:
+            gridder_item<DT, SPLIT, 3, kDefaultVB, false>(q.cls[cls], s, blk, sm);
+            break;
+        case 4:
+            gridder_item<DT, SPLIT, 4, kDefaultVB, false>(q.cls[cls], s, blk, sm);
+            break;
+        case 6:
+            gridder_item<DT, SPLIT, 6, kDefaultVB, false>(q.cls[cls], s, blk, sm);
+            break;
+        default:
+            gridder_item<DT, SPLIT, 8, kDefaultVB, false>(q.cls[cls], s, blk, sm);
+            break;
+        }
+        __syncthreads();
+    }
+}
+
 template <int DT, bool SPLIT, int VT, int PB, bool PM>
 __global__ __launch_bounds__(kThreads, (degridder_min_blocks<VT, PB>())) void degridder_mfma_queue(QueueArgs q)
 {
@@ -1071,7 +1108,6 @@ template <int DT, bool SPLIT> __global__ __launch_bounds__(kThreads) void degrid
 
 // ---- launchers ------------------------------------------------------------------------------------------------
 
-constexpr int kDefaultPT = 8, kDefaultVB = 256, kDefaultVT = 4, kDefaultPB = 256;
 
 // Gridder tiles per wave for subgrid size S.  A wave with more tiles spreads each visibility's staged work over
 // more pixels (issue efficiency relative to 8 tiles, measured on sizes the tiles cover exactly: 6 -> 0.965,
@@ -1193,6 +1229,24 @@ template <int DT, bool SPLIT> void *queue_kernel(bool gridder, int shape)
 
 void *queue_kernel_for(int precision, bool gridder, int shape)
 {
+    if (shape == kShapeTilesPerClass)
+    {
+        if (!gridder)
+            return nullptr;
+        switch (precision)
+        {
+        case IDGX_MIXED_BF16X3:
+            return reinterpret_cast<void *>(&gridder_mfma_queue_tiles<kBF16, true>);
+        case IDGX_MIXED_FP16X3:
+            return reinterpret_cast<void *>(&gridder_mfma_queue_tiles<kFP16, true>);
+        case IDGX_MIXED_BF16:
+            return reinterpret_cast<void *>(&gridder_mfma_queue_tiles<kBF16, false>);
+        case IDGX_MIXED_FP16:
+            return reinterpret_cast<void *>(&gridder_mfma_queue_tiles<kFP16, false>);
+        default:
+            return nullptr;
+        }
+    }
     switch (precision)
     {
     case IDGX_MIXED_BF16X3:

@@ -100,6 +100,7 @@ class AdderPlan(ctypes.Structure):
                 ("z_min", ctypes.c_int32), ("nr_layers", ctypes.c_int32)]
 
 
+SHAPE_TILES_PER_CLASS = -2  # gridder single launch: every item with its class's tiles per wave
 STAGE_GRID = 1
 STAGE_DEGRID = 2
 

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from helpers import oracle_degrid, oracle_grid, per_subgrid_rel_l2, rel_l2
-from idgx import STAGE_DEGRID, STAGE_GRID, Batch, observation
+from idgx import SHAPE_TILES_PER_CLASS, STAGE_DEGRID, STAGE_GRID, Batch, observation
 from idgx.device import DeviceBatch, DeviceWorkload, selection_lists, upload
 
 pytestmark = pytest.mark.gpu
@@ -372,11 +372,49 @@ def test_autotuner_tunes_the_single_launch_shape():
     tg, td = db.batch.autotune(db.data, repeats=1)
     g, d = db.batch.single_launch()
     shapes = Batch.shapes()
-    print(f"single launch autotuned: gridder {shapes[g]} {tg:.3f} ms, degridder {shapes[d]} {td:.3f} ms")
-    assert 0 <= g < len(shapes) and 0 <= d < len(shapes) and tg > 0 and td > 0
+    g_name = "tiles per class" if g == SHAPE_TILES_PER_CLASS else shapes[g]
+    print(f"single launch autotuned: gridder {g_name} {tg:.3f} ms, degridder {shapes[d]} {td:.3f} ms")
+    assert (g == SHAPE_TILES_PER_CLASS or 0 <= g < len(shapes)) and 0 <= d < len(shapes) and tg > 0 and td > 0
     _zero_outputs(db)
     db.grid()
     db.degrid()
     for k in range(3):
         assert rel_l2(db.subgrids(k), sub0[k]) < 2e-6, f"class {k}"
     assert rel_l2(db.visibilities_out(), vis0) < 2e-6
+
+
+def test_single_launch_with_each_class_tiles_equals_per_class_launches_bitwise():
+    """Gridder single launch with IDGX_SHAPE_TILES_PER_CLASS: one persistent kernel in which every item takes its
+    class's tiles per wave (3 / 8 / 6 for S = 24 / 32 / 48) gives the per-class launches' bits; changing a class's
+    shape afterwards rebuilds the merged queue."""
+    w = observation.make_observation("small")
+    db = DeviceBatch(_mixed_size_classes(w), precision="fp16x3")
+    shapes = Batch.shapes()
+    g_picks = [shapes.index((3, 256, 0)), shapes.index((8, 256, 0)), shapes.index((6, 256, 0))]
+    d = shapes.index((4, 256, 0))
+    for k, g in enumerate(g_picks):
+        db.batch.set_class_shape(k, g, d)
+    db.grid()
+    db.degrid()
+    sub0 = [db.subgrids(k) for k in range(3)]
+    vis0 = db.visibilities_out()
+    db.batch.set_single_launch(SHAPE_TILES_PER_CLASS, d)
+    assert db.batch.single_launch() == (SHAPE_TILES_PER_CLASS, d)
+    _zero_outputs(db)
+    db.grid()
+    db.degrid()
+    for k in range(3):
+        assert db.subgrids(k).tobytes() == sub0[k].tobytes(), f"class {k}"
+    assert db.visibilities_out().tobytes() == vis0.tobytes()
+    # Class 1 switches to 4 tiles: per-class launches and the single launch follow it together.
+    db.batch.set_class_shape(1, shapes.index((4, 256, 0)), d)
+    db.batch.set_single_launch(-1, -1)
+    _zero_outputs(db)
+    db.grid()
+    sub1 = [db.subgrids(k) for k in range(3)]
+    db.batch.set_single_launch(SHAPE_TILES_PER_CLASS, d)
+    _zero_outputs(db)
+    db.grid()
+    for k in range(3):
+        assert db.subgrids(k).tobytes() == sub1[k].tobytes(), f"class {k} after the shape change"
+    assert rel_l2(db.subgrids(1), sub0[1]) < 2e-6
