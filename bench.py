@@ -139,6 +139,8 @@ def kernel_key(precision, path, direction, shape=None):
     split = "true" if precision.endswith("x3") else "false"
     if path == "one-shot":
         return f"{direction}_mfma<{dt}, {split}, {8 if direction == 'gridder' else 4}, 256, false>"
+    if shape == "tiles per class":  # one launch, every item with its class's tiles per wave
+        return f"gridder_mfma_queue_tiles<{dt}, {split}>"
     if isinstance(shape, list):  # one shape per class: one kernel only when they agree
         if len(set(shape)) != 1:
             return None
@@ -179,7 +181,9 @@ class OneShot:
 
 
 class Batched:
-    """Persistent work-queue kernels over the whole plan (idgx_batch_*), tile shape chosen by the autotuner."""
+    """Persistent work-queue kernels over the whole plan (idgx_batch_*), tile shapes chosen by the autotuner.  A
+    plan of several subgrid classes runs in the batch's default mode: one persistent launch per direction over
+    the merged queue of every class (gridder items with their class's tiles per wave)."""
 
     def __init__(self, classes, d_vis, precision, stream):
         from idgx import Batch
@@ -188,11 +192,30 @@ class Batched:
         self.precision, self.path = precision, "batched"
         self.db = DeviceBatch(classes, precision=precision, visibilities_on_device=d_vis)
         tg, td = self.db.batch.autotune(self.db.data, repeats=2, stream=stream)
+        self.describe(tg, td)
+
+    def describe(self, tg, td):
+        """Shapes, launch mode and queue sizes of the batch as it now runs."""
+        from idgx import Batch
+
+        classes = [c["w"] for c in self.db.classes]
         shapes = Batch.shapes()
         per_class = [self.db.batch.class_shape(k) for k in range(len(classes))]
-        self.shapes = {"gridder": [shapes[g] for g, _ in per_class], "degridder": [shapes[d] for _, d in per_class]}
+        single = self.db.batch.single_launch()
+        self.shapes = {}
+        for i, direction in enumerate(("gridder", "degridder")):
+            if single[i] == -1:
+                self.shapes[direction] = [shapes[pc[i]] for pc in per_class]
+            elif single[i] < 0:  # one launch, each class with its own tiles per wave
+                self.shapes[direction] = "tiles per class"
+            else:
+                self.shapes[direction] = shapes[single[i]]
         items = self.db.batch.items()
-        self.info = {"shapes_per_class": [
+        self.info = {"launches": {d: ("one per class" if single[i] == -1 else "one over all classes")
+                                  for i, d in enumerate(("gridder", "degridder"))},
+                     "single_launch_shapes": {d: (self.shapes[d] if single[i] != -1 else None)
+                                              for i, d in enumerate(("gridder", "degridder"))},
+                     "shapes_per_class": [
                          {"subgrid_size": int(c["subgrid_size"]),
                           "gridder": {"tiles_per_wave": shapes[g][0], "lds_visibilities": shapes[g][1],
                                       "phase_mfma": shapes[g][2]},
@@ -363,9 +386,9 @@ def class_sample(bo, k, idx, d_vis, d_flags):
 
 
 def run_batched_config(args):
-    """Batched uneven workload: one persistent launch per subgrid class and direction (each class with its
-    autotuned tile shape), then the single-launch form (one launch per direction over the merged queue of every
-    class, one autotuned shape); flags as selection lists, all built in HBM; throughput counts the selected
+    """Batched uneven workload: the whole plan as one persistent launch per direction over the merged queue of
+    every subgrid class (gridder items with their class's autotuned tiles per wave), then, beside it, one launch
+    per class and direction; flags as selection lists, all built in HBM; throughput counts the selected
     (unflagged) visibilities."""
     import torch
 
@@ -424,8 +447,10 @@ def run_batched_config(args):
                "gridder_mvis_s": nvis / tg / 1e6, "degridder_mvis_s": nvis / td / 1e6,
                "gridder_tflops": flops / tg / 1e12, "degridder_tflops": flops / td / 1e12}
         rec.update(runner.info)
-        print(f"bench: {prec} batched: gridder {tg * 1e3:.2f} ms, degridder {td * 1e3:.2f} ms, "
-              f"{rec['round_trip_mvis_s']:.1f} Mvis/s", file=sys.stderr, flush=True)
+        print(f"bench: {prec} batched (gridder {runner.info['launches']['gridder']}, {runner.shapes['gridder']}; "
+              f"degridder {runner.info['launches']['degridder']}, {runner.shapes['degridder']}): gridder "
+              f"{tg * 1e3:.2f} ms, degridder {td * 1e3:.2f} ms, {rec['round_trip_mvis_s']:.1f} Mvis/s",
+              file=sys.stderr, flush=True)
         if not args.no_check:
             acc = {}
             for k, c in enumerate(bo["classes"]):
@@ -445,24 +470,17 @@ def run_batched_config(args):
                                 "degridder": rel_l2(vis_out[sel], v64[sel])}
             rec["rel_l2_vs_fp64"] = acc
         if len(bo["classes"]) > 1:
-            # The same plan as one persistent launch per direction over the merged queue of every class.
-            from idgx import Batch
-            runner.db.batch.set_single_launch(*runner.db.batch.class_shape(0))
-            sg_ms, sd_ms = runner.db.batch.autotune(runner.db.data, repeats=2, stream=stream)
-            sg, sd = runner.db.batch.single_launch()
-            s_tg, s_td, _ = time_form(runner, args.steps, args.warmup, stream)
-            shapes = Batch.shapes()
-            g_name = "tiles per class " + "/".join(str(shapes[runner.db.batch.class_shape(k)[0]][0])
-                                                   for k in range(len(bo["classes"]))) if sg < 0 else shapes[sg]
-            rec["single_launch"] = {"gridder_ms": s_tg * 1e3, "degridder_ms": s_td * 1e3,
-                                    "round_trip_mvis_s": nvis / (s_tg + s_td) / 1e6,
-                                    "gridder_shape": g_name, "degridder_shape": shapes[sd],
-                                    "autotune_ms": {"gridder": sg_ms, "degridder": sd_ms}}
-            print(f"bench: {prec} batched, single launch over all classes (gridder {g_name}, degridder "
-                  f"{shapes[sd]}): gridder {s_tg * 1e3:.2f} ms, "
-                  f"degridder {s_td * 1e3:.2f} ms, {rec['single_launch']['round_trip_mvis_s']:.1f} Mvis/s",
-                  file=sys.stderr, flush=True)
+            # Beside the single launch: one persistent launch per class and direction, with the per-class shapes
+            # the same autotuner run chose.
+            single = runner.db.batch.single_launch()
             runner.db.batch.set_single_launch(-1, -1)
+            p_tg, p_td, _ = time_form(runner, args.steps, args.warmup, stream)
+            rec["per_class_launches"] = {"gridder_ms": p_tg * 1e3, "degridder_ms": p_td * 1e3,
+                                         "round_trip_mvis_s": nvis / (p_tg + p_td) / 1e6}
+            print(f"bench: {prec} batched, one launch per class: gridder {p_tg * 1e3:.2f} ms, "
+                  f"degridder {p_td * 1e3:.2f} ms, {rec['per_class_launches']['round_trip_mvis_s']:.1f} Mvis/s",
+                  file=sys.stderr, flush=True)
+            runner.db.batch.set_single_launch(*single)
         forms.append(rec)
         runners.append(runner)
     head, runner = forms[-1], runners[-1]

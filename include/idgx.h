@@ -298,9 +298,11 @@ int idgx_plan_segments(const idgx_planner *planner, const float *d_uvw, const do
  *
  * A batch holds up to 4 subgrid classes (one subgrid size each).  Its work queues of (class, subgrid, block)
  * items are built on the host at creation, one per class and direction, sorted heaviest first, and consumed by
- * persistent workgroups through device atomic counters: by default one launch per class, each class with its own
- * tile shape; in single-launch mode one launch per direction over a merged queue of every class's items, heaviest
- * first across classes, under one tile shape.  Optional per-class visibility selections (flagging, ragged timesteps) restrict each subgrid to the
+ * persistent workgroups through device atomic counters.  A batch of several classes starts in single-launch
+ * mode: the whole plan is one launch per direction over a merged queue of every class's items, heaviest first
+ * across classes, the gridder's items each with their class's tiles per wave.  idgx_batch_set_single_launch(b,
+ * -1, -1) switches to one launch per class, each class with its own tile shape (a one-class batch runs that way
+ * from the start; for one class the two modes coincide).  Optional per-class visibility selections (flagging, ragged timesteps) restrict each subgrid to the
  * listed entries: subgrid s uses vis_index[vis_offset[s] .. vis_offset[s+1]), each entry (t << 16) | c relative to
  * the subgrid's first row and channel window.  Creation allocates the queue memory; the execution entry points
  * allocate nothing, and idgx_batch_capture records them into a hipGraph (no memset nodes) for idgx_batch_launch.
@@ -365,8 +367,8 @@ enum
 };
 
 /* Single-launch mode per direction: one persistent launch over the merged queue of all classes with the given
- * shape (for the gridder also IDGX_SHAPE_TILES_PER_CLASS); -1 returns that direction to per-class launches (the
- * default).  The per-class shapes are kept.  For a
+ * shape (for the gridder also IDGX_SHAPE_TILES_PER_CLASS); -1 switches that direction to per-class launches.
+ * Defaults: several classes (IDGX_SHAPE_TILES_PER_CLASS, the default degridder shape); one class (-1, -1).  The per-class shapes are kept.  For a
  * given shape the results equal the per-class launches' bit for bit (every item writes its own outputs).  get
  * reports -1 for a direction in per-class mode. */
 int idgx_batch_set_single_launch(idgx_batch *batch, int gridder_shape, int degridder_shape);

@@ -1,12 +1,13 @@
 // Batched execution of a whole plan: the subgrids of up to four subgrid classes (one subgrid size each) run in
 // persistent gridder (or degridder) launches fed from device work queues, optionally captured in a hipGraph, with
 // the tile shapes chosen by an autotuner that times the compiled shapes on the real data.  Two launch modes:
-// - per class (default): one launch per class, each with its own tile shape.  The tile that suits a subgrid size
-//   is the one that covers its pixels with the fewest idle lanes: S = 24 and 48 prefer fewer tiles per wave than
-//   S = 32.
-// - single launch: one launch per direction over one queue that holds the items of every class, heaviest first
-//   across classes (the kernel reads each item's class from the item), with one tile shape for all of them or,
-//   for the gridder, each class's own tiles per wave inside the one launch (IDGX_SHAPE_TILES_PER_CLASS).
+// - single launch (the default for several classes): one launch per direction over one queue that holds the
+//   items of every class, heaviest first across classes (the kernel reads each item's class from the item), with
+//   one tile shape for all of them or, for the gridder, each class's own tiles per wave inside the one launch
+//   (IDGX_SHAPE_TILES_PER_CLASS, the default).  The tile that suits a subgrid size is the one that covers its
+//   pixels with the fewest idle lanes: S = 24 and 48 prefer fewer tiles per wave than S = 32.
+// - per class (the default for one class, where the two coincide): one launch per class, each with its own tile
+//   shape.
 //
 // Work queue.  An item is (class, subgrid, block): a block of pixels for the gridder, a chunk of visibilities
 // for the degridder.  Items are built on the host from the plan (metadata, channel windows and visibility
@@ -375,6 +376,14 @@ extern "C" int idgx_batch_create(const idgx_batch_class *classes, int nr_classes
         rc = upload_queue(b, true, k, idgx::mixed_default_shape(true, S));
         if (rc == IDGX_OK)
             rc = upload_queue(b, false, k, idgx::mixed_default_shape(false, S));
+    }
+    // Several classes: the whole plan as one persistent launch per direction, gridder items with their class's
+    // tiles per wave, the degridder with its default shape (the same for every subgrid size).
+    if (rc == IDGX_OK && nr_classes > 1)
+    {
+        rc = upload_single_queue(b, true, idgx::kShapeTilesPerClass);
+        if (rc == IDGX_OK)
+            rc = upload_single_queue(b, false, idgx::mixed_default_shape(false, b->plans[0].subgrid_size));
     }
     if (rc != IDGX_OK)
     {

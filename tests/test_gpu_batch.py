@@ -110,9 +110,12 @@ def _mixed_size_classes(w):
 
 
 def test_mixed_subgrid_sizes_in_one_batch():
+    """Three subgrid sizes in one batch, by default one persistent launch per direction over all of them (gridder
+    items with their class's tiles): the same bits as the one-shot kernels of each class."""
     w = observation.make_observation("small")
     shared = _mixed_size_classes(w)
     db = DeviceBatch(shared, precision="bf16x3")
+    assert db.batch.single_launch()[0] == SHAPE_TILES_PER_CLASS
     db.grid()
     for k, c in enumerate(shared):
         ref, _ = _oneshot(c, "bf16x3")
@@ -159,6 +162,7 @@ def test_graph_of_a_mixed_size_batch_replays_its_per_class_launches():
 
     w = observation.make_observation("small")
     db = DeviceBatch(_mixed_size_classes(w), precision="fp16x3")
+    db.batch.set_single_launch(-1, -1)
     shapes = Batch.shapes()
     for k, (g, d) in enumerate([((3, 256, 0), (4, 256, 0)), ((8, 256, 0), (6, 256, 0)), ((6, 256, 0), (2, 256, 0))]):
         db.batch.set_class_shape(k, shapes.index(g), shapes.index(d))
@@ -207,6 +211,7 @@ def test_every_tile_shape_gives_the_same_results(precision):
     to rounding, for subgrid sizes whose rows are not a multiple of the tile (S = 24, 48) and for S = 32."""
     w = observation.make_observation("small")
     db = DeviceBatch(_mixed_size_classes(w), precision=precision)
+    db.batch.set_single_launch(-1, -1)  # each compiled shape on its own launches
     db.grid()
     db.degrid()
     sub0 = [db.subgrids(k) for k in range(3)]
@@ -229,6 +234,7 @@ def test_classes_run_with_their_own_tile_shapes():
     them beyond rounding."""
     w = observation.make_observation("small")
     db = DeviceBatch(_mixed_size_classes(w), precision="fp16x3")
+    db.batch.set_single_launch(-1, -1)
     db.grid()
     db.degrid()
     sub0 = [db.subgrids(k) for k in range(3)]
@@ -277,6 +283,10 @@ def test_single_launch_over_all_classes_equals_per_class_launches_bitwise(gshape
     db = DeviceBatch(_mixed_size_classes(w), precision="fp16x3")
     shapes = Batch.shapes()
     g, d = shapes.index(gshape), shapes.index(dshape)
+    default_tiles = tuple(shapes[db.batch.class_shape(k)[0]][0] for k in range(3))
+    assert db.batch.single_launch() == (SHAPE_TILES_PER_CLASS, shapes.index((4, 256, 0)))  # several classes
+    assert default_tiles == (3, 8, 6)
+    db.batch.set_single_launch(-1, -1)
     db.batch.set_shape(g, d)
     assert db.batch.single_launch() == (-1, -1)
     items_per_class = db.batch.items()
