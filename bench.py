@@ -447,6 +447,8 @@ def run_batched_config(args):
                "gridder_mvis_s": nvis / tg / 1e6, "degridder_mvis_s": nvis / td / 1e6,
                "gridder_tflops": flops / tg / 1e12, "degridder_tflops": flops / td / 1e12}
         rec.update(runner.info)
+        launches = runner.db.batch.last_launches()
+        rec["persistent_launches_per_pass"] = {"gridder": launches[0], "degridder": launches[1]}
         print(f"bench: {prec} batched (gridder {runner.info['launches']['gridder']}, {runner.shapes['gridder']}; "
               f"degridder {runner.info['launches']['degridder']}, {runner.shapes['degridder']}): gridder "
               f"{tg * 1e3:.2f} ms, degridder {td * 1e3:.2f} ms, {rec['round_trip_mvis_s']:.1f} Mvis/s",

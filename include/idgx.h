@@ -391,6 +391,10 @@ int idgx_batch_degrid(idgx_batch *batch, const idgx_batch_data *data, void *stre
 int idgx_batch_capture(idgx_batch *batch, const idgx_batch_data *data, int stages, void *stream);
 int idgx_batch_launch(idgx_batch *batch, void *stream);
 
+/* Persistent gridder / degridder kernel launches that the latest idgx_batch_grid / idgx_batch_degrid (or the
+ * capture that recorded them) issued: 1 in single-launch mode, one per class with work otherwise. */
+int idgx_batch_last_launches(const idgx_batch *batch, int *gridder_launches, int *degridder_launches);
+
 /* Queue sizes (all classes) and resident workgroups (the largest launch) of the current shapes. */
 int idgx_batch_items(const idgx_batch *batch, int *gridder_items, int *degridder_items, int *gridder_workgroups,
                      int *degridder_workgroups);

@@ -50,6 +50,7 @@ struct idgx_batch
     int single_shape[2] = {-1, -1};
     int single_tiles[idgx::kMaxClasses] = {}; // gridder tiles per class of a kShapeTilesPerClass merged queue
     int single_items[2] = {};
+    int last_launches[2] = {}; // persistent launches the latest run of each direction issued
     int single_workgroups[2] = {};
     size_t single_offset[2] = {};
     int *d_counters = nullptr; // [direction][class]
@@ -166,6 +167,12 @@ int upload_queue(idgx_batch *b, bool gridder, int k, int shape)
     return b->workgroups[dir][k] > 0 ? IDGX_OK : IDGX_ERR_DEVICE;
 }
 
+// Whether direction `dir` runs as one launch over the merged queue (a shape index or kShapeTilesPerClass).
+bool single_mode(const idgx_batch *b, int dir)
+{
+    return b->single_shape[dir] != -1;
+}
+
 // Merged queue of every class under one shape (kShapeTilesPerClass: each class's items cut by its own gridder
 // shape's tiles per wave), heaviest first across classes (stable: ties keep class order).
 int upload_single_queue(idgx_batch *b, bool gridder, int shape)
@@ -258,7 +265,8 @@ int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st, i
     if (rc != IDGX_OK)
         return rc;
     hipError_t e = idgx::launch_reset_counters(b->d_counters + dir * idgx::kMaxClasses, idgx::kMaxClasses, st);
-    if (only < 0 && b->single_shape[dir] >= 0)
+    b->last_launches[dir] = 0;
+    if (only < 0 && single_mode(b, dir))
     {
         for (int k = 0; k < b->nr_classes && e == hipSuccess && !gridder; ++k)
             if (b->nr_items[dir][k] > 0)
@@ -273,6 +281,7 @@ int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st, i
         const int sh = b->single_shape[dir], wg = b->single_workgroups[dir];
         e = gridder ? idgx::launch_grid_mixed_queue(q, b->precision, sh, wg, st)
                     : idgx::launch_degrid_mixed_queue(q, b->precision, sh, wg, st);
+        b->last_launches[dir] += e == hipSuccess;
         return to_status(e);
     }
     for (int k = 0; k < b->nr_classes && e == hipSuccess; ++k)
@@ -288,6 +297,7 @@ int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st, i
         q.counter = b->d_counters + dir * idgx::kMaxClasses + k;
         e = gridder ? idgx::launch_grid_mixed_queue(q, b->precision, b->shape[0][k], b->workgroups[0][k], st)
                     : idgx::launch_degrid_mixed_queue(q, b->precision, b->shape[1][k], b->workgroups[1][k], st);
+        b->last_launches[dir] += e == hipSuccess;
     }
     return to_status(e);
 }
@@ -555,7 +565,7 @@ extern "C" int idgx_batch_autotune(idgx_batch *b, const idgx_batch_data *data, i
                 rc = upload_queue(b, gridder, k, best_shape);
             total[dir] += std::max(best, 0.F);
         }
-        if (rc != IDGX_OK || b->single_shape[dir] < 0)
+        if (rc != IDGX_OK || !single_mode(b, dir))
             continue;
         // Single-launch mode: the one shape of the merged queue (for the gridder also each class's tuned tiles in
         // the one launch), timed on the whole direction.
@@ -648,7 +658,7 @@ extern "C" int idgx_batch_items(const idgx_batch *b, int *gridder_items, int *de
     int items[2] = {0, 0}, wg[2] = {0, 0};
     for (int dir = 0; dir < 2; ++dir)
     {
-        if (b->single_shape[dir] >= 0)
+        if (single_mode(b, dir))
         {
             items[dir] = b->single_items[dir];
             wg[dir] = b->single_workgroups[dir];
@@ -668,6 +678,17 @@ extern "C" int idgx_batch_items(const idgx_batch *b, int *gridder_items, int *de
         *gridder_workgroups = wg[0];
     if (degridder_workgroups)
         *degridder_workgroups = wg[1];
+    return IDGX_OK;
+}
+
+extern "C" int idgx_batch_last_launches(const idgx_batch *b, int *gridder_launches, int *degridder_launches)
+{
+    if (!b)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    if (gridder_launches)
+        *gridder_launches = b->last_launches[0];
+    if (degridder_launches)
+        *degridder_launches = b->last_launches[1];
     return IDGX_OK;
 }
 

@@ -174,6 +174,7 @@ def lib():
         L.idgx_batch_get_class_shape.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                                  ctypes.POINTER(ctypes.c_int)]
         L.idgx_batch_set_single_launch.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.idgx_batch_last_launches.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.idgx_batch_get_single_launch.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.idgx_batch_shape_info.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
         L.idgx_batch_autotune.argtypes = [vp, ctypes.POINTER(BatchData), ctypes.c_int, vp,
@@ -497,6 +498,13 @@ class Batch:
         g, d = ctypes.c_int(), ctypes.c_int()
         _check(lib().idgx_batch_get_single_launch(self.handle, ctypes.byref(g), ctypes.byref(d)),
                "idgx_batch_get_single_launch")
+        return g.value, d.value
+
+    def last_launches(self):
+        """(gridder, degridder) persistent launches the latest grid / degrid call issued."""
+        g, d = ctypes.c_int(), ctypes.c_int()
+        _check(lib().idgx_batch_last_launches(self.handle, ctypes.byref(g), ctypes.byref(d)),
+               "idgx_batch_last_launches")
         return g.value, d.value
 
     @staticmethod

@@ -117,6 +117,7 @@ def test_mixed_subgrid_sizes_in_one_batch():
     db = DeviceBatch(shared, precision="bf16x3")
     assert db.batch.single_launch()[0] == SHAPE_TILES_PER_CLASS
     db.grid()
+    assert db.batch.last_launches()[0] == 1
     for k, c in enumerate(shared):
         ref, _ = _oneshot(c, "bf16x3")
         assert db.subgrids(k).tobytes() == ref.tobytes(), f"class S={c['subgrid_size']}"
@@ -295,6 +296,7 @@ def test_single_launch_over_all_classes_equals_per_class_launches_bitwise(gshape
     sub0 = [db.subgrids(k) for k in range(3)]
     vis0 = db.visibilities_out()
 
+    assert db.batch.last_launches() == (3, 3)
     db.batch.set_single_launch(g, d)
     assert db.batch.single_launch() == (g, d)
     gi, di, gw, dw = db.batch.items()
@@ -302,6 +304,7 @@ def test_single_launch_over_all_classes_equals_per_class_launches_bitwise(gshape
     _zero_outputs(db)
     db.grid()
     db.degrid()
+    assert db.batch.last_launches() == (1, 1)
     for k in range(3):
         assert db.subgrids(k).tobytes() == sub0[k].tobytes(), f"class {k}"
     assert db.visibilities_out().tobytes() == vis0.tobytes()
@@ -399,6 +402,7 @@ def test_single_launch_with_each_class_tiles_equals_per_class_launches_bitwise()
     shape afterwards rebuilds the merged queue."""
     w = observation.make_observation("small")
     db = DeviceBatch(_mixed_size_classes(w), precision="fp16x3")
+    db.batch.set_single_launch(-1, -1)
     shapes = Batch.shapes()
     g_picks = [shapes.index((3, 256, 0)), shapes.index((8, 256, 0)), shapes.index((6, 256, 0))]
     d = shapes.index((4, 256, 0))
@@ -408,11 +412,13 @@ def test_single_launch_with_each_class_tiles_equals_per_class_launches_bitwise()
     db.degrid()
     sub0 = [db.subgrids(k) for k in range(3)]
     vis0 = db.visibilities_out()
+    assert db.batch.last_launches() == (3, 3)
     db.batch.set_single_launch(SHAPE_TILES_PER_CLASS, d)
     assert db.batch.single_launch() == (SHAPE_TILES_PER_CLASS, d)
     _zero_outputs(db)
     db.grid()
     db.degrid()
+    assert db.batch.last_launches() == (1, 1)
     for k in range(3):
         assert db.subgrids(k).tobytes() == sub0[k].tobytes(), f"class {k}"
     assert db.visibilities_out().tobytes() == vis0.tobytes()
