@@ -998,7 +998,8 @@ __device__ __forceinline__ int xcd_contiguous(int b, int n)
 // also writes the slot n table.  Subgrids outside the arrays are skipped (the degridder skips them too).  Each XCD
 // takes a contiguous run of subgrids (plan order: baseline by baseline), so the stations' A-terms it reads stay in
 // its own L2 instead of every XCD streaming the whole table.
-template <int DT, bool SPLIT> __global__ __launch_bounds__(kThreads) void degrid_prepare_kernel(KernelArgs a)
+template <int DT, bool SPLIT, int CACHED>
+__global__ __launch_bounds__(kThreads) void degrid_prepare_kernel(KernelArgs a)
 {
     __shared__ unsigned maxbits;
     const int s = xcd_contiguous(blockIdx.x, gridDim.x), tid = threadIdx.x;
@@ -1021,7 +1022,9 @@ template <int DT, bool SPLIT> __global__ __launch_bounds__(kThreads) void degrid
         return; // uniform
     const float2 *sub = reinterpret_cast<const float2 *>(a.subgrids) + static_cast<size_t>(s) * kCorr * P;
     uint4 *dst = const_cast<uint4 *>(a.prepared) + static_cast<size_t>(s) * nslots * 2;
-    constexpr int kCached = 4; // slots per thread held in registers between the max and the split (S <= 32)
+    // Slots per thread held in registers between the FP16 scale and the split, so that the subgrid is read once:
+    // 4 covers S <= 32, 9 covers S <= 48; larger subgrids read it twice.
+    constexpr int kCached = CACHED;
     if (nslots <= kCached * kThreads)
     {
         // One pass over the subgrid: pixels stay in registers while the workgroup finds the FP16 scale.
@@ -1378,19 +1381,28 @@ hipError_t launch_degrid_prepare(const KernelArgs &a, int precision, hipStream_t
     if (!a.prepared || !a.prepared_scale || !a.slot_n)
         return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>(a.p.nr_subgrids));
+    // FP16 needs the subgrid's maximum before the split: hold up to 9 slots per thread for S <= 48 (BF16 has no
+    // scale and streams the slots in one pass either way).
+    const bool wide = a.p.S * degrid_row_stride(a.p.S) > 4 * kThreads;
     switch (precision)
     {
     case IDGX_MIXED_BF16X3:
-        degrid_prepare_kernel<kBF16, true><<<grid, kThreads, 0, st>>>(a);
+        degrid_prepare_kernel<kBF16, true, 4><<<grid, kThreads, 0, st>>>(a);
         break;
     case IDGX_MIXED_FP16X3:
-        degrid_prepare_kernel<kFP16, true><<<grid, kThreads, 0, st>>>(a);
+        if (wide)
+            degrid_prepare_kernel<kFP16, true, 9><<<grid, kThreads, 0, st>>>(a);
+        else
+            degrid_prepare_kernel<kFP16, true, 4><<<grid, kThreads, 0, st>>>(a);
         break;
     case IDGX_MIXED_BF16:
-        degrid_prepare_kernel<kBF16, false><<<grid, kThreads, 0, st>>>(a);
+        degrid_prepare_kernel<kBF16, false, 4><<<grid, kThreads, 0, st>>>(a);
         break;
     case IDGX_MIXED_FP16:
-        degrid_prepare_kernel<kFP16, false><<<grid, kThreads, 0, st>>>(a);
+        if (wide)
+            degrid_prepare_kernel<kFP16, false, 9><<<grid, kThreads, 0, st>>>(a);
+        else
+            degrid_prepare_kernel<kFP16, false, 4><<<grid, kThreads, 0, st>>>(a);
         break;
     default:
         return hipErrorInvalidValue;
