@@ -434,3 +434,33 @@ def test_single_launch_with_each_class_tiles_equals_per_class_launches_bitwise()
     for k in range(3):
         assert db.subgrids(k).tobytes() == sub1[k].tobytes(), f"class {k} after the shape change"
     assert rel_l2(db.subgrids(1), sub0[1]) < 2e-6
+
+
+def test_graph_of_the_default_single_launch_replays_match_eager():
+    """A mixed-size batch in its default mode (one launch per direction, gridder items with their class's tiles)
+    captured in a hipGraph: every replay equals the eager run bit for bit, one persistent launch per direction."""
+    import torch
+
+    w = observation.make_observation("small")
+    db = DeviceBatch(_mixed_size_classes(w), precision="fp16x3")
+    assert db.batch.single_launch()[0] == SHAPE_TILES_PER_CLASS
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        db.grid(stream)
+        db.degrid(stream)
+    stream.synchronize()
+    assert db.batch.last_launches() == (1, 1)
+    sub0 = [db.subgrids(k) for k in range(3)]
+    vis0 = db.visibilities_out()
+    db.batch.capture(db.data, STAGE_GRID | STAGE_DEGRID, stream)
+    assert db.batch.last_launches() == (1, 1)
+    for r in range(3):
+        _zero_outputs(db)
+        torch.cuda.synchronize()
+        db.batch.launch(stream)
+        stream.synchronize()
+        for k in range(3):
+            assert db.subgrids(k).tobytes() == sub0[k].tobytes(), f"replay {r}, class {k}"
+        assert db.visibilities_out().tobytes() == vis0.tobytes(), f"replay {r}"
+        x = torch.ones(1000, device="cuda").sum()  # eager work between replays
+        assert float(x) == 1000.0
