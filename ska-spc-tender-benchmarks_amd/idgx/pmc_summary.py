@@ -1,7 +1,9 @@
-"""Turn rocprofv3 --pmc CSV output into per-kernel HBM traffic and instruction-mix figures.
+"""Turn rocprofv3 --pmc CSV output into per-kernel HBM traffic and instruction-mix figures, or into a table of
+any counters per kernel.
 
     python -m idgx.pmc_summary --cal-fetch DIR --cal-write DIR --fetch DIR --write DIR [--sq DIR] \
         --cal-mib 1024 --out profiles/hbm_traffic_ska_low
+    python -m idgx.pmc_summary --counters DIR [DIR ...] --out profiles/pipe_utilisation_ska_low [--title ...]
 
 FETCH_SIZE / WRITE_SIZE are read in KiB per dispatch.  The calibration runs are a plain device-to-device copy of
 --cal-mib MiB (idgx.calibrate); scale factor = true bytes / counter bytes of that copy, applied to the kernels'
@@ -55,15 +57,50 @@ def mean(v):
     return sum(v) / len(v) if v else float("nan")
 
 
+def counter_table(dirs, out, title, command):
+    """Mean per dispatch of every counter in `dirs` (one rocprofv3 --pmc run each), one row per kernel."""
+    per = collections.defaultdict(dict)
+    names = []
+    for d in dirs:
+        for k, counters in load(d).items():
+            for c, v in counters.items():
+                per[k][c] = v
+                if c not in names:
+                    names.append(c)
+    lines = [f"# {title}", ""]
+    if command:
+        lines += ["Commands:", "", "```", command, "```", ""]
+    lines += ["| kernel | launches | " + " | ".join(names) + " |", "|---|---|" + "---|" * len(names)]
+    res = {}
+    for k in sorted(per):
+        if k == "copy":
+            continue
+        n = max(len(v) for v in per[k].values())
+        res[k] = {c: mean(v) for c, v in per[k].items()}
+        lines.append(f"| `{k}` | {n} | " + " | ".join(f"{mean(per[k].get(c, [])):.4g}" for c in names) + " |")
+    with open(out + ".md", "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(out + ".json", "w") as f:
+        json.dump(res, f, indent=1)
+    print("\n".join(lines))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     for k in ("cal-fetch", "cal-write", "fetch", "write"):
-        ap.add_argument("--" + k, required=True)
+        ap.add_argument("--" + k)
     ap.add_argument("--sq")
+    ap.add_argument("--counters", nargs="+", help="tabulate every counter of these runs per kernel")
+    ap.add_argument("--title", default="Counters per kernel (rocprofv3 --pmc)")
     ap.add_argument("--cal-mib", type=float, default=1024)
     ap.add_argument("--out", required=True)
     ap.add_argument("--command", default="")
     a = ap.parse_args()
+    if a.counters:
+        return counter_table(a.counters, a.out, a.title, a.command)
+    if not (a.cal_fetch and a.cal_write and a.fetch and a.write):
+        ap.error("--cal-fetch, --cal-write, --fetch and --write are required without --counters")
     true_bytes = a.cal_mib * 1024 * 1024
     f_fetch = true_bytes / (mean(load(a.cal_fetch)["copy"]["FETCH_SIZE"]) * 1024)
     f_write = true_bytes / (mean(load(a.cal_write)["copy"]["WRITE_SIZE"]) * 1024)
