@@ -262,20 +262,24 @@ def time_form(runner, steps, warmup, stream):
 def time_cycle(runner, dw, steps, warmup, stream):
     """Average time (ms) of each stage of an imaging cycle on resident buffers: clear the master grid, grid, subgrid
     FFT, add to the grid, master-grid FFT to the image and back (deconvolution would sit between the two), split
-    from the grid, inverse subgrid FFT, degrid (HIP events on `stream`)."""
+    from the grid, inverse subgrid FFT, degrid (HIP events on `stream`).  The master-grid transforms are restricted
+    to the subgrids' bounding box (idgx_grid_fft_box: the uv grid is zero outside it on the way to the image, and
+    only its rows are needed on the way back), and the clear zeroes only what the transform to the image reads
+    (idgx_grid_clear_box); the full transforms are timed beside the cycle."""
     import torch
 
-    from idgx import FFT_BACKWARD, FFT_FORWARD
+    from idgx import FFT_BACKWARD, FFT_FORWARD, grid_box, grid_clear_box, grid_fft_box
 
     plan = dw.plan()
+    box = grid_box(plan, dw.w["metadata"])
     d_sub, _ = runner.buffers()
     d_grid = dw.grid_buffer()
-    stages = [("grid_clear", lambda: d_grid.zero_()),
+    stages = [("grid_clear", lambda: grid_clear_box(plan, d_grid, 1, box, stream)),
               ("gridder", lambda: runner.grid(stream)),
               ("subgrid_fft", lambda: dw.fft(plan, FFT_FORWARD, d_sub, stream)),
               ("adder", lambda: dw.add(plan, d_grid, d_sub, stream)),
-              ("grid_fft_to_image", lambda: dw.grid_fft(plan, d_grid, 1, FFT_BACKWARD, stream)),
-              ("grid_fft_to_uv", lambda: dw.grid_fft(plan, d_grid, 1, FFT_FORWARD, stream)),
+              ("grid_fft_to_image", lambda: grid_fft_box(plan, d_grid, 1, FFT_BACKWARD, box, stream)),
+              ("grid_fft_to_uv", lambda: grid_fft_box(plan, d_grid, 1, FFT_FORWARD, box, stream)),
               ("splitter", lambda: dw.split(plan, d_grid, d_sub, stream)),
               ("subgrid_ifft", lambda: dw.fft(plan, FFT_BACKWARD, d_sub, stream)),
               ("degridder", lambda: runner.degrid(stream))]
@@ -294,8 +298,21 @@ def time_cycle(runner, dw, steps, warmup, stream):
             torch.cuda.synchronize()
             for i in range(len(stages)):
                 acc[i] += ev[i].elapsed_time(ev[i + 1])
+        # The full transforms of the same grid, for comparison (not part of the cycle).
+        full = {"to_image": lambda: dw.grid_fft(plan, d_grid, 1, FFT_BACKWARD, stream),
+                "to_uv": lambda: dw.grid_fft(plan, d_grid, 1, FFT_FORWARD, stream)}
+        full_ms = {}
+        for name, f in full.items():
+            f()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(steps):
+                f()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            full_ms[name] = e0.elapsed_time(e1) / steps
     del d_grid
-    return {name: acc[i] / steps for i, (name, _) in enumerate(stages)}
+    return {name: acc[i] / steps for i, (name, _) in enumerate(stages)}, box, full_ms
 
 
 def time_pcie(nvis_bytes, sub_bytes, repeats=3):
@@ -641,12 +658,15 @@ def main():
     plain = forms[0]
     cycle = None
     if args.cycle or args.config == "full_cycle":
-        cycle = time_cycle(runners[-1], dw, args.steps, args.warmup, stream)
+        cycle, box, full_ms = time_cycle(runners[-1], dw, args.steps, args.warmup, stream)
         total = sum(cycle.values())
         cycle = {"stages_ms": cycle, "total_ms": total, "cycle_mvis_s": nvis / (total / 1e3) / 1e6,
-                 "precision": head["precision"], "path": head["path"]}
+                 "precision": head["precision"], "path": head["path"], "grid_fft_box": list(box),
+                 "full_grid_fft_ms": full_ms}
         print(f"bench: imaging cycle ({head['precision']} {head['path']}): " +
-              ", ".join(f"{k} {v:.2f} ms" for k, v in cycle["stages_ms"].items()), file=sys.stderr, flush=True)
+              ", ".join(f"{k} {v:.2f} ms" for k, v in cycle["stages_ms"].items()) +
+              f"; total {total:.2f} ms; master-grid transforms on the box {list(box)} (full transforms: to image "
+              f"{full_ms['to_image']:.2f} ms, to uv {full_ms['to_uv']:.2f} ms)", file=sys.stderr, flush=True)
     tg, td = head["gridder_ms"] / 1e3, head["degridder_ms"] / 1e3
     dominant, t_dom = ("degridder", td) if td > tg else ("gridder", tg)
     achieved = flops / t_dom / 1e12

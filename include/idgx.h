@@ -232,6 +232,29 @@ int idgx_split_subgrids_layered(const idgx_plan *plan, const idgx_metadata *d_me
  * Calls that overlap on one device share the scratch, so issue them on one stream. */
 int idgx_grid_fft(const idgx_plan *plan, void *d_grid, int nr_layers, int direction, void *stream);
 
+/* The bounding box {x0, x1, y0, y1} of a plan's subgrids on the master grid, cells [x0, x1) x [y0, y1) clamped to
+ * [0, G): the only cells the adder writes and the splitter reads.  {0, 0, 0, 0} for an empty plan. */
+int idgx_grid_box(const idgx_plan *plan, const idgx_metadata *h_metadata, int32_t box[4]);
+
+/* idgx_grid_fft restricted to a box of cells {x0, x1, y0, y1} of every plane (0 <= x0 <= x1 <= G, same for y), for
+ * grids whose uv cells outside the box carry nothing (a plan's subgrids cover the box of idgx_grid_box):
+ *   BACKWARD (uv -> image): the grid must be zero outside the box.  The column transforms run only over the
+ *     64-column tiles that meet [x0, x1), and the row transforms read only those columns.  The image is complete
+ *     and equals idgx_grid_fft's bit for bit.
+ *   FORWARD (image -> uv): only rows [y0, y1) of the uv grid are produced, equal to idgx_grid_fft's bit for bit;
+ *     the other rows are left undefined.
+ * Grid sizes outside the three-pass kernels' range (see idgx_grid_fft) run the full transform.  No counterpart in
+ * the reference (its benchmark stops at the subgrids); it is the sparse master-grid transform of an imaging cycle
+ * whose uv coverage is smaller than the grid. */
+int idgx_grid_fft_box(const idgx_plan *plan, void *d_grid, int nr_layers, int direction, const int32_t box[4],
+                      void *stream);
+
+/* Zeroes every cell that idgx_grid_fft_box(BACKWARD) reads for this box (the 64-column tiles that meet [x0, x1),
+ * all rows; the whole grid for sizes outside the three-pass kernels' range), with a fill kernel: enough to start
+ * an imaging cycle's grid when the adder writes only inside the box, even if a FORWARD transform left the rest of
+ * the grid undefined. */
+int idgx_grid_clear_box(const idgx_plan *plan, void *d_grid, int nr_layers, const int32_t box[4], void *stream);
+
 /* Plain batched 2-D DFT, the transform of the reference's FFT benchmark (heFFTe fft2d, fft/inc/fft_benchmark.h:65-99,
  * run by fft/src/fft_benchmark.cpp:240-345): for b < batch,
  *   d_out[b][ky][kx] = sum_{y < ny, x < nx} d_in[b][y][x] exp(sign 2 pi i (ky y / ny + kx x / nx)),

@@ -321,6 +321,71 @@ extern "C" int idgx_grid_fft(const idgx_plan *plan, void *d_grid, int nr_layers,
                                            static_cast<hipStream_t>(stream)));
 }
 
+extern "C" int idgx_grid_box(const idgx_plan *plan, const idgx_metadata *h_metadata, int32_t box[4])
+{
+    const int rc = check_plan_struct(plan);
+    if (rc != IDGX_OK)
+        return rc;
+    if (!box || (plan->nr_subgrids > 0 && !h_metadata))
+        return IDGX_ERR_INVALID_ARGUMENT;
+    const long long G = plan->grid_size, S = plan->subgrid_size;
+    box[0] = box[1] = box[2] = box[3] = 0;
+    if (plan->nr_subgrids == 0)
+        return IDGX_OK;
+    long long x0 = G, x1 = 0, y0 = G, y1 = 0;
+    for (int s = 0; s < plan->nr_subgrids; ++s)
+    {
+        const long long x = h_metadata[s].x, y = h_metadata[s].y;
+        x0 = std::min(x0, x);
+        x1 = std::max(x1, x + S);
+        y0 = std::min(y0, y);
+        y1 = std::max(y1, y + S);
+    }
+    // Clamped to the grid (the adder and splitter skip the cells of a subgrid that overhang it).
+    x0 = std::max(0LL, std::min(x0, G));
+    y0 = std::max(0LL, std::min(y0, G));
+    x1 = std::max(x0, std::min(x1, G));
+    y1 = std::max(y0, std::min(y1, G));
+    box[0] = static_cast<int32_t>(x0);
+    box[1] = static_cast<int32_t>(x1);
+    box[2] = static_cast<int32_t>(y0);
+    box[3] = static_cast<int32_t>(y1);
+    return IDGX_OK;
+}
+
+extern "C" int idgx_grid_fft_box(const idgx_plan *plan, void *d_grid, int nr_layers, int direction,
+                                 const int32_t box[4], void *stream)
+{
+    const int rc = check_plan_struct(plan);
+    if (rc != IDGX_OK)
+        return rc;
+    const int G = plan->grid_size;
+    if (nr_layers < 1 || !d_grid || !box || (direction != IDGX_FFT_FORWARD && direction != IDGX_FFT_BACKWARD) ||
+        box[0] < 0 || box[0] > box[1] || box[1] > G || box[2] < 0 || box[2] > box[3] || box[3] > G)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    if (!idgx::grid_fft_supported(G))
+        return IDGX_ERR_UNSUPPORTED;
+    const int b[4] = {box[0], box[1], box[2], box[3]};
+    return to_status(idgx::launch_grid_fft(d_grid, G, nr_layers, direction == IDGX_FFT_FORWARD,
+                                           static_cast<hipStream_t>(stream), b));
+}
+
+extern "C" int idgx_grid_clear_box(const idgx_plan *plan, void *d_grid, int nr_layers, const int32_t box[4],
+                                   void *stream)
+{
+    const int rc = check_plan_struct(plan);
+    if (rc != IDGX_OK)
+        return rc;
+    const int G = plan->grid_size;
+    if (nr_layers < 1 || !d_grid || !box || box[0] < 0 || box[0] > box[1] || box[1] > G || box[2] < 0 ||
+        box[2] > box[3] || box[3] > G)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    if (!idgx::grid_fft_supported(G))
+        return IDGX_ERR_UNSUPPORTED;
+    const int b[4] = {box[0], box[1], box[2], box[3]};
+    return to_status(idgx::launch_grid_clear_box(d_grid, G, nr_layers, b, static_cast<hipStream_t>(stream)));
+}
+
 extern "C" int idgx_fft2d(const void *d_in, void *d_out, int nx, int ny, int64_t batch, int sign, int precision,
                           void *stream)
 {

@@ -217,6 +217,9 @@ struct FftArgs
     int log_n1, log_n2;
     int nplanes;       // planes of this launch
     int centred;       // 1: the (-1)^(x + y) sign flips of the centred transform, 0: a plain DFT
+    int ct0, nct;      // column passes: column tiles [ct0, ct0 + nct) of every row set (all gx / kColTile: dense)
+    int cx0, cx1;      // row pass: scratch columns outside [cx0, cx1) are read as zero (0, gx: dense)
+    int ry0, nry;      // row pass: rows [ry0, ry0 + nry) of each plane (0, gy: dense)
 };
 
 constexpr int ilog2(int n)
@@ -271,17 +274,25 @@ template <int L, int SG> __global__ __launch_bounds__(L / 16) void fft_row_kerne
     __syncthreads();
 
     const bool flip = A.centred != 0;
-    const int rows = A.nplanes * A.gy; // planes are contiguous: global row y is row y % gy of plane y / gy
+    // Work item i is row ry0 + i % nry of plane i / nry; planes are contiguous, so its global row is
+    // plane * gy + row.  Scratch columns outside [cx0, cx1) are taken as zero without being read.
+    const int rows = A.nplanes * A.nry;
+    auto global_row = [&](int i) { return (i / A.nry) * A.gy + A.ry0 + i % A.nry; };
+    auto load = [&](int gr, int col) {
+        return col >= A.cx0 && col < A.cx1 ? A.scratch[static_cast<size_t>(gr) * L + col] : make_float2(0.F, 0.F);
+    };
     float2 nxt[16];
-    int y = blockIdx.x;
-    if (y < rows)
+    int i = blockIdx.x;
+    if (i < rows)
     {
+        const int gr = global_row(i);
 #pragma unroll
         for (int r = 0; r < 16; ++r)
-            nxt[r] = A.scratch[static_cast<size_t>(y) * L + threadIdx.x + r * kQ];
+            nxt[r] = load(gr, threadIdx.x + r * kQ);
     }
-    for (; y < rows; y += gridDim.x)
+    for (; i < rows; i += gridDim.x)
     {
+        const int y = global_row(i);
         // An opaque copy of the thread index per row: what derives from it (twiddles, slots) is formed where it is
         // used instead of being hoisted out of the row loop and held in registers for every row.
         int j0 = threadIdx.x;
@@ -294,12 +305,13 @@ template <int L, int SG> __global__ __launch_bounds__(L / 16) void fft_row_kerne
 #pragma unroll
         for (int r = 0; r < 16; ++r)
             v[r] = nxt[r];
-        const int yn = y + gridDim.x;
-        if (yn < rows)
+        const int in_ = i + gridDim.x;
+        if (in_ < rows)
         {
+            const int gn = global_row(in_);
 #pragma unroll
             for (int r = 0; r < 16; ++r)
-                nxt[r] = A.scratch[static_cast<size_t>(yn) * L + j0 + r * kQ];
+                nxt[r] = load(gn, j0 + r * kQ);
         }
         int Ns = 1;
 #pragma unroll
@@ -393,14 +405,19 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
     __shared__ float buf[L * kColTile];
     const int c = threadIdx.x & (kColTile - 1);
     const int G = A.gx; // row stride
-    const int tmask = (1 << A.log_tiles) - 1;
-    // Tile index = plane x (tiles per plane) + (n2 or k1) x (G / kColTile) + column tile.
-    const int log_tpp = A.log_tiles + (PASS == kPassCol1 ? A.log_n2 : A.log_n1);
+    // Tile index = (plane x Q + q) x nct + (column tile - ct0), q = n2 (C1) or k1 (C2), Q = N2 or N1.
+    const int log_q = PASS == kPassCol1 ? A.log_n2 : A.log_n1;
     const size_t plane_elems = static_cast<size_t>(G) * A.gy;
     const float2 *in = PASS == kPassCol1 ? A.src : A.grid;
+    auto decode = [&](int tile, int &pl, int &x, int &q) {
+        const int rest = tile / A.nct;
+        x = (A.ct0 + tile - rest * A.nct) * kColTile + c;
+        q = rest & ((1 << log_q) - 1);
+        pl = rest >> log_q;
+    };
     auto element = [&](int tile, int i) {
-        const int pl = tile >> log_tpp, t = tile & ((1 << log_tpp) - 1);
-        const int x = (t & tmask) * kColTile + c, q = t >> A.log_tiles;
+        int pl, x, q;
+        decode(tile, pl, x, q);
         const int y = PASS == kPassCol1 ? A.N2 * i + q : A.N2 * q + i;
         return pl * plane_elems + static_cast<size_t>(y) * G + x;
     };
@@ -418,8 +435,8 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
     {
         int j0 = threadIdx.x / kColTile; // opaque per tile: keeps twiddles and slots out of registers across tiles
         asm volatile("" : "+v"(j0));
-        const int pl = tile >> log_tpp, t = tile & ((1 << log_tpp) - 1);
-        const int x = (t & tmask) * kColTile + c, q = t >> A.log_tiles;
+        int pl, x, q;
+        decode(tile, pl, x, q);
         const size_t base = pl * plane_elems;
         float2 v[16];
         // C1 applies the input sign (-1)^(x + y), y = N2 n1 + n2 with N2 even: (-1)^(x + n2) for the whole thread.
@@ -475,9 +492,11 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
                         f2mul(u[r], twiddle<SG>(q * k, A.log_gy));
                 else
                 {
+                    // Only the rows the row pass reads ([ry0, ry0 + nry), all rows when dense).
                     const int row = q + A.N1 * k;
-                    A.scratch[base + static_cast<size_t>(row) * G + x] =
-                        (A.centred && (row & 1)) ? make_float2(-u[r].x, -u[r].y) : u[r];
+                    if (static_cast<unsigned>(row - A.ry0) < static_cast<unsigned>(A.nry))
+                        A.scratch[base + static_cast<size_t>(row) * G + x] =
+                            (A.centred && (row & 1)) ? make_float2(-u[r].x, -u[r].y) : u[r];
                 }
             }
         }
@@ -528,7 +547,7 @@ template <int L, int SG> hipError_t launch_rows(const FftArgs &A, hipStream_t st
 {
     int blocks = 0;
     const hipError_t e =
-        resident_blocks(reinterpret_cast<const void *>(&fft_row_kernel<L, SG>), L / 16, A.nplanes * A.gy, blocks);
+        resident_blocks(reinterpret_cast<const void *>(&fft_row_kernel<L, SG>), L / 16, A.nplanes * A.nry, blocks);
     if (e != hipSuccess)
         return e;
     fft_row_kernel<L, SG><<<blocks, L / 16, 0, st>>>(A);
@@ -593,9 +612,25 @@ hipError_t fft_scratch(long long elements, float2 *&out)
 // group 0.57 ms per direction, two planes 0.64 ms).
 constexpr long long kScratchBytes = 64LL << 20;
 
+// Restriction of a transform to a box of cells [x0, x1) x [y0, y1) of every plane.  kSparseInput: the input is
+// zero outside the box, so the column passes take only the column tiles that meet [x0, x1) and the row pass reads
+// only those columns (zeros elsewhere); the whole output is produced.  kRowsOut: only rows [y0, y1) of the output
+// are produced (column pass 2 stores and the row pass transforms only those rows).
+enum BoxMode
+{
+    kFullTransform = 0,
+    kSparseInput = 1,
+    kRowsOut = 2
+};
+struct FftBox
+{
+    int mode = kFullTransform;
+    int x0 = 0, x1 = 0, y0 = 0, y1 = 0;
+};
+
 template <int SG>
 hipError_t stockham_fft2d(const float2 *src, float2 *dst, int gx, int gy, long long planes, bool centred,
-                          hipStream_t st)
+                          hipStream_t st, const FftBox &box = FftBox())
 {
     FftArgs A;
     const long long plane = static_cast<long long>(gx) * gy;
@@ -612,16 +647,38 @@ hipError_t stockham_fft2d(const float2 *src, float2 *dst, int gx, int gy, long l
     A.N2 = 1 << A.log_n2;
     A.log_tiles = ilog2(gx / kColTile);
     A.centred = centred ? 1 : 0;
-    const long long tiles = gx / kColTile;
+    A.ct0 = 0;
+    A.nct = gx / kColTile;
+    A.cx0 = 0;
+    A.cx1 = gx;
+    A.ry0 = 0;
+    A.nry = gy;
+    if (box.mode == kSparseInput)
+    {
+        const int t0 = box.x0 / kColTile, t1 = (box.x1 + kColTile - 1) / kColTile;
+        A.ct0 = t0;
+        A.nct = std::max(0, t1 - t0);
+        A.cx0 = t0 * kColTile;
+        A.cx1 = std::max(A.cx0, t1 * kColTile);
+    }
+    else if (box.mode == kRowsOut)
+    {
+        A.ry0 = box.y0;
+        A.nry = std::max(0, box.y1 - box.y0);
+    }
     for (long long p = 0; p < planes && e == hipSuccess; p += group)
     {
         A.nplanes = static_cast<int>(std::min(group, planes - p));
         A.src = src + static_cast<size_t>(p) * plane;
         A.grid = dst + static_cast<size_t>(p) * plane;
-        e = launch_col_pass<kPassCol1, SG>(A, A.N1, static_cast<int>(A.nplanes * tiles * A.N2), st);
-        if (e == hipSuccess)
-            e = launch_col_pass<kPassCol2, SG>(A, A.N2, static_cast<int>(A.nplanes * tiles * A.N1), st);
-        if (e == hipSuccess)
+        const long long col_sets = static_cast<long long>(A.nplanes) * A.nct;
+        if (col_sets > 0)
+        {
+            e = launch_col_pass<kPassCol1, SG>(A, A.N1, static_cast<int>(col_sets * A.N2), st);
+            if (e == hipSuccess)
+                e = launch_col_pass<kPassCol2, SG>(A, A.N2, static_cast<int>(col_sets * A.N1), st);
+        }
+        if (e == hipSuccess && A.nry > 0)
             e = launch_row_pass<SG>(A, st);
     }
     return e;
@@ -671,14 +728,59 @@ bool grid_fft_supported(int G)
     return G >= 4 && G % 4 == 0;
 }
 
-hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward, hipStream_t st)
+// Zeroes columns [c0, c0 + width) of every row of every plane (width even: two cells per 16-byte store).
+__global__ __launch_bounds__(kThreads) void clear_band_kernel(float4 *grid, long long rows, int G, int c0, int width)
+{
+    const long long half = width / 2;
+    const long long i = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= rows * half)
+        return;
+    const long long r = i / half, c = c0 / 2 + (i - r * half);
+    grid[r * (G / 2) + c] = make_float4(0.F, 0.F, 0.F, 0.F);
+}
+
+hipError_t launch_grid_clear_box(void *grid, int G, long long nr_layers, const int *box, hipStream_t st)
+{
+    const long long planes = nr_layers * kCorr;
+    int c0 = 0, c1 = G;
+    if (stockham_size(G))
+    {
+        // The columns the boxed BACKWARD transform reads: whole column tiles around [x0, x1).
+        c0 = box[0] / kColTile * kColTile;
+        c1 = std::max(c0, (box[1] + kColTile - 1) / kColTile * kColTile);
+    }
+    const long long items = planes * G * ((c1 - c0) / 2);
+    if (items == 0)
+        return hipSuccess;
+    const long long blocks = (items + kThreads - 1) / kThreads;
+    if (blocks > 0x7fffffffLL)
+        return hipErrorInvalidConfiguration;
+    clear_band_kernel<<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(static_cast<float4 *>(grid), planes * G, G,
+                                                                          c0, c1 - c0);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward, hipStream_t st, const int *box)
 {
     const long long planes = nr_layers * kCorr;
     if (planes > 0x7fffffffLL || static_cast<long long>(G) * G > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
     float2 *g = static_cast<float2 *>(grid);
     if (stockham_size(G))
-        return forward ? stockham_fft2d<1>(g, g, G, G, planes, true, st) : stockham_fft2d<-1>(g, g, G, G, planes, true, st);
+    {
+        // With a box: the image direction needs only the box's rows; the uv grid is zero outside the box.
+        FftBox b;
+        if (box)
+        {
+            b.mode = forward ? kRowsOut : kSparseInput;
+            b.x0 = box[0];
+            b.x1 = box[1];
+            b.y0 = box[2];
+            b.y1 = box[3];
+        }
+        return forward ? stockham_fft2d<1>(g, g, G, G, planes, true, st, b)
+                       : stockham_fft2d<-1>(g, g, G, G, planes, true, st, b);
+    }
     hipfftHandle plan;
     hipError_t e = fft_plan(G, G, planes, HIPFFT_C2C, plan);
     if (e != hipSuccess)

@@ -123,7 +123,13 @@ hipError_t launch_plan(const idgx_planner &pl, const float *uvw, const double *f
                        long long capacity, long long *h_count, hipStream_t st);
 
 bool grid_fft_supported(int G);
-hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward, hipStream_t st);
+// box: nullptr, or {x0, x1, y0, y1} (0 <= x0 <= x1 <= G, same for y): BACKWARD takes the grid as zero outside the
+// box, FORWARD produces only rows [y0, y1).  Sizes outside the three-pass kernels' range ignore it.
+hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward, hipStream_t st,
+                           const int *box = nullptr);
+// Zeroes the cells a boxed BACKWARD transform reads (whole 64-column tiles around [x0, x1), every row; the whole
+// grid for sizes outside the three-pass kernels' range).
+hipError_t launch_grid_clear_box(void *grid, int G, long long nr_layers, const int *box, hipStream_t st);
 bool fft2d_supported(int nx, int ny);
 // Plain 2-D DFT of `batch` planes of ny rows of nx points, out = sum in exp(sign 2 pi i (...)), unnormalised;
 // in == out is in place.

@@ -156,6 +156,10 @@ def lib():
                                                     ctypes.POINTER(AdderPlan), vp, vp]
         L.idgx_split_subgrids_layered.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
         L.idgx_grid_fft.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp]
+        L.idgx_grid_fft_box.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int32), vp]
+        L.idgx_grid_box.argtypes = [ctypes.POINTER(Plan), vp, ctypes.POINTER(ctypes.c_int32)]
+        L.idgx_grid_clear_box.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int32), vp]
         L.idgx_fft2d.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int, vp]
         L.idgx_plan_workspace_bytes.argtypes = [ctypes.c_int64]
         L.idgx_plan_workspace_bytes.restype = ctypes.c_size_t
@@ -359,6 +363,29 @@ def grid_fft(plan, d_grid, nr_layers, direction, stream=None):
     """idgx_grid_fft: in-place centred 2-D transform of a master grid complex64 [nr_layers][4][G][G]."""
     _check(lib().idgx_grid_fft(ctypes.byref(plan), _dptr(d_grid), int(nr_layers), int(direction), _stream(stream)),
            "idgx_grid_fft")
+
+
+def grid_box(plan, metadata):
+    """idgx_grid_box: (x0, x1, y0, y1), the bounding box of the plan's subgrids on the master grid."""
+    assert metadata.dtype == METADATA_DTYPE and metadata.flags.c_contiguous
+    box = (ctypes.c_int32 * 4)()
+    _check(lib().idgx_grid_box(ctypes.byref(plan), _hptr(metadata), box), "idgx_grid_box")
+    return tuple(box)
+
+
+def grid_fft_box(plan, d_grid, nr_layers, direction, box, stream=None):
+    """idgx_grid_fft_box: the master-grid transform restricted to box = (x0, x1, y0, y1): BACKWARD takes the grid
+    as zero outside the box, FORWARD produces only rows y0 .. y1 - 1."""
+    b = (ctypes.c_int32 * 4)(*[int(v) for v in box])
+    _check(lib().idgx_grid_fft_box(ctypes.byref(plan), _dptr(d_grid), int(nr_layers), int(direction), b,
+                                   _stream(stream)), "idgx_grid_fft_box")
+
+
+def grid_clear_box(plan, d_grid, nr_layers, box, stream=None):
+    """idgx_grid_clear_box: zero the cells a boxed BACKWARD transform reads."""
+    b = (ctypes.c_int32 * 4)(*[int(v) for v in box])
+    _check(lib().idgx_grid_clear_box(ctypes.byref(plan), _dptr(d_grid), int(nr_layers), b, _stream(stream)),
+           "idgx_grid_clear_box")
 
 
 FFT_FP32 = 0

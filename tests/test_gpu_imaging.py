@@ -7,7 +7,8 @@ import pytest
 
 import oracle
 from helpers import rel_l2
-from idgx import FFT_BACKWARD, FFT_FORWARD, METADATA_DTYPE, CHANNELS_DTYPE, grid_fft, observation
+from idgx import FFT_BACKWARD, FFT_FORWARD, METADATA_DTYPE, CHANNELS_DTYPE, grid_box, grid_clear_box, grid_fft, \
+    grid_fft_box, observation
 from idgx.device import DeviceWorkload, download, upload
 
 pytestmark = pytest.mark.gpu
@@ -65,6 +66,82 @@ def _fft_plan(G):
     md["nr_timesteps"] = 1
     md["station2"] = 1
     return DeviceWorkload(_grid_workload(G, md)).plan()
+
+
+@pytest.mark.parametrize("G", [2048, 1028])
+def test_boxed_grid_fft_equals_the_full_transform(G):
+    """idgx_grid_fft_box: BACKWARD on a grid that is zero outside a box (not aligned to the 64-column tiles) gives
+    the full transform's image bit for bit; FORWARD gives the full transform's rows inside the box bit for bit.
+    The shared scratch holds an unrelated transform's data first, so a read of a column the box leaves out would
+    show.  G = 1028 runs the rocFFT path, which ignores the box.  Whole-grid and empty boxes too."""
+    rng = np.random.default_rng(7)
+    plan = _fft_plan(G)
+    shape = (2, 4, G, G)
+    junk = upload((rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64))
+    grid_fft(plan, junk, 2, FFT_BACKWARD)  # leaves unrelated data in the scratch
+    box = (300, 701, 1000, 1403) if G == 2048 else (10, 90, 500, 530)
+    x = np.zeros(shape, np.complex64)
+    x[..., box[2]:box[3], box[0]:box[1]] = (rng.standard_normal((2, 4, box[3] - box[2], box[1] - box[0])) +
+                                            1j * rng.standard_normal((2, 4, box[3] - box[2], box[1] - box[0])))
+    d_box, d_full = upload(x), upload(x)
+    grid_fft_box(plan, d_box, 2, FFT_BACKWARD, box)
+    grid_fft(plan, d_full, 2, FFT_BACKWARD)
+    img = download(d_full, np.complex64, shape)
+    assert download(d_box, np.complex64, shape).tobytes() == img.tobytes()
+    assert rel_l2(img[1, 2], _centred_np(x[1, 2].astype(np.complex128), -1)) < 1e-6
+    # FORWARD from the dense image: the rows of the box match, the rest is not produced.
+    d_box, d_full = upload(img), upload(img)
+    grid_fft_box(plan, d_box, 2, FFT_FORWARD, box)
+    grid_fft(plan, d_full, 2, FFT_FORWARD)
+    rows = slice(box[2], box[3])
+    got, full = download(d_box, np.complex64, shape), download(d_full, np.complex64, shape)
+    assert got[..., rows, :].tobytes() == full[..., rows, :].tobytes()
+    assert rel_l2(full[0, 0][rows, box[0]:box[1]] / np.float32(G * G), x[0, 0][rows, box[0]:box[1]]) < 1e-6
+    # Whole-grid box: the full transform; empty box backward: zeros in, zeros out.
+    d_box, d_full = upload(x), upload(x)
+    grid_fft_box(plan, d_box, 2, FFT_BACKWARD, (0, G, 0, G))
+    grid_fft(plan, d_full, 2, FFT_BACKWARD)
+    assert download(d_box, np.complex64, shape).tobytes() == download(d_full, np.complex64, shape).tobytes()
+    d0 = upload(np.zeros(shape, np.complex64))
+    grid_fft_box(plan, d0, 2, FFT_BACKWARD, (0, 0, 0, 0))
+    assert not download(d0, np.complex64, shape).any()
+
+
+def test_boxed_imaging_cycle_equals_the_full_cycle():
+    """Two imaging cycles in a row (clear, add, master FFT to the image and back, split) on the subgrids' bounding
+    box: idgx_grid_clear_box + idgx_grid_fft_box give the split subgrids of the full clear and full transforms bit
+    for bit, the second cycle starting from the grid the boxed FORWARD transform left undefined outside the box."""
+    import torch
+
+    w = observation.make_observation("small")
+    dw = DeviceWorkload(w)
+    plan = dw.plan(precision="fp16x3")
+    dw.grid(plan)
+    dw.fft(plan, FFT_FORWARD, dw.d_subgrids)
+    d_uv = dw.d_subgrids.clone()
+    box = grid_box(plan, w["metadata"])
+    G = int(w["grid_size"])
+    assert 0 < box[1] - box[0] < G and 0 < box[3] - box[2] < G
+
+    def cycle(boxed, d_grid):
+        if boxed:
+            grid_clear_box(plan, d_grid, 1, box)
+        else:
+            d_grid.zero_()
+        dw.add(plan, d_grid, d_uv)
+        for direction in (FFT_BACKWARD, FFT_FORWARD):
+            if boxed:
+                grid_fft_box(plan, d_grid, 1, direction, box)
+            else:
+                grid_fft(plan, d_grid, 1, direction)
+        out = torch.empty_like(d_uv)
+        dw.split(plan, d_grid, out)
+        return out.cpu().numpy().tobytes()
+
+    d_full, d_box = dw.grid_buffer(), dw.grid_buffer()
+    d_box.fill_(7)  # not zero anywhere: the boxed clear must zero what the transform to the image reads
+    for r in range(2):
+        assert cycle(True, d_box) == cycle(False, d_full), f"cycle {r}"
 
 
 @pytest.mark.parametrize("G", [1024, 2048, 4096, 8192])

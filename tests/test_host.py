@@ -110,6 +110,20 @@ def test_host_buffer_calls_validate_on_the_host_before_touching_the_device():
     assert L.idgx_host_degrid(ctypes.byref(p), *args, 1, None) == idgx.ERR_INVALID_METADATA
 
 
+def test_grid_box_bounds_every_subgrid_clamped_to_the_grid():
+    """idgx_grid_box (host only): the bounding box of the subgrids' cells, clamped to [0, G)."""
+    w = observation.make_observation("small", visibilities=False)
+    p = _plan(w)
+    md = w["metadata"].copy()
+    S, G = w["subgrid_size"], w["grid_size"]
+    box = idgx.grid_box(p, md)
+    assert box == (md["x"].min(), md["x"].max() + S, md["y"].min(), md["y"].max() + S)
+    md["x"][0], md["y"][1] = -5, G - 3  # overhanging subgrids: the box stops at the grid's edges
+    box = idgx.grid_box(p, md)
+    assert box[0] == 0 and box[3] == G
+    assert idgx.grid_box(_plan(w, nr_subgrids=0), md[:0].copy()) == (0, 0, 0, 0)
+
+
 def test_planner_covers_observation_and_keeps_subgrids_on_grid():
     w = observation.make_observation("ska_low", visibilities=False)
     md, ch = w["metadata"], w["channels"]
