@@ -1383,7 +1383,8 @@ hipError_t launch_degrid_prepare(const KernelArgs &a, int precision, hipStream_t
     const dim3 grid(static_cast<unsigned>(a.p.nr_subgrids));
     // FP16 needs the subgrid's maximum before the split: hold up to 9 slots per thread for S <= 48 (BF16 has no
     // scale and streams the slots in one pass either way).
-    const bool wide = a.p.S * degrid_row_stride(a.p.S) > 4 * kThreads;
+    const int nslots = a.p.S * degrid_row_stride(a.p.S);
+    const bool wide = nslots > 4 * kThreads && nslots <= 9 * kThreads;
     switch (precision)
     {
     case IDGX_MIXED_BF16X3:
