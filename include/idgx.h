@@ -241,8 +241,9 @@ int idgx_grid_box(const idgx_plan *plan, const idgx_metadata *h_metadata, int32_
  *   BACKWARD (uv -> image): the grid must be zero outside the box.  The column transforms run only over the
  *     64-column tiles that meet [x0, x1), and the row transforms read only those columns.  The image is complete
  *     and equals idgx_grid_fft's bit for bit.
- *   FORWARD (image -> uv): only rows [y0, y1) of the uv grid are produced, equal to idgx_grid_fft's bit for bit;
- *     the other rows are left undefined.
+ *   FORWARD (image -> uv): only the box's cells of the uv grid are produced (with the rest of the 64-column tiles
+ *     around it), equal to idgx_grid_fft's to FP32 rounding: the row transforms run first, dense and in place,
+ *     then the column transforms only over the box's column tiles.  The other cells are left undefined.
  * Grid sizes outside the three-pass kernels' range (see idgx_grid_fft) run the full transform.  No counterpart in
  * the reference (its benchmark stops at the subgrids); it is the sparse master-grid transform of an imaging cycle
  * whose uv coverage is smaller than the grid. */

@@ -220,6 +220,7 @@ struct FftArgs
     int ct0, nct;      // column passes: column tiles [ct0, ct0 + nct) of every row set (all gx / kColTile: dense)
     int cx0, cx1;      // row pass: scratch columns outside [cx0, cx1) are read as zero (0, gx: dense)
     int ry0, nry;      // row pass: rows [ry0, ry0 + nry) of each plane (0, gy: dense)
+    int rows_first;    // 1: the row pass runs first, in place (scratch == grid), and takes the input signs
 };
 
 constexpr int ilog2(int n)
@@ -278,8 +279,12 @@ template <int L, int SG> __global__ __launch_bounds__(L / 16) void fft_row_kerne
     // plane * gy + row.  Scratch columns outside [cx0, cx1) are taken as zero without being read.
     const int rows = A.nplanes * A.nry;
     auto global_row = [&](int i) { return (i / A.nry) * A.gy + A.ry0 + i % A.nry; };
+    const bool sign_in = flip && A.rows_first; // (-1)^(x + y) of the centred transform's input
     auto load = [&](int gr, int col) {
-        return col >= A.cx0 && col < A.cx1 ? A.scratch[static_cast<size_t>(gr) * L + col] : make_float2(0.F, 0.F);
+        float2 v = col >= A.cx0 && col < A.cx1 ? A.scratch[static_cast<size_t>(gr) * L + col] : make_float2(0.F, 0.F);
+        if (sign_in && ((col + gr) & 1)) // gy is even: the row's parity within its plane is that of gr
+            v = make_float2(-v.x, -v.y);
+        return v;
     };
     float2 nxt[16];
     int i = blockIdx.x;
@@ -440,7 +445,7 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
         const size_t base = pl * plane_elems;
         float2 v[16];
         // C1 applies the input sign (-1)^(x + y), y = N2 n1 + n2 with N2 even: (-1)^(x + n2) for the whole thread.
-        const float sg_in = PASS == kPassCol1 && A.centred && ((x + q) & 1) ? -1.F : 1.F;
+        const float sg_in = PASS == kPassCol1 && A.centred && !A.rows_first && ((x + q) & 1) ? -1.F : 1.F;
 #pragma unroll
         for (int r = 0; r < 16; ++r)
             v[r] = make_float2(sg_in * nxt[r].x, sg_in * nxt[r].y);
@@ -612,10 +617,26 @@ hipError_t fft_scratch(long long elements, float2 *&out)
 // group 0.57 ms per direction, two planes 0.64 ms).
 constexpr long long kScratchBytes = 64LL << 20;
 
+// Copies rows [y0, y0 + ny) x columns [c0, c0 + nc) of every plane from src to dst (planes of G x G).
+__global__ __launch_bounds__(kThreads) void copy_box_kernel(const float2 *src, float2 *dst, int G, int y0, int ny, int c0,
+                                                            int nc, long long total)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= total)
+        return;
+    const long long r = i / nc; // plane * ny + row
+    const int c = c0 + static_cast<int>(i - r * nc);
+    const long long pl = r / ny;
+    const size_t e = (static_cast<size_t>(pl) * G + y0 + (r - pl * ny)) * G + c;
+    dst[e] = src[e];
+}
+
 // Restriction of a transform to a box of cells [x0, x1) x [y0, y1) of every plane.  kSparseInput: the input is
 // zero outside the box, so the column passes take only the column tiles that meet [x0, x1) and the row pass reads
-// only those columns (zeros elsewhere); the whole output is produced.  kRowsOut: only rows [y0, y1) of the output
-// are produced (column pass 2 stores and the row pass transforms only those rows).
+// only those columns (zeros elsewhere); the whole output is produced.  kRowsOut: only the box's cells (within the
+// column tiles around it) of the output are produced: the row pass runs first, in place and dense, then the column
+// passes take only the box's column tiles and column pass 2 stores only its rows into the scratch, from which they
+// are copied back (one dense pass instead of two; the rounding differs from the column-first order).
 enum BoxMode
 {
     kFullTransform = 0,
@@ -661,10 +682,50 @@ hipError_t stockham_fft2d(const float2 *src, float2 *dst, int gx, int gy, long l
         A.cx0 = t0 * kColTile;
         A.cx1 = std::max(A.cx0, t1 * kColTile);
     }
-    else if (box.mode == kRowsOut)
+    A.rows_first = 0;
+    if (box.mode == kRowsOut)
     {
-        A.ry0 = box.y0;
-        A.nry = std::max(0, box.y1 - box.y0);
+        const int t0 = box.x0 / kColTile, t1 = (box.x1 + kColTile - 1) / kColTile;
+        const int ny = std::max(0, box.y1 - box.y0);
+        if (src != dst)
+            return hipErrorInvalidValue; // the grid transforms run in place
+        if (t1 <= t0 || ny == 0)
+            return hipSuccess; // nothing to produce
+        for (long long p = 0; p < planes && e == hipSuccess; p += group)
+        {
+            A.nplanes = static_cast<int>(std::min(group, planes - p));
+            float2 *g = dst + static_cast<size_t>(p) * plane;
+            // 1. Rows, in place and dense, with the input signs.
+            FftArgs R = A;
+            R.scratch = g;
+            R.grid = g;
+            R.rows_first = 1;
+            e = launch_row_pass<SG>(R, st);
+            // 2. Columns of the box's column tiles; column pass 2 stores the box's rows into the scratch.
+            FftArgs C = A;
+            C.src = g;
+            C.grid = g;
+            C.rows_first = 1;
+            C.ct0 = t0;
+            C.nct = t1 - t0;
+            C.ry0 = box.y0;
+            C.nry = ny;
+            const long long col_sets = static_cast<long long>(C.nplanes) * C.nct;
+            if (e == hipSuccess)
+                e = launch_col_pass<kPassCol1, SG>(C, C.N1, static_cast<int>(col_sets * C.N2), st);
+            if (e == hipSuccess)
+                e = launch_col_pass<kPassCol2, SG>(C, C.N2, static_cast<int>(col_sets * C.N1), st);
+            // 3. The box back into the grid.
+            const int nc = (t1 - t0) * kColTile;
+            const long long total = static_cast<long long>(C.nplanes) * ny * nc;
+            if (e == hipSuccess)
+            {
+                copy_box_kernel<<<static_cast<unsigned>((total + kThreads - 1) / kThreads), kThreads, 0, st>>>(
+                    C.scratch, g, gx, box.y0, ny, t0 * kColTile, nc, total);
+                e = hipGetLastError();
+            }
+        }
+        return e;
     }
     for (long long p = 0; p < planes && e == hipSuccess; p += group)
     {

@@ -124,7 +124,7 @@ hipError_t launch_plan(const idgx_planner &pl, const float *uvw, const double *f
 
 bool grid_fft_supported(int G);
 // box: nullptr, or {x0, x1, y0, y1} (0 <= x0 <= x1 <= G, same for y): BACKWARD takes the grid as zero outside the
-// box, FORWARD produces only rows [y0, y1).  Sizes outside the three-pass kernels' range ignore it.
+// box, FORWARD produces only the box's cells.  Sizes outside the three-pass kernels' range ignore it.
 hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward, hipStream_t st,
                            const int *box = nullptr);
 // Zeroes the cells a boxed BACKWARD transform reads (whole 64-column tiles around [x0, x1), every row; the whole

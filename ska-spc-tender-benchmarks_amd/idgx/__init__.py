@@ -375,7 +375,7 @@ def grid_box(plan, metadata):
 
 def grid_fft_box(plan, d_grid, nr_layers, direction, box, stream=None):
     """idgx_grid_fft_box: the master-grid transform restricted to box = (x0, x1, y0, y1): BACKWARD takes the grid
-    as zero outside the box, FORWARD produces only rows y0 .. y1 - 1."""
+    as zero outside the box, FORWARD produces only the box's cells."""
     b = (ctypes.c_int32 * 4)(*[int(v) for v in box])
     _check(lib().idgx_grid_fft_box(ctypes.byref(plan), _dptr(d_grid), int(nr_layers), int(direction), b,
                                    _stream(stream)), "idgx_grid_fft_box")

@@ -71,7 +71,8 @@ def _fft_plan(G):
 @pytest.mark.parametrize("G", [2048, 1028])
 def test_boxed_grid_fft_equals_the_full_transform(G):
     """idgx_grid_fft_box: BACKWARD on a grid that is zero outside a box (not aligned to the 64-column tiles) gives
-    the full transform's image bit for bit; FORWARD gives the full transform's rows inside the box bit for bit.
+    the full transform's image bit for bit; FORWARD gives the full transform's cells inside the box to FP32
+    rounding (its row transforms run first).
     The shared scratch holds an unrelated transform's data first, so a read of a column the box leaves out would
     show.  G = 1028 runs the rocFFT path, which ignores the box.  Whole-grid and empty boxes too."""
     rng = np.random.default_rng(7)
@@ -89,14 +90,16 @@ def test_boxed_grid_fft_equals_the_full_transform(G):
     img = download(d_full, np.complex64, shape)
     assert download(d_box, np.complex64, shape).tobytes() == img.tobytes()
     assert rel_l2(img[1, 2], _centred_np(x[1, 2].astype(np.complex128), -1)) < 1e-6
-    # FORWARD from the dense image: the rows of the box match, the rest is not produced.
+    # FORWARD from the dense image: the box's cells match, the rest is not produced.
     d_box, d_full = upload(img), upload(img)
     grid_fft_box(plan, d_box, 2, FFT_FORWARD, box)
     grid_fft(plan, d_full, 2, FFT_FORWARD)
-    rows = slice(box[2], box[3])
+    rows, cols = slice(box[2], box[3]), slice(box[0], box[1])
     got, full = download(d_box, np.complex64, shape), download(d_full, np.complex64, shape)
-    assert got[..., rows, :].tobytes() == full[..., rows, :].tobytes()
-    assert rel_l2(full[0, 0][rows, box[0]:box[1]] / np.float32(G * G), x[0, 0][rows, box[0]:box[1]]) < 1e-6
+    e_box = rel_l2(got[..., rows, cols], full[..., rows, cols])
+    print(f"G={G} boxed FORWARD vs the full transform on the box: {e_box:.2e}")
+    assert e_box < 1e-6
+    assert rel_l2(got[0, 0][rows, cols] / np.float32(G * G), x[0, 0][rows, cols]) < 1e-6
     # Whole-grid box: the full transform; empty box backward: zeros in, zeros out.
     d_box, d_full = upload(x), upload(x)
     grid_fft_box(plan, d_box, 2, FFT_BACKWARD, (0, G, 0, G))
@@ -109,8 +112,9 @@ def test_boxed_grid_fft_equals_the_full_transform(G):
 
 def test_boxed_imaging_cycle_equals_the_full_cycle():
     """Two imaging cycles in a row (clear, add, master FFT to the image and back, split) on the subgrids' bounding
-    box: idgx_grid_clear_box + idgx_grid_fft_box give the split subgrids of the full clear and full transforms bit
-    for bit, the second cycle starting from the grid the boxed FORWARD transform left undefined outside the box."""
+    box: idgx_grid_clear_box + idgx_grid_fft_box give the split subgrids of the full clear and full transforms to
+    FP32 rounding, the second cycle starting from the grid the boxed FORWARD transform left undefined outside the
+    box."""
     import torch
 
     w = observation.make_observation("small")
@@ -136,12 +140,14 @@ def test_boxed_imaging_cycle_equals_the_full_cycle():
                 grid_fft(plan, d_grid, 1, direction)
         out = torch.empty_like(d_uv)
         dw.split(plan, d_grid, out)
-        return out.cpu().numpy().tobytes()
+        return out.cpu().numpy().view(np.complex64)
 
     d_full, d_box = dw.grid_buffer(), dw.grid_buffer()
     d_box.fill_(7)  # not zero anywhere: the boxed clear must zero what the transform to the image reads
     for r in range(2):
-        assert cycle(True, d_box) == cycle(False, d_full), f"cycle {r}"
+        e = rel_l2(cycle(True, d_box), cycle(False, d_full))
+        print(f"boxed cycle {r} vs the full cycle: {e:.2e}")
+        assert e < 1e-6, f"cycle {r}"
 
 
 @pytest.mark.parametrize("G", [1024, 2048, 4096, 8192])
