@@ -124,6 +124,24 @@ def test_grid_box_bounds_every_subgrid_clamped_to_the_grid():
     assert idgx.grid_box(_plan(w, nr_subgrids=0), md[:0].copy()) == (0, 0, 0, 0)
 
 
+def test_boxed_grid_calls_reject_bad_boxes_without_touching_the_device():
+    """idgx_grid_fft_box / idgx_grid_clear_box validate the box on the host (this runs without a GPU)."""
+    w = observation.make_observation("small", visibilities=False)
+    p = _plan(w)
+    G = w["grid_size"]
+    L = idgx.lib()
+    fake = ctypes.c_void_p(1)  # never dereferenced: every call below fails its checks first
+    for box in [(-1, 10, 0, 10), (10, 5, 0, 10), (0, G + 1, 0, 10), (0, 10, 20, 10), (0, 10, 0, G + 1)]:
+        b = (ctypes.c_int32 * 4)(*box)
+        assert L.idgx_grid_fft_box(ctypes.byref(p), fake, 1, idgx.FFT_FORWARD, b, None) == idgx.ERR_INVALID_ARGUMENT
+        assert L.idgx_grid_clear_box(ctypes.byref(p), fake, 1, b, None) == idgx.ERR_INVALID_ARGUMENT
+    good = (ctypes.c_int32 * 4)(0, 10, 0, 10)
+    assert L.idgx_grid_fft_box(ctypes.byref(p), fake, 0, idgx.FFT_FORWARD, good, None) == idgx.ERR_INVALID_ARGUMENT
+    assert L.idgx_grid_fft_box(ctypes.byref(p), fake, 1, 7, good, None) == idgx.ERR_INVALID_ARGUMENT
+    assert L.idgx_grid_fft_box(ctypes.byref(p), None, 1, idgx.FFT_FORWARD, good, None) == idgx.ERR_INVALID_ARGUMENT
+    assert L.idgx_grid_fft_box(ctypes.byref(p), fake, 1, idgx.FFT_FORWARD, None, None) == idgx.ERR_INVALID_ARGUMENT
+
+
 def test_planner_covers_observation_and_keeps_subgrids_on_grid():
     w = observation.make_observation("ska_low", visibilities=False)
     md, ch = w["metadata"], w["channels"]
