@@ -9,7 +9,7 @@
 //   every lane reads the same LDS address per visibility (broadcast, conflict-free).  Taper and A-term are
 //   applied in the epilogue, which overwrites the subgrid.
 // Degridder: one workgroup owns 256 (timestep, channel-chunk) items of one subgrid, each thread one timestep x
-//   up to 4 channels.  Tapered, A-term-corrected pixels plus (l, m, n, phase offset) are staged through LDS in
+//   up to 2 channels.  Tapered, A-term-corrected pixels plus (l, m, n, phase offset) are staged through LDS in
 //   batches of 512 pixels; each thread sums its visibilities over all pixels in registers, so no cross-lane
 //   reduction is needed.
 #include "idgx_device.hpp"
@@ -62,7 +62,11 @@ __device__ __forceinline__ void channel_window(const KernelArgs &a, int s, int &
     }
 }
 
-template <int NPIX, int SC> __global__ __launch_bounds__(kThreads) void gridder_plain(KernelArgs a)
+// Five workgroups per CU (five waves per SIMD) with the hardware sin/cos.  The kernel needs 89 VGPRs with or
+// without the bound, yet the bounded build measured 3 % faster on the SKA-Low-like workload (the compiler
+// schedules for the tighter budget).
+template <int NPIX, int SC>
+__global__ __launch_bounds__(kThreads, SC == kSincosHw ? 5 : 1) void gridder_plain(KernelArgs a)
 {
     __shared__ float4 s_vis[2 * kGridBatchVis];
     __shared__ float4 s_uvw[kGridBatchVis];
@@ -179,7 +183,11 @@ template <int NPIX, int SC> __global__ __launch_bounds__(kThreads) void gridder_
     }
 }
 
-template <int NCH, int SC> __global__ __launch_bounds__(kThreads) void degridder_plain(KernelArgs a, int per)
+// Six workgroups per CU with the hardware sin/cos, the most its 24 KB of LDS allows.  Two channels per thread fit
+// that in 79 VGPRs without spills; four channels per thread needed 99 VGPRs (four waves per SIMD) and ran 10 %
+// slower.
+template <int NCH, int SC>
+__global__ __launch_bounds__(kThreads, SC == kSincosHw ? 6 : 1) void degridder_plain(KernelArgs a, int per)
 {
     __shared__ float4 s_pix[2 * kDegridPixBatch];
     __shared__ float4 s_geo[kDegridPixBatch]; // l, m, n, phase offset
@@ -305,7 +313,7 @@ template <int NCH, int SC> __global__ __launch_bounds__(kThreads) void degridder
 } // namespace
 
 constexpr int kGridPix = 4;
-constexpr int kDegridCh = 4;
+constexpr int kDegridCh = 2;
 
 hipError_t launch_grid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream)
 {
