@@ -243,7 +243,9 @@ int idgx_grid_box(const idgx_plan *plan, const idgx_metadata *h_metadata, int32_
  *     and equals idgx_grid_fft's bit for bit.
  *   FORWARD (image -> uv): only the box's cells of the uv grid are produced (with the rest of the 64-column tiles
  *     around it), equal to idgx_grid_fft's to FP32 rounding: the row transforms run first, dense and in place,
- *     then the column transforms only over the box's column tiles.  The other cells are left undefined.
+ *     then the column transforms only over the box's column tiles.  The other cells are left undefined.  A box
+ *     whose column tiles and rows cover most of the grid, where that order would move more data than the full
+ *     transform, runs the full transform (every cell produced, bit for bit idgx_grid_fft's).
  * Grid sizes outside the three-pass kernels' range (see idgx_grid_fft) run the full transform.  No counterpart in
  * the reference (its benchmark stops at the subgrids); it is the sparse master-grid transform of an imaging cycle
  * whose uv coverage is smaller than the grid. */

@@ -74,7 +74,8 @@ def test_boxed_grid_fft_equals_the_full_transform(G):
     the full transform's image bit for bit; FORWARD gives the full transform's cells inside the box to FP32
     rounding (its row transforms run first).
     The shared scratch holds an unrelated transform's data first, so a read of a column the box leaves out would
-    show.  G = 1028 runs the rocFFT path, which ignores the box.  Whole-grid and empty boxes too."""
+    show.  G = 1028 runs the rocFFT path, which ignores the box.  Whole-grid and empty boxes too, and a FORWARD box
+    that covers most of the grid (the full transform runs)."""
     rng = np.random.default_rng(7)
     plan = _fft_plan(G)
     shape = (2, 4, G, G)
@@ -108,6 +109,12 @@ def test_boxed_grid_fft_equals_the_full_transform(G):
     d0 = upload(np.zeros(shape, np.complex64))
     grid_fft_box(plan, d0, 2, FFT_BACKWARD, (0, 0, 0, 0))
     assert not download(d0, np.complex64, shape).any()
+    # FORWARD on a box that covers most of the grid runs the full transform (cheaper than rows first): bit for bit.
+    big = (0, G - 1, 40, G - 30)
+    d_box, d_full = upload(img), upload(img)
+    grid_fft_box(plan, d_box, 2, FFT_FORWARD, big)
+    grid_fft(plan, d_full, 2, FFT_FORWARD)
+    assert download(d_box, np.complex64, shape).tobytes() == download(d_full, np.complex64, shape).tobytes()
 
 
 def test_boxed_imaging_cycle_equals_the_full_cycle():
