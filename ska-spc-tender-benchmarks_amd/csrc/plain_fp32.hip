@@ -9,7 +9,7 @@
 //   every lane reads the same LDS address per visibility (broadcast, conflict-free).  Taper and A-term are
 //   applied in the epilogue, which overwrites the subgrid.
 // Degridder: one workgroup owns 256 (timestep, channel-chunk) items of one subgrid, each thread one timestep x
-//   up to 2 channels.  Tapered, A-term-corrected pixels plus (l, m, n, phase offset) are staged through LDS in
+//   up to 2 or 4 channels.  Tapered, A-term-corrected pixels plus (l, m, n, phase offset) are staged through LDS in
 //   batches of 512 pixels; each thread sums its visibilities over all pixels in registers, so no cross-lane
 //   reduction is needed.
 #include "idgx_device.hpp"
@@ -183,11 +183,16 @@ __global__ __launch_bounds__(kThreads, SC == kSincosHw ? 5 : 1) void gridder_pla
     }
 }
 
-// Six workgroups per CU with the hardware sin/cos, the most its 24 KB of LDS allows.  Two channels per thread fit
-// that in 79 VGPRs without spills; four channels per thread needed 99 VGPRs (four waves per SIMD) and ran 10 %
-// slower.
+// Workgroups per CU with the hardware sin/cos: two channels per thread fit six (the most the 24 KB of LDS allows)
+// in 79 VGPRs without spills, four channels per thread fit five in 96 VGPRs.
+template <int NCH, int SC> constexpr int degrid_plain_min_blocks()
+{
+    return SC == kSincosHw ? (NCH <= 2 ? 6 : 5) : 1;
+}
+
 template <int NCH, int SC>
-__global__ __launch_bounds__(kThreads, SC == kSincosHw ? 6 : 1) void degridder_plain(KernelArgs a, int per)
+__global__ __launch_bounds__(kThreads, (degrid_plain_min_blocks<NCH, SC>())) void degridder_plain(KernelArgs a,
+                                                                                                  int per)
 {
     __shared__ float4 s_pix[2 * kDegridPixBatch];
     __shared__ float4 s_geo[kDegridPixBatch]; // l, m, n, phase offset
@@ -313,7 +318,6 @@ __global__ __launch_bounds__(kThreads, SC == kSincosHw ? 6 : 1) void degridder_p
 } // namespace
 
 constexpr int kGridPix = 4;
-constexpr int kDegridCh = 2;
 
 hipError_t launch_grid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream)
 {
@@ -326,18 +330,30 @@ hipError_t launch_grid_plain(const KernelArgs &a, bool accurate_sincos, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_degrid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream)
+template <int NCH> hipError_t launch_degrid_plain_chunks(const KernelArgs &a, bool accurate_sincos, hipStream_t stream)
 {
-    const long long items = static_cast<long long>(a.p.max_nr_timesteps) * ((a.p.nr_channels + kDegridCh - 1) / kDegridCh);
+    const long long items = static_cast<long long>(a.p.max_nr_timesteps) * ((a.p.nr_channels + NCH - 1) / NCH);
     const long long per = items > 0 ? (items + kThreads - 1) / kThreads : 1;
     const long long blocks = static_cast<long long>(a.p.nr_subgrids) * per;
     if (blocks > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
     const dim3 grid(static_cast<unsigned>(blocks));
     if (accurate_sincos)
-        degridder_plain<kDegridCh, kSincosOcml><<<grid, kThreads, 0, stream>>>(a, static_cast<int>(per));
+        degridder_plain<NCH, kSincosOcml><<<grid, kThreads, 0, stream>>>(a, static_cast<int>(per));
     else
-        degridder_plain<kDegridCh, kSincosHw><<<grid, kThreads, 0, stream>>>(a, static_cast<int>(per));
+        degridder_plain<NCH, kSincosHw><<<grid, kThreads, 0, stream>>>(a, static_cast<int>(per));
     return hipGetLastError();
+}
+
+// Channels per degridder thread.  Four make each pixel's LDS reads and index term serve more visibilities, but a
+// chunk that runs past the end of a subgrid's channel window idles its lanes: four channels when every subgrid
+// takes all channels (no windows) and their count is a multiple of 4, two otherwise.  Measured: the reference
+// generator's workload (16 channels, no windows) 17.1 ms with four, 18.3 ms with two; the SKA-Low-like workload
+// (windows of 13.5 channels on average: 90 % of the lanes busy with four, 96.7 % with two) 82.3 and 79.6 ms.
+hipError_t launch_degrid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream)
+{
+    if (!a.channels && a.p.nr_channels % 4 == 0)
+        return launch_degrid_plain_chunks<4>(a, accurate_sincos, stream);
+    return launch_degrid_plain_chunks<2>(a, accurate_sincos, stream);
 }
 } // namespace idgx
