@@ -10,6 +10,7 @@ import pytest
 from helpers import oracle_degrid, oracle_grid, per_subgrid_rel_l2, rel_l2
 from idgx import Batch, host_grid, make_plan, observation, grid, degrid
 from idgx.device import DeviceBatch, DeviceWorkload, upload
+from idgx.workload import reference_workload
 
 pytestmark = pytest.mark.gpu
 
@@ -148,3 +149,30 @@ def test_subgrids_without_visibilities(precision):
         for r0, nt, c0, nc in windows:
             assert not bvis[r0:r0 + nt, c0:c0 + nc].any()
         assert _vis_errors(w, bvis, v_ref).max() <= bar
+
+
+@pytest.mark.parametrize("C", [12, 10])
+def test_plain_degridder_channel_chunks_on_ragged_plans_without_windows(C):
+    """Without channel windows the plain degridder takes 4 channels per thread when the channel count is a multiple
+    of 4 (C = 12) and 2 otherwise (C = 10).  Both on subgrids of 0, 37, 64 and 128 timesteps: every covered
+    visibility meets the bar against the FP32 restatement of the reference CPU path, and the rows of the subgrid
+    without timesteps stay zero."""
+    w = reference_workload(4, 2, 128, C, 32, 1024)
+    md = w["metadata"].copy()
+    md["nr_timesteps"][0] = 0
+    md["nr_timesteps"][1] = 37
+    md["nr_timesteps"][2::3] = 64
+    w = dict(w, metadata=md)
+    rng = np.random.default_rng(C)
+    n = w["nr_subgrids"]
+    sub = (rng.standard_normal((n, 4, 32, 32)) + 1j * rng.standard_normal((n, 4, 32, 32))).astype(np.complex64)
+    dw = DeviceWorkload(w)
+    plan = dw.plan()
+    dw.d_subgrids.copy_(upload(sub))
+    dw.degrid(plan)
+    vis = dw.visibilities_out()
+    errs = _vis_errors(w, vis, oracle_degrid(w, sub))
+    print(f"C={C}: {errs.size} subgrids with visibilities, worst rel-L2 {errs.max():.2e}")
+    r0 = int(md["time_offset"][0])
+    assert not vis[r0:r0 + 128].any()
+    assert errs.max() <= PARITY
