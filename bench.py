@@ -180,6 +180,38 @@ class OneShot:
         return self.dw.d_subgrids, self.dw.d_vis_out
 
 
+class PlainPerClass:
+    """Plain FP32 over a batched workload: the one-shot kernels once per subgrid class (idgx_grid / idgx_degrid) on
+    the batch's buffers.  The plain kernels take no selection lists, so every sample of each subgrid's window is
+    processed, the flagged ones as zeros, as in the reference CPU path."""
+
+    def __init__(self, db):
+        from idgx import PLAIN_FP32
+
+        self.db, self.precision, self.path = db, "fp32", "one-shot per class"
+        self.plans = []
+        for c in db.classes:
+            p = type(c["plan"]).from_buffer_copy(c["plan"])
+            p.precision = PLAIN_FP32
+            self.plans.append(p)
+
+    def grid(self, stream):
+        from idgx import grid
+
+        db = self.db
+        for p, c in zip(self.plans, db.classes):
+            grid(p, c["d_metadata"], c["d_channels"], db.d_uvw, db.d_wavenumbers, db.d_visibilities,
+                 c["d_spheroidal"], c["d_aterms"], c["d_subgrids"], stream)
+
+    def degrid(self, stream):
+        from idgx import degrid
+
+        db = self.db
+        for p, c in zip(self.plans, db.classes):
+            degrid(p, c["d_metadata"], c["d_channels"], db.d_uvw, db.d_wavenumbers, c["d_subgrids"],
+                   c["d_spheroidal"], c["d_aterms"], db.d_vis_out, stream)
+
+
 class Batched:
     """Persistent work-queue kernels over the whole plan (idgx_batch_*), tile shapes chosen by the autotuner.  A
     plan of several subgrid classes runs in the batch's default mode: one persistent launch per direction over
@@ -453,6 +485,26 @@ def run_batched_config(args):
           f"{window} ({1 - nvis / window:.1%} flagged or off-grid), setup {time.perf_counter() - t_setup:.1f} s",
           file=sys.stderr, flush=True)
 
+    def fp64_errors(db):
+        """Relative L2 error against the FP64 oracle on about 60 subgrids per class (selected samples only)."""
+        acc = {}
+        for k, c in enumerate(bo["classes"]):
+            every = args.check_every or max(1, int(c["nr_subgrids"]) // 60)
+            idx = np.arange(every // 2, int(c["nr_subgrids"]), every)
+            sw, sel, rows = class_sample(bo, k, idx, d_vis, d_flags)
+            S = int(c["subgrid_size"])
+            d_sub = db.classes[k]["d_subgrids"]
+            subs = d_sub.view(-1, 4 * S * S * 8)[torch.from_numpy(idx).to(dev)].cpu().numpy()
+            subs = subs.view(np.complex64).reshape(idx.size, 4, S, S)
+            vis_out = db.d_vis_out.view(-1, C * 32)[rows].cpu().numpy().view(np.complex64)
+            vis_out = vis_out.reshape(-1, C, 4)
+            g64 = oracle_grid(sw, fp64=True)
+            v64 = oracle_degrid(sw, subs, fp64=True)
+            acc[f"S{S}"] = {"subgrids": int(idx.size), "gridder": rel_l2(subs, g64),
+                            "gridder_worst_subgrid": float(per_subgrid_rel_l2(subs, g64).max()),
+                            "degridder": rel_l2(vis_out[sel], v64[sel])}
+        return acc
+
     precisions = [args.precision] if args.precision != "fp32" else ["fp16x3"]
     forms = []
     runners = []
@@ -471,22 +523,7 @@ def run_batched_config(args):
               f"{tg * 1e3:.2f} ms, degridder {td * 1e3:.2f} ms, {rec['round_trip_mvis_s']:.1f} Mvis/s",
               file=sys.stderr, flush=True)
         if not args.no_check:
-            acc = {}
-            for k, c in enumerate(bo["classes"]):
-                every = args.check_every or max(1, int(c["nr_subgrids"]) // 60)
-                idx = np.arange(every // 2, int(c["nr_subgrids"]), every)
-                sw, sel, rows = class_sample(bo, k, idx, d_vis, d_flags)
-                S = int(c["subgrid_size"])
-                d_sub = runner.db.classes[k]["d_subgrids"]
-                subs = d_sub.view(-1, 4 * S * S * 8)[torch.from_numpy(idx).to(dev)].cpu().numpy()
-                subs = subs.view(np.complex64).reshape(idx.size, 4, S, S)
-                vis_out = runner.db.d_vis_out.view(-1, C * 32)[rows].cpu().numpy().view(np.complex64)
-                vis_out = vis_out.reshape(-1, C, 4)
-                g64 = oracle_grid(sw, fp64=True)
-                v64 = oracle_degrid(sw, subs, fp64=True)
-                acc[f"S{S}"] = {"subgrids": int(idx.size), "gridder": rel_l2(subs, g64),
-                                "gridder_worst_subgrid": float(per_subgrid_rel_l2(subs, g64).max()),
-                                "degridder": rel_l2(vis_out[sel], v64[sel])}
+            acc = fp64_errors(runner.db)
             rec["rel_l2_vs_fp64"] = acc
         if len(bo["classes"]) > 1:
             # Beside the single launch: one persistent launch per class and direction, with the per-class shapes
@@ -502,6 +539,22 @@ def run_batched_config(args):
             runner.db.batch.set_single_launch(*single)
         forms.append(rec)
         runners.append(runner)
+    # Plain FP32 beside the batch: the flagged samples zeroed in HBM (the mixed forms and their checks are done,
+    # and their selection lists skip these samples anyway), then the one-shot kernels once per class.
+    vv = d_vis.view(n_rows, C, 8)
+    for r0 in range(0, n_rows, 1 << 20):
+        r1 = min(n_rows, r0 + (1 << 20))
+        vv[r0:r1].mul_((~d_flags[r0:r1]).unsqueeze(-1).to(vv.dtype))
+    plain_runner = PlainPerClass(runners[-1].db)
+    p_tg, p_td, p_step = time_form(plain_runner, args.steps, args.warmup, stream)
+    plain = {"precision": "fp32", "path": plain_runner.path, "gridder_ms": p_tg * 1e3, "degridder_ms": p_td * 1e3,
+             "step_ms": p_step * 1e3, "round_trip_mvis_s": nvis / (p_tg + p_td) / 1e6,
+             "note": "counts the selected visibilities, as the mixed forms do; processes every sample of each "
+                     "subgrid's window (flagged samples as zeros)"}
+    if not args.no_check:
+        plain["rel_l2_vs_fp64"] = fp64_errors(plain_runner.db)
+    print(f"bench: fp32 one-shot per class: gridder {p_tg * 1e3:.2f} ms, degridder {p_td * 1e3:.2f} ms, "
+          f"{plain['round_trip_mvis_s']:.1f} Mvis/s", file=sys.stderr, flush=True)
     head, runner = forms[-1], runners[-1]
     tg, td = head["gridder_ms"] / 1e3, head["degridder_ms"] / 1e3
     dominant, t_dom = ("degridder", td) if td > tg else ("gridder", tg)
@@ -562,8 +615,9 @@ def run_batched_config(args):
             "classes": [{"subgrid_size": int(c["subgrid_size"]), "subgrids": int(c["nr_subgrids"]),
                          "visibilities": int(c["nr_visibilities"])} for c in bo["classes"]],
         },
-        "speedup_vs_plain_fp32": None,
+        "speedup_vs_plain_fp32": head["round_trip_mvis_s"] / plain["round_trip_mvis_s"],
         "forms": forms,
+        "plain_fp32": plain,
         "planner": dict(planner, note="device time includes uploading the segments and downloading the plan"),
         "gflops_per_pass": flops / 1e9,
         "gsincos_per_pass": sincos / 1e9,
