@@ -4,8 +4,8 @@
 // gridding/src/degridding_benchmark.cpp:157-166): the whole file is compiled with contraction off, so
 // `a * b + c` rounds twice; the complex multiply-accumulates are written as explicit fused multiply-adds.
 //
-// Gridder: one workgroup of 256 threads owns 1024 pixels of one subgrid (4 per thread, 4 correlations each,
-//   held in VGPRs).  Visibilities and uvw of the subgrid stream through LDS in batches of 256 visibilities;
+// Gridder: one workgroup of 256 threads owns 1024 or 768 pixels of one subgrid (4 or 3 per thread, 4 correlations
+//   each, held in VGPRs).  Visibilities and uvw of the subgrid stream through LDS in batches of 256 visibilities;
 //   every lane reads the same LDS address per visibility (broadcast, conflict-free).  Taper and A-term are
 //   applied in the epilogue, which overwrites the subgrid.
 // Degridder: one workgroup owns 256 (timestep, channel-chunk) items of one subgrid, each thread one timestep x
@@ -317,17 +317,27 @@ __global__ __launch_bounds__(kThreads, (degrid_plain_min_blocks<NCH, SC>())) voi
 }
 } // namespace
 
-constexpr int kGridPix = 4;
-
-hipError_t launch_grid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream)
+template <int NPIX> hipError_t launch_grid_plain_pixels(const KernelArgs &a, bool accurate_sincos, hipStream_t stream)
 {
     const int P = a.p.S * a.p.S;
-    const dim3 grid(a.p.nr_subgrids, (P + kThreads * kGridPix - 1) / (kThreads * kGridPix));
+    const dim3 grid(a.p.nr_subgrids, (P + kThreads * NPIX - 1) / (kThreads * NPIX));
     if (accurate_sincos)
-        gridder_plain<kGridPix, kSincosOcml><<<grid, kThreads, 0, stream>>>(a);
+        gridder_plain<NPIX, kSincosOcml><<<grid, kThreads, 0, stream>>>(a);
     else
-        gridder_plain<kGridPix, kSincosHw><<<grid, kThreads, 0, stream>>>(a);
+        gridder_plain<NPIX, kSincosHw><<<grid, kThreads, 0, stream>>>(a);
     return hipGetLastError();
+}
+
+// Pixels per gridder thread: 4 (1024 per workgroup) unless 3 (768) leaves fewer lanes idle, as for S = 24 (576
+// pixels: 75 % of the lanes busy with 3, 56 % with 4) and S = 48 (2304: 100 % with 3, 75 % with 4).  Lanes past
+// the last pixel still run the visibility loop, so idle lanes cost time; the per-pixel cost is the same for both.
+hipError_t launch_grid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream)
+{
+    const long long P = static_cast<long long>(a.p.S) * a.p.S;
+    const long long slots4 = (P + 1023) / 1024 * 1024, slots3 = (P + 767) / 768 * 768;
+    if (slots3 < slots4)
+        return launch_grid_plain_pixels<3>(a, accurate_sincos, stream);
+    return launch_grid_plain_pixels<4>(a, accurate_sincos, stream);
 }
 
 template <int NCH> hipError_t launch_degrid_plain_chunks(const KernelArgs &a, bool accurate_sincos, hipStream_t stream)

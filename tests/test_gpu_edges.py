@@ -176,3 +176,23 @@ def test_plain_degridder_channel_chunks_on_ragged_plans_without_windows(C):
     r0 = int(md["time_offset"][0])
     assert not vis[r0:r0 + 128].any()
     assert errs.max() <= PARITY
+
+
+@pytest.mark.parametrize("S", [24, 40, 48])
+def test_plain_pixels_per_thread_by_subgrid_size(S):
+    """The plain gridder takes 3 pixels per thread where that leaves fewer idle lanes (S = 24, 48) and 4 otherwise
+    (S = 40); both directions meet the bar against the FP32 restatement of the reference CPU path."""
+    w = observation.make_observation("small", subgrid_size=S, timesteps_per_subgrid=64)
+    w = observation.subset(w, np.arange(0, w["nr_subgrids"], 3))
+    rng = np.random.default_rng(S)
+    sub = (rng.standard_normal((w["nr_subgrids"], 4, S, S)) +
+           1j * rng.standard_normal((w["nr_subgrids"], 4, S, S))).astype(np.complex64)
+    dw = DeviceWorkload(w)
+    plan = dw.plan()
+    dw.grid(plan)
+    eg = per_subgrid_rel_l2(dw.subgrids(), oracle_grid(w)).max()
+    dw.d_subgrids.copy_(upload(sub))
+    dw.degrid(plan)
+    ed = _vis_errors(w, dw.visibilities_out(), oracle_degrid(w, sub)).max()
+    print(f"S={S} ({w['nr_subgrids']} subgrids) plain: gridder worst {eg:.2e}, degridder worst {ed:.2e}")
+    assert eg <= PARITY and ed <= PARITY
