@@ -328,12 +328,17 @@ int idgx_plan_segments(const idgx_planner *planner, const float *d_uvw, const do
  * mode: the whole plan is one launch per direction over a merged queue of every class's items, heaviest first
  * across classes, the gridder's items each with their class's tiles per wave.  idgx_batch_set_single_launch(b,
  * -1, -1) switches to one launch per class, each class with its own tile shape (a one-class batch runs that way
- * from the start; for one class the two modes coincide).  Optional per-class visibility selections (flagging, ragged timesteps) restrict each subgrid to the
- * listed entries: subgrid s uses vis_index[vis_offset[s] .. vis_offset[s+1]), each entry (t << 16) | c relative to
- * the subgrid's first row and channel window.  Creation allocates the queue memory; the execution entry points
- * allocate nothing, and idgx_batch_capture records them into a hipGraph (no memset nodes) for idgx_batch_launch.
- * Batches run the mixed-precision forms (precision != IDGX_PLAIN_FP32).  A batch is used from one host thread at a
- * time. */
+ * from the start; for one class the two modes coincide).  Optional per-class visibility selections (flagging,
+ * ragged timesteps) restrict each subgrid to the listed entries: subgrid s uses vis_index[vis_offset[s] ..
+ * vis_offset[s+1]), each entry (t << 16) | c relative to the subgrid's first row and channel window.  Creation
+ * allocates the queue memory; the execution entry points allocate nothing, and idgx_batch_capture records them into
+ * a hipGraph (no memset nodes) for idgx_batch_launch.
+ * Every precision batches: the mixed forms run their persistent MFMA kernels (shapes idgx_batch_shape_info), plain
+ * FP32 the persistent forms of the plain kernels (shapes idgx_batch_plain_shape_info), which skip unselected
+ * (flagged) samples instead of multiplying them by zero and otherwise round like idgx_grid / idgx_degrid.  They
+ * replace the reference's one-block-per-subgrid launches over every sample (gridding/src/gridding_benchmark.cu:
+ * 323-332, gridding/src/degridding_benchmark.cu:283).  All classes of a batch share precision and flags.  A batch
+ * is used from one host thread at a time. */
 
 typedef struct idgx_batch idgx_batch;
 
@@ -375,10 +380,15 @@ typedef struct
 int idgx_batch_create(const idgx_batch_class *classes, int nr_classes, idgx_batch **batch);
 void idgx_batch_destroy(idgx_batch *batch);
 
-/* Tile shapes: shape i = (tiles of 16 pixels / visibilities per wave, LDS staging depth, phases on the matrix
- * core). */
+/* Tile shapes of the mixed forms: shape i = (tiles of 16 pixels / visibilities per wave, LDS staging depth, phases
+ * on the matrix core). */
 int idgx_batch_shape_count(void);
 int idgx_batch_shape_info(int shape, int *tiles_per_wave, int *lds_batch, int *phase_mfma);
+/* Shapes of a plain FP32 batch: shape i = (gridder pixels per thread, degridder visibilities per thread).  The
+ * shape arguments of the calls below index this table when the batch's precision is IDGX_PLAIN_FP32, and with
+ * IDGX_SHAPE_TILES_PER_CLASS each gridder item takes its class's pixels per thread. */
+int idgx_batch_plain_shape_count(void);
+int idgx_batch_plain_shape_info(int shape, int *gridder_pixels_per_thread, int *degridder_visibilities_per_thread);
 /* set_shape sets every class's shapes; get_shape reports class 0's. */
 int idgx_batch_set_shape(idgx_batch *batch, int gridder_shape, int degridder_shape);
 int idgx_batch_get_shape(const idgx_batch *batch, int *gridder_shape, int *degridder_shape);

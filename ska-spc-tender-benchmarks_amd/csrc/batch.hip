@@ -21,8 +21,12 @@
 // splits the pixels into the packed words the degridder stages (32 B per pixel slot), so the work items copy
 // them instead of recomputing them for every chunk of visibilities.
 //
-// Device memory (queues, counters, the prepared pixels: 32 B per pixel per subgrid) is allocated once in
-// idgx_batch_create; the execution entry points allocate nothing.
+// Plain FP32 batches run the persistent forms of the plain kernels (pixels per thread for the gridder, visibilities
+// per thread for the degridder as their shapes), with the same queues, launch modes, capture and autotuner, and
+// need no prepared pixels: their degridder items apply taper and A-terms as they stage the pixels.
+//
+// Device memory (queues, counters, and for the mixed forms the prepared pixels: 32 B per pixel per subgrid) is
+// allocated once in idgx_batch_create; the execution entry points allocate nothing.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -86,6 +90,65 @@ void drop_graph(idgx_batch *b)
     b->graph = nullptr;
 }
 
+// Shape tables: the plain FP32 kernels and the mixed-precision kernels have shapes of their own.
+bool plain(const idgx_batch *b)
+{
+    return b->precision == IDGX_PLAIN_FP32;
+}
+
+bool accurate_sincos(const idgx_batch *b)
+{
+    return (b->plans[0].flags & IDGX_FLAG_ACCURATE_SINCOS) != 0;
+}
+
+int shape_count(const idgx_batch *b)
+{
+    return plain(b) ? idgx::plain_shape_count() : idgx::mixed_shape_count();
+}
+
+int default_shape(const idgx_batch *b, bool gridder, int S)
+{
+    return plain(b) ? idgx::plain_default_shape(gridder, S) : idgx::mixed_default_shape(gridder, S);
+}
+
+int grid_items_per_subgrid(const idgx_batch *b, int shape, int S)
+{
+    return plain(b) ? idgx::plain_grid_items_per_subgrid(shape, S) : idgx::grid_items_per_subgrid(shape, S);
+}
+
+long long grid_item_pixels(const idgx_batch *b, int shape, int S, int item)
+{
+    return plain(b) ? idgx::plain_grid_item_pixels(shape, S, item) : idgx::grid_item_pixels(shape, S, item);
+}
+
+long long degrid_visibilities_per_item(const idgx_batch *b, int shape)
+{
+    return plain(b) ? idgx::plain_degrid_visibilities_per_item(shape) : idgx::degrid_visibilities_per_item(shape);
+}
+
+// Per-item tiles of a gridder single launch with each class's own shape: tiles per wave (mixed) or pixels per
+// thread (plain) of the class's gridder shape.
+int class_tiles(const idgx_batch *b, int shape)
+{
+    return plain(b) ? idgx::plain_shape(shape).pixels : idgx::mixed_shape(shape).tiles;
+}
+
+int workgroups_for(const idgx_batch *b, bool gridder, int shape)
+{
+    return plain(b) ? idgx::plain_queue_workgroups(accurate_sincos(b), gridder, shape)
+                    : idgx::queue_workgroups(b->precision, gridder, shape);
+}
+
+hipError_t launch_queue(const idgx_batch *b, const QueueArgs &q, bool gridder, int shape, int workgroups,
+                        hipStream_t st)
+{
+    if (plain(b))
+        return gridder ? idgx::launch_grid_plain_queue(q, accurate_sincos(b), shape, workgroups, st)
+                       : idgx::launch_degrid_plain_queue(q, accurate_sincos(b), shape, workgroups, st);
+    return gridder ? idgx::launch_grid_mixed_queue(q, b->precision, shape, workgroups, st)
+                   : idgx::launch_degrid_mixed_queue(q, b->precision, shape, workgroups, st);
+}
+
 struct Item
 {
     uint2 v;
@@ -96,13 +159,13 @@ struct Item
 std::vector<Item> build_items(const idgx_batch *b, bool gridder, int k, int shape)
 {
     std::vector<Item> items;
-    const long long per = idgx::degrid_visibilities_per_item(shape);
+    const long long per = degrid_visibilities_per_item(b, shape);
     const int S = b->plans[k].subgrid_size;
     const long long P = static_cast<long long>(S) * S;
-    const int gitems = idgx::grid_items_per_subgrid(shape, S);
+    const int gitems = grid_items_per_subgrid(b, shape, S);
     std::vector<double> gpix(gitems);
     for (int i = 0; i < gitems; ++i)
-        gpix[i] = static_cast<double>(idgx::grid_item_pixels(shape, S, i));
+        gpix[i] = static_cast<double>(grid_item_pixels(b, shape, S, i));
     for (int s = 0; s < b->plans[k].nr_subgrids; ++s)
     {
         const long long V = b->vis_count[k][s];
@@ -128,15 +191,15 @@ std::vector<Item> build_items(const idgx_batch *b, bool gridder, int k, int shap
 size_t max_items(const idgx_batch *b, bool gridder, int k)
 {
     size_t m = 0;
-    for (int sh = 0; sh < idgx::mixed_shape_count(); ++sh)
+    for (int sh = 0; sh < shape_count(b); ++sh)
     {
         size_t n = 0;
         if (gridder)
             n = static_cast<size_t>(b->plans[k].nr_subgrids) *
-                static_cast<size_t>(idgx::grid_items_per_subgrid(sh, b->plans[k].subgrid_size));
+                static_cast<size_t>(grid_items_per_subgrid(b, sh, b->plans[k].subgrid_size));
         else
         {
-            const long long per = idgx::degrid_visibilities_per_item(sh);
+            const long long per = degrid_visibilities_per_item(b, sh);
             for (const int64_t V : b->vis_count[k])
                 n += static_cast<size_t>((V + per - 1) / per);
         }
@@ -163,7 +226,7 @@ int upload_queue(idgx_batch *b, bool gridder, int k, int shape)
     }
     b->nr_items[dir][k] = static_cast<int>(flat.size());
     b->shape[dir][k] = shape;
-    b->workgroups[dir][k] = idgx::queue_workgroups(b->precision, gridder, shape);
+    b->workgroups[dir][k] = workgroups_for(b, gridder, shape);
     return b->workgroups[dir][k] > 0 ? IDGX_OK : IDGX_ERR_DEVICE;
 }
 
@@ -185,7 +248,7 @@ int upload_single_queue(idgx_batch *b, bool gridder, int shape)
         const std::vector<Item> ki = build_items(b, gridder, k, per_class ? b->shape[0][k] : shape);
         items.insert(items.end(), ki.begin(), ki.end());
         if (per_class)
-            b->single_tiles[k] = idgx::mixed_shape(b->shape[0][k]).tiles;
+            b->single_tiles[k] = class_tiles(b, b->shape[0][k]);
     }
     std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
     size_t room = 0;
@@ -205,7 +268,7 @@ int upload_single_queue(idgx_batch *b, bool gridder, int shape)
     }
     b->single_items[dir] = static_cast<int>(flat.size());
     b->single_shape[dir] = shape;
-    b->single_workgroups[dir] = idgx::queue_workgroups(b->precision, gridder, shape);
+    b->single_workgroups[dir] = workgroups_for(b, gridder, shape);
     return b->single_workgroups[dir] > 0 ? IDGX_OK : IDGX_ERR_DEVICE;
 }
 
@@ -245,7 +308,7 @@ int make_queue_args(const idgx_batch *b, const idgx_batch_data *d, bool gridder,
         a.subgrids_out = c.d_subgrids;
         a.vis_index = c.d_vis_index;
         a.vis_offset = reinterpret_cast<const long long *>(c.d_vis_offset);
-        if (!gridder)
+        if (!gridder && !plain(b))
         {
             a.prepared = b->d_prepared[k];
             a.prepared_scale = b->d_prepared_scale[k];
@@ -268,7 +331,7 @@ int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st, i
     b->last_launches[dir] = 0;
     if (only < 0 && single_mode(b, dir))
     {
-        for (int k = 0; k < b->nr_classes && e == hipSuccess && !gridder; ++k)
+        for (int k = 0; k < b->nr_classes && e == hipSuccess && !gridder && !plain(b); ++k)
             if (b->nr_items[dir][k] > 0)
                 e = idgx::launch_degrid_prepare(q.cls[k], b->precision, st);
         if (e != hipSuccess || b->single_items[dir] == 0)
@@ -278,9 +341,7 @@ int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st, i
         q.counter = b->d_counters + dir * idgx::kMaxClasses;
         for (int k = 0; k < idgx::kMaxClasses; ++k)
             q.tiles[k] = b->single_tiles[k];
-        const int sh = b->single_shape[dir], wg = b->single_workgroups[dir];
-        e = gridder ? idgx::launch_grid_mixed_queue(q, b->precision, sh, wg, st)
-                    : idgx::launch_degrid_mixed_queue(q, b->precision, sh, wg, st);
+        e = launch_queue(b, q, gridder, b->single_shape[dir], b->single_workgroups[dir], st);
         b->last_launches[dir] += e == hipSuccess;
         return to_status(e);
     }
@@ -288,15 +349,14 @@ int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st, i
     {
         if ((only >= 0 && k != only) || b->nr_items[dir][k] == 0)
             continue;
-        if (!gridder)
+        if (!gridder && !plain(b))
             e = idgx::launch_degrid_prepare(q.cls[k], b->precision, st);
         if (e != hipSuccess)
             break;
         q.items = b->d_items[dir] + b->offset[dir][k];
         q.nr_items = b->nr_items[dir][k];
         q.counter = b->d_counters + dir * idgx::kMaxClasses + k;
-        e = gridder ? idgx::launch_grid_mixed_queue(q, b->precision, b->shape[0][k], b->workgroups[0][k], st)
-                    : idgx::launch_degrid_mixed_queue(q, b->precision, b->shape[1][k], b->workgroups[1][k], st);
+        e = launch_queue(b, q, gridder, b->shape[dir][k], b->workgroups[dir][k], st);
         b->last_launches[dir] += e == hipSuccess;
     }
     return to_status(e);
@@ -319,13 +379,11 @@ extern "C" int idgx_batch_create(const idgx_batch_class *classes, int nr_classes
             return rc;
         if (c.plan.precision != p0.precision || c.plan.nr_channels != p0.nr_channels || c.plan.nr_rows != p0.nr_rows ||
             c.plan.nr_stations != p0.nr_stations || c.plan.grid_size != p0.grid_size ||
-            c.plan.image_size != p0.image_size)
+            c.plan.image_size != p0.image_size || c.plan.flags != p0.flags)
             return IDGX_ERR_INVALID_ARGUMENT; // classes differ only in subgrid size / subgrids / A-term slots
         if (c.plan.nr_subgrids >= (1 << 28))
             return IDGX_ERR_UNSUPPORTED;
     }
-    if (p0.precision == IDGX_PLAIN_FP32)
-        return IDGX_ERR_UNSUPPORTED; // the persistent forms are the mixed-precision kernels
     idgx_batch *b = new (std::nothrow) idgx_batch();
     if (!b)
         return IDGX_ERR_OUT_OF_MEMORY;
@@ -369,7 +427,7 @@ extern "C" int idgx_batch_create(const idgx_batch_class *classes, int nr_classes
     }
     if (e == hipSuccess)
         e = hipMalloc(&b->d_counters, 2 * idgx::kMaxClasses * sizeof(int));
-    for (int k = 0; k < nr_classes && e == hipSuccess; ++k)
+    for (int k = 0; k < nr_classes && e == hipSuccess && !plain(b); ++k)
     {
         const int S = b->plans[k].subgrid_size;
         const long long n = std::max(1, b->plans[k].nr_subgrids);
@@ -383,9 +441,9 @@ extern "C" int idgx_batch_create(const idgx_batch_class *classes, int nr_classes
     for (int k = 0; k < nr_classes && rc == IDGX_OK; ++k)
     {
         const int S = b->plans[k].subgrid_size;
-        rc = upload_queue(b, true, k, idgx::mixed_default_shape(true, S));
+        rc = upload_queue(b, true, k, default_shape(b, true, S));
         if (rc == IDGX_OK)
-            rc = upload_queue(b, false, k, idgx::mixed_default_shape(false, S));
+            rc = upload_queue(b, false, k, default_shape(b, false, S));
     }
     // Several classes: the whole plan as one persistent launch per direction, gridder items with their class's
     // tiles per wave, the degridder with its default shape (the same for every subgrid size).
@@ -393,7 +451,7 @@ extern "C" int idgx_batch_create(const idgx_batch_class *classes, int nr_classes
     {
         rc = upload_single_queue(b, true, idgx::kShapeTilesPerClass);
         if (rc == IDGX_OK)
-            rc = upload_single_queue(b, false, idgx::mixed_default_shape(false, b->plans[0].subgrid_size));
+            rc = upload_single_queue(b, false, default_shape(b, false, b->plans[0].subgrid_size));
     }
     if (rc != IDGX_OK)
     {
@@ -423,10 +481,28 @@ extern "C" int idgx_batch_shape_info(int shape, int *tiles_per_wave, int *lds_ba
     return IDGX_OK;
 }
 
+extern "C" int idgx_batch_plain_shape_count(void)
+{
+    return idgx::plain_shape_count();
+}
+
+extern "C" int idgx_batch_plain_shape_info(int shape, int *gridder_pixels_per_thread,
+                                           int *degridder_visibilities_per_thread)
+{
+    if (shape < 0 || shape >= idgx::plain_shape_count())
+        return IDGX_ERR_INVALID_ARGUMENT;
+    const idgx::PlainShape s = idgx::plain_shape(shape);
+    if (gridder_pixels_per_thread)
+        *gridder_pixels_per_thread = s.pixels;
+    if (degridder_visibilities_per_thread)
+        *degridder_visibilities_per_thread = s.entries;
+    return IDGX_OK;
+}
+
 extern "C" int idgx_batch_set_class_shape(idgx_batch *b, int cls, int gridder_shape, int degridder_shape)
 {
-    if (!b || cls < 0 || cls >= b->nr_classes || gridder_shape < 0 || gridder_shape >= idgx::mixed_shape_count() ||
-        degridder_shape < 0 || degridder_shape >= idgx::mixed_shape_count())
+    if (!b || cls < 0 || cls >= b->nr_classes || gridder_shape < 0 || gridder_shape >= shape_count(b) ||
+        degridder_shape < 0 || degridder_shape >= shape_count(b))
         return IDGX_ERR_INVALID_ARGUMENT;
     drop_graph(b);
     int rc = IDGX_OK;
@@ -470,8 +546,8 @@ extern "C" int idgx_batch_get_shape(const idgx_batch *b, int *gridder_shape, int
 
 extern "C" int idgx_batch_set_single_launch(idgx_batch *b, int gridder_shape, int degridder_shape)
 {
-    if (!b || gridder_shape < idgx::kShapeTilesPerClass || gridder_shape >= idgx::mixed_shape_count() ||
-        degridder_shape < -1 || degridder_shape >= idgx::mixed_shape_count())
+    if (!b || gridder_shape < idgx::kShapeTilesPerClass || gridder_shape >= shape_count(b) ||
+        degridder_shape < -1 || degridder_shape >= shape_count(b))
         return IDGX_ERR_INVALID_ARGUMENT;
     drop_graph(b);
     const int want[2] = {gridder_shape, degridder_shape};
@@ -538,7 +614,7 @@ extern "C" int idgx_batch_autotune(idgx_batch *b, const idgx_batch_data *data, i
         {
             float best = -1.F;
             int best_shape = b->shape[dir][k];
-            for (int sh = 0; sh < idgx::mixed_shape_count() && rc == IDGX_OK; ++sh)
+            for (int sh = 0; sh < shape_count(b) && rc == IDGX_OK; ++sh)
             {
                 rc = upload_queue(b, gridder, k, sh);
                 if (rc != IDGX_OK)
@@ -571,7 +647,7 @@ extern "C" int idgx_batch_autotune(idgx_batch *b, const idgx_batch_data *data, i
         // the one launch), timed on the whole direction.
         float best = -1.F;
         int best_shape = b->single_shape[dir];
-        for (int sh = gridder ? idgx::kShapeTilesPerClass : 0; sh < idgx::mixed_shape_count() && rc == IDGX_OK; ++sh)
+        for (int sh = gridder ? idgx::kShapeTilesPerClass : 0; sh < shape_count(b) && rc == IDGX_OK; ++sh)
         {
             if (sh == -1)
                 continue;

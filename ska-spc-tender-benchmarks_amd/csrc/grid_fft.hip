@@ -249,7 +249,16 @@ template <int SG> __device__ __forceinline__ float2 twiddle(int q, int log_g)
 // L = 16384: 2 x 1024 threads, 2 x 72 KB of LDS); each hides the other's loads and barriers.  Measured against the
 // form that loaded the next row during the transform (97-109 VGPRs): 3-4 % faster at 1024 .. 4096, 0.5 % at 8192,
 // 3-6 % at 16384, and 10 % on the boxed transform to the image, whose row reads are mostly skipped.
-template <int L, int SG> __global__ __launch_bounds__(L / 16, 2) void fft_row_kernel(FftArgs A)
+// The second __launch_bounds__ argument is HIP's minimum number of waves per SIMD (amdgpu-waves-per-eu): two
+// workgroups of L / 16 threads per CU put max(2, L / 2048) waves on each SIMD, which caps the kernel at 512 / that
+// many VGPRs (64 at L = 16384), so the two-workgroup occupancy described above is enforced, not incidental.
+template <int L> constexpr int fft_row_waves_per_simd()
+{
+    return L / 2048 > 2 ? L / 2048 : 2;
+}
+
+template <int L, int SG>
+__global__ __launch_bounds__(L / 16, fft_row_waves_per_simd<L>()) void fft_row_kernel(FftArgs A)
 {
     constexpr int kLog = ilog2(L);
     constexpr int kStages16 = kLog / 4;

@@ -47,6 +47,27 @@ struct QueueArgs
     int tiles[kMaxClasses]; // gridder tiles per wave of each class (shape kShapeTilesPerClass only)
 };
 
+// Takes the next queue item (all threads of the workgroup see the same one); returns false when the queue is
+// exhausted.  The counter only grows, so every workgroup leaves after at most nr_items + 1 attempts in total.
+__device__ __forceinline__ bool next_queue_item(const QueueArgs &q, int &cls, int &s, int &blk)
+{
+    __shared__ int s_item;
+    if (threadIdx.x == 0)
+        s_item = atomicAdd(q.counter, 1);
+    __syncthreads();
+    // The same value in every lane: read it into a scalar register, so that the item and the class's arguments
+    // (QueueArgs::cls[cls]) are scalar loads rather than per-lane copies held in vector registers.
+    const int i = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (i >= q.nr_items)
+        return false;
+    const uint2 it = q.items[i];
+    cls = __builtin_amdgcn_readfirstlane(static_cast<int>(it.x >> 28));
+    s = __builtin_amdgcn_readfirstlane(static_cast<int>(it.x & 0x0fffffffu));
+    blk = __builtin_amdgcn_readfirstlane(static_cast<int>(it.y));
+    return true;
+}
+
 // Gridder queue "shape" of a single launch over several classes in which each item takes its class's tiles per
 // wave (QueueArgs::tiles: 2, 3, 4, 6 or 8), with the default LDS batch and phases on the VALU.
 constexpr int kShapeTilesPerClass = IDGX_SHAPE_TILES_PER_CLASS;
@@ -90,6 +111,26 @@ __host__ __device__ inline int degrid_row_stride(int S)
 
 hipError_t launch_grid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream);
 hipError_t launch_degrid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream);
+
+// Persistent work-queue forms of the plain FP32 kernels (batches).  Shape i: gridder pixels per thread and
+// degridder visibilities per thread; kShapeTilesPerClass (gridder) gives every item its class's pixels per thread
+// (QueueArgs::tiles).
+struct PlainShape
+{
+    int pixels;  // gridder pixels per thread (a workgroup item covers 256 x pixels pixels of one subgrid)
+    int entries; // degridder visibilities per thread (an item covers 256 x entries selected visibilities)
+};
+int plain_shape_count();
+PlainShape plain_shape(int i);
+int plain_default_shape(bool gridder, int S);
+int plain_grid_items_per_subgrid(int shape, int S);
+long long plain_grid_item_pixels(int shape, int S, int item);
+int plain_degrid_visibilities_per_item(int shape);
+hipError_t launch_grid_plain_queue(const QueueArgs &q, bool accurate_sincos, int shape, int workgroups,
+                                   hipStream_t st);
+hipError_t launch_degrid_plain_queue(const QueueArgs &q, bool accurate_sincos, int shape, int workgroups,
+                                     hipStream_t st);
+int plain_queue_workgroups(bool accurate_sincos, bool gridder, int shape);
 hipError_t launch_grid_mixed(const KernelArgs &a, int precision, bool phase_mfma, hipStream_t stream);
 hipError_t launch_degrid_mixed(const KernelArgs &a, int precision, bool phase_mfma, hipStream_t stream);
 

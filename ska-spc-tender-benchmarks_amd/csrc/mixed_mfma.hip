@@ -906,31 +906,12 @@ __global__ __launch_bounds__(kThreads, (degridder_min_blocks<VT, PB>())) void de
     degridder_item<DT, SPLIT, VT, PB, PM>(a, blockIdx.x - vblk * a.p.nr_subgrids, vblk, sm);
 }
 
-// Takes the next queue item (all threads of the workgroup see the same one); returns false when the queue is
-// exhausted.  The counter only grows, so every workgroup leaves after at most nr_items + 1 attempts in total.
-__device__ __forceinline__ bool next_item(const QueueArgs &q, int &cls, int &s, int &blk)
-{
-    __shared__ int s_item;
-    if (threadIdx.x == 0)
-        s_item = atomicAdd(q.counter, 1);
-    __syncthreads();
-    const int i = s_item;
-    __syncthreads();
-    if (i >= q.nr_items)
-        return false;
-    const uint2 it = q.items[i];
-    cls = static_cast<int>(it.x >> 28);
-    s = static_cast<int>(it.x & 0x0fffffffu);
-    blk = static_cast<int>(it.y);
-    return true;
-}
-
 template <int DT, bool SPLIT, int PT, int VB, bool PM>
 __global__ __launch_bounds__(kThreads, (gridder_min_blocks<PT, VB>())) void gridder_mfma_queue(QueueArgs q)
 {
     __shared__ GridderShared<PT, VB> sm;
     int cls, s, blk;
-    while (next_item(q, cls, s, blk))
+    while (next_queue_item(q, cls, s, blk))
     {
         gridder_item<DT, SPLIT, PT, VB, PM>(q.cls[cls], s, blk, sm);
         __syncthreads();
@@ -945,7 +926,7 @@ __global__ __launch_bounds__(kThreads, (gridder_min_blocks<8, kDefaultVB>())) vo
 {
     __shared__ GridderShared<8, kDefaultVB> sm;
     int cls, s, blk;
-    while (next_item(q, cls, s, blk))
+    while (next_queue_item(q, cls, s, blk))
     {
         switch (q.tiles[cls]) // workgroup-uniform
         {
@@ -974,7 +955,7 @@ __global__ __launch_bounds__(kThreads, (degridder_min_blocks<VT, PB>())) void de
 {
     __shared__ DegridderShared<VT, PB> sm;
     int cls, s, blk;
-    while (next_item(q, cls, s, blk))
+    while (next_queue_item(q, cls, s, blk))
         degridder_item<DT, SPLIT, VT, PB, PM>(q.cls[cls], s, blk, sm);
 }
 

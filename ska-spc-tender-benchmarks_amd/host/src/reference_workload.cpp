@@ -4,7 +4,7 @@
 // gridding/src/degridding_benchmark.cpp:201-230) seed glibc's rand() with srand(0) and then call the
 // initialize_* routines of gridding/src/gridding_initialization.cpp in a fixed order.  The drop-in driver must
 // feed the HIP backend exactly those inputs, so this file evaluates the same expressions with the same float /
-// double promotions and the same rand() call order; tests/test_workload.py pins every array against the golden
+// double promotions and the same rand() call order; tests/test_host.py pins every array against the golden
 // vectors produced from the compiled reference.  Built with -ffp-contract=off.
 #include "reference_workload.h"
 

@@ -181,6 +181,7 @@ def lib():
         L.idgx_batch_last_launches.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.idgx_batch_get_single_launch.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.idgx_batch_shape_info.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
+        L.idgx_batch_plain_shape_info.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 2
         L.idgx_batch_autotune.argtypes = [vp, ctypes.POINTER(BatchData), ctypes.c_int, vp,
                                           ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         L.idgx_batch_grid.argtypes = [vp, ctypes.POINTER(BatchData), vp]
@@ -467,6 +468,7 @@ class Batch:
         _check(lib().idgx_batch_create(arr, len(classes), ctypes.byref(h)), "idgx_batch_create")
         self.handle = h
         self.nr_classes = len(classes)
+        self.precision = int(classes[0][0].precision)
 
     def close(self):
         if self.handle:
@@ -536,6 +538,7 @@ class Batch:
 
     @staticmethod
     def shapes():
+        """Shapes of the mixed forms: (tiles per wave, LDS batch, phases on the matrix core)."""
         out = []
         for i in range(lib().idgx_batch_shape_count()):
             t, b, pm = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -543,6 +546,21 @@ class Batch:
                    "idgx_batch_shape_info")
             out.append((t.value, b.value, pm.value))
         return out
+
+    @staticmethod
+    def plain_shapes():
+        """Shapes of plain FP32 batches: (gridder pixels per thread, degridder visibilities per thread)."""
+        out = []
+        for i in range(lib().idgx_batch_plain_shape_count()):
+            px, nv = ctypes.c_int(), ctypes.c_int()
+            _check(lib().idgx_batch_plain_shape_info(i, ctypes.byref(px), ctypes.byref(nv)),
+                   "idgx_batch_plain_shape_info")
+            out.append((px.value, nv.value))
+        return out
+
+    def shape_table(self):
+        """The shape table this batch's shape indices refer to."""
+        return Batch.plain_shapes() if self.precision == PLAIN_FP32 else Batch.shapes()
 
     def autotune(self, data, repeats=2, stream=None):
         g, d = ctypes.c_float(), ctypes.c_float()

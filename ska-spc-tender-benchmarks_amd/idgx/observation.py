@@ -254,9 +254,9 @@ CONFIGS = {
 # Batched uneven workloads (make_batched_observation): baselines split into length terciles with subgrid sizes
 # 24 / 32 / 48, per-baseline time chunks log-uniform in t_range, per-baseline flag fraction U(0, flag_max).
 BATCHED_CONFIGS = {
-    # 256 stations (half in a 1 km core, the rest in a 20 km disk), 512 x 1 s, 128 ch 150-180 MHz, G = 8192.
+    # 256 stations (half in a 1 km core, the rest in a 20 km disk), 512 x 1 s, 128 ch 150-200 MHz, G = 8192.
     "batched_uneven": dict(n_stations=256, radius_m=20000.0, core_fraction=0.5, core_radius_m=1000.0, n_times=512,
-                           f_lo=150e6, f_hi=180e6, n_channels=128, subgrid_sizes=(24, 32, 48), t_range=(16, 256),
+                           f_lo=150e6, f_hi=200e6, n_channels=128, subgrid_sizes=(24, 32, 48), t_range=(16, 256),
                            grid_size=8192, image_size=0.1, dec_deg=-30.0, zero_w=False, flag_max=0.6, seed=4),
     # The same construction at a size the oracle checks in seconds.
     "batched_uneven_small": dict(n_stations=12, radius_m=3000.0, core_fraction=0.5, core_radius_m=500.0,
@@ -378,6 +378,45 @@ def make_batched_observation(name=None, *, visibilities=True, flags=True, **over
             c["nr_visibilities"] = int((md["nr_timesteps"].astype(np.int64) * ch["nr_channels"]).sum())
         classes.append(c)
     return dict(shared, config=cfg, classes=classes, flags=fl,
+                nr_visibilities=sum(c["nr_visibilities"] for c in classes))
+
+
+def subset_batched(bo, indices, *, seed=0):
+    """Subgrids indices[k] of every class k of a batched observation (built with visibilities=False, flags=False) as
+    a compact batched observation: the rows they read gathered into one row space (class by class), seeded N(0, 1)
+    visibilities for those rows, flags drawn for those rows only (per-baseline fraction f_b ~ U(0, flag_max) from
+    the configuration's seed, as baseline_flags draws them, per-sample uniforms from `seed`) and each class's
+    selection lists.  The plan records, channel windows and uvw rows are the full plan's, so the subset exercises
+    the full workload's geometry (subgrid sizes, windows, ragged timesteps, long baselines) at a size the oracle
+    checks in seconds.  'source_rows' maps the rows back to the full observation."""
+    from .device import selection_lists
+
+    cfg = bo["config"]
+    n_times, C = cfg["n_times"], bo["wavenumbers"].shape[0]
+    parts, cursor = [], 0
+    for k, c in enumerate(bo["classes"]):
+        sw = subset(dict(c, visibilities=None), indices[k])
+        sw["metadata"]["time_offset"] += cursor
+        cursor += sw["nr_rows"]
+        parts.append(sw)
+    rows = np.concatenate([sw["source_rows"] for sw in parts]) if parts else np.zeros(0, np.int64)
+    n_rows = rows.size
+    frac = np.random.default_rng(cfg["seed"] + 2).uniform(0.0, cfg["flag_max"], len(bo["classes"][0]["stations"][0]))
+    frac = frac.astype(np.float32)
+    rng = np.random.default_rng(seed + 17)
+    flags = rng.random((n_rows, C), dtype=np.float32) < frac[rows // n_times][:, None]
+    shared = dict(uvw=np.ascontiguousarray(bo["uvw"][rows]), frequencies=bo["frequencies"],
+                  wavenumbers=bo["wavenumbers"], nr_rows=n_rows, grid_size=bo["grid_size"],
+                  nr_stations=bo["nr_stations"], image_size=bo["image_size"], w_step=bo["w_step"],
+                  visibilities=random_visibilities(n_rows, C, cfg["seed"] + seed))
+    classes = []
+    for sw in parts:
+        c = dict(sw, **shared)
+        c["max_nr_timesteps"] = int(c["metadata"]["nr_timesteps"].max()) if c["metadata"].size else 0
+        c["vis_index"], c["vis_offset"] = selection_lists(c, flags)
+        c["nr_visibilities"] = int(c["vis_offset"][-1])
+        classes.append(c)
+    return dict(shared, config=cfg, classes=classes, flags=flags, source_rows=rows,
                 nr_visibilities=sum(c["nr_visibilities"] for c in classes))
 
 

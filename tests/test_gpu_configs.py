@@ -124,3 +124,97 @@ def test_device_selection_lists_match_the_host_lists(uneven):
         d_vi, d_off, h_off = device_selection_lists(c, d_flags, chunk_entries=4096)
         assert np.array_equal(h_off, c["vis_offset"]) and np.array_equal(d_off.cpu().numpy(), c["vis_offset"])
         assert np.array_equal(d_vi.cpu().numpy().view(np.uint32), c["vis_index"])
+
+
+# ---- config 4 on its own workload: the full 256-station plan, every k-th subgrid of each class ----------------------
+
+@pytest.fixture(scope="module")
+def c4():
+    """The full batched_uneven plan (256 stations, 512 x 1 s, 128 channels 150-200 MHz, S = 24 / 32 / 48 by
+    baseline-length tercile, time chunks log-uniform in [16, 256], G = 8192), and a subset of about 60 subgrids per
+    class spread evenly over each class's plan, with seeded visibilities and per-baseline flags on their rows."""
+    bo = observation.make_batched_observation("batched_uneven", visibilities=False, flags=False)
+    idx = [np.unique(np.linspace(0, int(c["nr_subgrids"]) - 1, 60).astype(np.int64)) for c in bo["classes"]]
+    sb = observation.subset_batched(bo, idx, seed=9)
+    return bo, sb
+
+
+def _c4_describe(sb):
+    for c in sb["classes"]:
+        ch = c["channels"]
+        print(f"  S={c['subgrid_size']}: {c['nr_subgrids']} subgrids, T {int(c['metadata']['nr_timesteps'].min())}.."
+              f"{int(c['metadata']['nr_timesteps'].max())}, window {int(ch['nr_channels'].min())}.."
+              f"{int(ch['nr_channels'].max())} channels, {c['nr_visibilities']} selected visibilities")
+
+
+@pytest.mark.parametrize("precision", ["fp16x3", "bf16x3"])
+def test_c4_subset_single_launch_vs_fp64(c4, precision):
+    """The default single persistent launch per direction over the three classes of the real plan against the FP64
+    oracle: worst subgrid <= 5e-5 (gridder), per class <= 5e-5 (degridder); flagged samples left unwritten."""
+    _, sb = c4
+    _c4_describe(sb)
+    flags, C = sb["flags"], sb["wavenumbers"].shape[0]
+    vis0 = np.where(flags[..., None], 0, sb["visibilities"]).astype(np.complex64)
+    db = DeviceBatch(sb["classes"], precision=precision)
+    db.grid()
+    db.degrid()
+    assert db.batch.last_launches() == (1, 1)
+    vis = db.visibilities_out()
+    assert not np.any(vis[flags]), "flagged visibilities must not be written"
+    for k, c in enumerate(sb["classes"]):
+        got = db.subgrids(k)
+        eg = per_subgrid_rel_l2(got, oracle_grid(dict(c, visibilities=vis0), fp64=True))
+        v64 = oracle_degrid(dict(c, visibilities=vis0), got, fp64=True)
+        sel = _window_mask(c, sb["nr_rows"], C) & ~flags
+        ev = rel_l2(vis[sel], v64[sel])
+        print(f"C4 {precision} S={c['subgrid_size']}: gridder worst subgrid vs FP64 {eg.max():.3e}, degridder {ev:.3e}")
+        assert eg.max() <= BAR and ev <= BAR
+
+
+def test_c4_subset_plain_vs_cpu_restatement(c4):
+    """Plain FP32 on the real plan's classes, against the CPU restatement of the reference path with the flagged
+    samples zeroed (<= 1e-5 per subgrid / per class): the plain batch (one launch per direction, flagged samples
+    skipped) and the one-shot kernels per class (every sample of each window)."""
+    _, sb = c4
+    flags, C = sb["flags"], sb["wavenumbers"].shape[0]
+    vis0 = np.where(flags[..., None], 0, sb["visibilities"]).astype(np.complex64)
+    db = DeviceBatch(sb["classes"], precision="fp32")
+    db.grid()
+    db.degrid()
+    vis = db.visibilities_out()
+    assert not np.any(vis[flags]), "flagged visibilities must not be written"
+    for k, c in enumerate(sb["classes"]):
+        cw = dict(c, visibilities=vis0)
+        got = db.subgrids(k)
+        ref = oracle_grid(cw)
+        eg = per_subgrid_rel_l2(got, ref)
+        sel = _window_mask(c, sb["nr_rows"], C) & ~flags
+        ev = rel_l2(vis[sel], oracle_degrid(cw, got)[sel])
+        dw = DeviceWorkload(cw)
+        plan = dw.plan(precision="fp32")
+        dw.grid(plan)
+        one = dw.subgrids()
+        eo = per_subgrid_rel_l2(one, ref)
+        dw.degrid(plan)
+        eov = rel_l2(dw.visibilities_out()[sel], oracle_degrid(cw, one)[sel])
+        print(f"C4 plain S={c['subgrid_size']}: batch gridder worst subgrid vs CPU {eg.max():.3e}, degridder "
+              f"{ev:.3e}; one-shot {eo.max():.3e} / {eov:.3e}")
+        assert max(eg.max(), ev, eo.max(), eov) <= 1e-5
+
+
+def test_c4_device_planner_matches_the_host_planner(c4):
+    """idgx_plan_segments on the full C4 geometry (every ragged time segment of every class) gives the host
+    planner's metadata and channel windows bit for bit."""
+    from idgx import plan_segments_device
+
+    bo, _ = c4
+    cfg = bo["config"]
+    total = 0
+    for c in bo["classes"]:
+        md_d, ch_d = plan_segments_device(bo["uvw"], c["frequencies64"], *c["stations"], *c["segments"],
+                                          subgrid_size=int(c["subgrid_size"]), grid_size=cfg["grid_size"],
+                                          image_size=cfg["image_size"], nr_times=cfg["n_times"])
+        assert md_d.size == c["metadata"].size
+        assert md_d.tobytes() == c["metadata"].tobytes() and ch_d.tobytes() == c["channels"].tobytes()
+        total += md_d.size
+    print(f"C4 device plan: {total} subgrids in {len(bo['classes'])} classes, bit-identical to the host plan")
