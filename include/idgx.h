@@ -435,6 +435,13 @@ int idgx_batch_last_launches(const idgx_batch *batch, int *gridder_launches, int
 int idgx_batch_items(const idgx_batch *batch, int *gridder_items, int *degridder_items, int *gridder_workgroups,
                      int *degridder_workgroups);
 
+/* Work per thread of the plain FP32 one-shot kernels for a plan (idgx_grid / idgx_degrid with IDGX_PLAIN_FP32):
+ * gridder pixels per thread (4, or 3 where that leaves fewer lanes idle: S = 24, 48) and degridder channels per
+ * thread (4 without channel windows when the channel count is a multiple of 4, else 2).  They name the kernel
+ * instantiation a profile reports (gridder_plain<pixels, 1>, degridder_plain<channels, 1>).  No GPU needed. */
+int idgx_plain_kernel_shape(const idgx_plan *plan, int has_channel_windows, int *gridder_pixels_per_thread,
+                            int *degridder_channels_per_thread);
+
 /* Name of a status code. */
 const char *idgx_strerror(int status);
 

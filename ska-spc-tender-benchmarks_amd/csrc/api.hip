@@ -455,6 +455,19 @@ extern "C" int idgx_plan_segments(const idgx_planner *pl, const float *d_uvw, co
     return rc;
 }
 
+extern "C" int idgx_plain_kernel_shape(const idgx_plan *plan, int has_channel_windows, int *gridder_pixels_per_thread,
+                                       int *degridder_channels_per_thread)
+{
+    const int rc = check_plan_struct(plan);
+    if (rc != IDGX_OK)
+        return rc;
+    if (gridder_pixels_per_thread)
+        *gridder_pixels_per_thread = idgx::plain_grid_pixels_for(plan->subgrid_size);
+    if (degridder_channels_per_thread)
+        *degridder_channels_per_thread = idgx::plain_degrid_channels_for(has_channel_windows != 0, plan->nr_channels);
+    return IDGX_OK;
+}
+
 extern "C" const char *idgx_strerror(int status)
 {
     switch (status)

@@ -111,6 +111,8 @@ __host__ __device__ inline int degrid_row_stride(int S)
 
 hipError_t launch_grid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream);
 hipError_t launch_degrid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream);
+int plain_grid_pixels_for(int S);                                  // pixels per thread of launch_grid_plain
+int plain_degrid_channels_for(bool channel_windows, int nr_channels); // channels per thread of launch_degrid_plain
 
 // Persistent work-queue forms of the plain FP32 kernels (batches).  Shape i: gridder pixels per thread and
 // degridder visibilities per thread; kShapeTilesPerClass (gridder) gives every item its class's pixels per thread

@@ -796,9 +796,7 @@ template <int NPIX> hipError_t launch_grid_plain_pixels(const KernelArgs &a, boo
 // the last pixel still run the visibility loop, so idle lanes cost time; the per-pixel cost is the same for both.
 hipError_t launch_grid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream)
 {
-    const long long P = static_cast<long long>(a.p.S) * a.p.S;
-    const long long slots4 = (P + 1023) / 1024 * 1024, slots3 = (P + 767) / 768 * 768;
-    if (slots3 < slots4)
+    if (plain_grid_pixels_for(a.p.S) == 3)
         return launch_grid_plain_pixels<3>(a, accurate_sincos, stream);
     return launch_grid_plain_pixels<4>(a, accurate_sincos, stream);
 }
@@ -825,16 +823,23 @@ template <int NCH> hipError_t launch_degrid_plain_chunks(const KernelArgs &a, bo
 // (windows of 13.5 channels on average: 90 % of the lanes busy with four, 96.7 % with two) 82.3 and 79.6 ms.
 hipError_t launch_degrid_plain(const KernelArgs &a, bool accurate_sincos, hipStream_t stream)
 {
-    if (!a.channels && a.p.nr_channels % 4 == 0)
+    if (plain_degrid_channels_for(a.channels != nullptr, a.p.nr_channels) == 4)
         return launch_degrid_plain_chunks<4>(a, accurate_sincos, stream);
     return launch_degrid_plain_chunks<2>(a, accurate_sincos, stream);
 }
+
 // Pixels per gridder thread for subgrid size S: 4 unless 3 leaves fewer lanes idle (see launch_grid_plain).
 int plain_grid_pixels_for(int S)
 {
     const long long P = static_cast<long long>(S) * S;
     const long long slots4 = (P + 1023) / 1024 * 1024, slots3 = (P + 767) / 768 * 768;
     return slots3 < slots4 ? 3 : 4;
+}
+
+// Channels per degridder thread (see launch_degrid_plain).
+int plain_degrid_channels_for(bool channel_windows, int nr_channels)
+{
+    return !channel_windows && nr_channels % 4 == 0 ? 4 : 2;
 }
 
 int plain_shape_count()

@@ -182,6 +182,7 @@ def lib():
         L.idgx_batch_get_single_launch.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.idgx_batch_shape_info.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
         L.idgx_batch_plain_shape_info.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 2
+        L.idgx_plain_kernel_shape.argtypes = [ctypes.POINTER(Plan), ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 2
         L.idgx_batch_autotune.argtypes = [vp, ctypes.POINTER(BatchData), ctypes.c_int, vp,
                                           ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         L.idgx_batch_grid.argtypes = [vp, ctypes.POINTER(BatchData), vp]
@@ -440,6 +441,14 @@ def plan_segments_device(uvw, frequencies, station1, station2, seg_bl, seg_t0, s
     mdh = md.cpu().numpy().view(METADATA_DTYPE)[:k].copy()
     chh = ch.cpu().numpy().view(CHANNELS_DTYPE)[:k].copy()
     return mdh, chh
+
+
+def plain_kernel_shape(plan, has_channel_windows):
+    """(gridder pixels per thread, degridder channels per thread) of the plain FP32 one-shot kernels for `plan`."""
+    px, nch = ctypes.c_int(), ctypes.c_int()
+    _check(lib().idgx_plain_kernel_shape(ctypes.byref(plan), int(bool(has_channel_windows)), ctypes.byref(px),
+                                         ctypes.byref(nch)), "idgx_plain_kernel_shape")
+    return px.value, nch.value
 
 
 def _check(rc, what):

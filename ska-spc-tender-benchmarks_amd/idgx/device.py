@@ -281,6 +281,21 @@ class DeviceBatch:
         self.batch = Batch(specs)
         self.data = Batch.data(self.d_uvw, self.d_wavenumbers, self.d_visibilities, self.d_vis_out, self.classes)
 
+    def rebatch(self, precision):
+        """Replace the batch by one of another precision over the same HBM buffers (the old batch, and its device
+        memory, is released first: a mixed batch holds prepared pixels as large as the subgrids)."""
+        from . import Batch, PRECISIONS
+
+        self.batch.close()
+        specs = []
+        for c in self.classes:
+            plan = type(c["plan"]).from_buffer_copy(c["plan"])
+            plan.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+            c["plan"] = plan
+            w = c["w"]
+            specs.append((plan, w["metadata"], w.get("channels"), w.get("vis_offset")))
+        self.batch = Batch(specs)
+
     def grid(self, stream=None):
         self.batch.grid(self.data, stream)
 
