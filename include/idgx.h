@@ -232,6 +232,15 @@ int idgx_split_subgrids_layered(const idgx_plan *plan, const idgx_metadata *d_me
  * Calls that overlap on one device share the scratch, so issue them on one stream. */
 int idgx_grid_fft(const idgx_plan *plan, void *d_grid, int nr_layers, int direction, void *stream);
 
+/* idgx_grid_fft that allocates nothing (no hipMalloc / hipFree, no library plan), for callers that cannot allocate:
+ * code under hipGraph capture, tight serving loops, harnesses that forbid device allocation.  No counterpart in the
+ * reference.  Same kernels and results as idgx_grid_fft, bit for bit; its scratch is a fixed 128 MiB array in the
+ * library's device code (one copy per GPU, held from the library's load until the process ends), so G must be
+ * 1024, 2048 or 4096 (IDGX_ERR_UNSUPPORTED otherwise: larger planes do not fit the array, and other sizes would
+ * need rocFFT, which allocates).  Every call on a GPU shares that GPU's array: calls that overlap on one GPU (two
+ * streams, two host threads) race, so issue them on one stream. */
+int idgx_grid_fft_noalloc(const idgx_plan *plan, void *d_grid, int nr_layers, int direction, void *stream);
+
 /* The bounding box {x0, x1, y0, y1} of a plan's subgrids on the master grid, cells [x0, x1) x [y0, y1) clamped to
  * [0, G): the only cells the adder writes and the splitter reads.  {0, 0, 0, 0} for an empty plan. */
 int idgx_grid_box(const idgx_plan *plan, const idgx_metadata *h_metadata, int32_t box[4]);

@@ -321,6 +321,19 @@ extern "C" int idgx_grid_fft(const idgx_plan *plan, void *d_grid, int nr_layers,
                                            static_cast<hipStream_t>(stream)));
 }
 
+extern "C" int idgx_grid_fft_noalloc(const idgx_plan *plan, void *d_grid, int nr_layers, int direction, void *stream)
+{
+    const int rc = check_plan_struct(plan);
+    if (rc != IDGX_OK)
+        return rc;
+    if (nr_layers < 1 || !d_grid || (direction != IDGX_FFT_FORWARD && direction != IDGX_FFT_BACKWARD))
+        return IDGX_ERR_INVALID_ARGUMENT;
+    if (!idgx::grid_fft_noalloc_supported(plan->grid_size))
+        return IDGX_ERR_UNSUPPORTED; // G outside 1024, 2048, 4096: the scratch would not fit the fixed array
+    return to_status(idgx::launch_grid_fft_noalloc(d_grid, plan->grid_size, nr_layers, direction == IDGX_FFT_FORWARD,
+                                                   static_cast<hipStream_t>(stream)));
+}
+
 extern "C" int idgx_grid_box(const idgx_plan *plan, const idgx_metadata *h_metadata, int32_t box[4])
 {
     const int rc = check_plan_struct(plan);

@@ -605,6 +605,40 @@ hipError_t fft_scratch(long long elements, float2 *&out)
     return hipSuccess;
 }
 
+// Scratch of the allocation-free transform (idgx_grid_fft_noalloc): a file-scope array of fixed size, one copy per
+// GPU for the life of the process, big enough for one group of planes as stockham_fft2d groups them (64 MB, or one
+// plane when a plane is larger), i.e. planes of up to 4096 x 4096 points.  Its device address comes from
+// hipGetSymbolAddress, taken once per device with that device current; host code never uses the array's name as a
+// device address.  Every call on a GPU shares that GPU's array, so calls that overlap on one GPU race; no call needs
+// its contents to start at zero (column pass 2 writes every element that the row pass reads).
+constexpr long long kStaticScratchBytes = 128LL << 20;
+static __device__ __align__(256) float2 scratch[kStaticScratchBytes / sizeof(float2)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_static_scratch_mutex;
+float2 *g_static_scratch[kMaxDevices] = {};
+
+hipError_t static_scratch(float2 *&out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    if (dev < 0 || dev >= kMaxDevices)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_static_scratch_mutex);
+    if (!g_static_scratch[dev])
+    {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        g_static_scratch[dev] = static_cast<float2 *>(p);
+    }
+    out = g_static_scratch[dev];
+    return hipSuccess;
+}
+
 // Planes of gy rows of gx points (both powers of two in 1024 .. 16384), src -> dst (in place when equal).  Each
 // launch takes a group of planes, so that small planes still fill the chip; the scratch holds one group, at most
 // kScratchBytes (or one plane when a plane is larger).  Kept small enough that the scratch written by column pass 2
@@ -644,14 +678,24 @@ struct FftBox
     int x0 = 0, x1 = 0, y0 = 0, y1 = 0;
 };
 
+// no_alloc: the scratch is the file-scope array (the caller has checked that a group fits it) instead of a plane
+// allocated on first use.
 template <int SG>
 hipError_t stockham_fft2d(const float2 *src, float2 *dst, int gx, int gy, long long planes, bool centred,
-                          hipStream_t st, const FftBox &box = FftBox())
+                          hipStream_t st, const FftBox &box = FftBox(), bool no_alloc = false)
 {
     FftArgs A;
     const long long plane = static_cast<long long>(gx) * gy;
     const long long group = std::max(1LL, std::min(planes, kScratchBytes / (plane * static_cast<long long>(sizeof(float2)))));
-    hipError_t e = fft_scratch(group * plane, A.scratch);
+    hipError_t e = hipSuccess;
+    if (no_alloc)
+    {
+        if (group * plane * static_cast<long long>(sizeof(float2)) > kStaticScratchBytes)
+            return hipErrorInvalidValue;
+        e = static_scratch(A.scratch);
+    }
+    else
+        e = fft_scratch(group * plane, A.scratch);
     if (e != hipSuccess)
         return e;
     A.gx = gx;
@@ -864,6 +908,22 @@ hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward,
     if (hipfftExecC2C(plan, d, d, forward ? HIPFFT_BACKWARD : HIPFFT_FORWARD) != HIPFFT_SUCCESS)
         return hipErrorUnknown;
     return checkerboard(grid, planes, G, st);
+}
+
+bool grid_fft_noalloc_supported(int G)
+{
+    return stockham_size(G) && static_cast<long long>(G) * G * static_cast<long long>(sizeof(float2)) <=
+                                   kStaticScratchBytes;
+}
+
+hipError_t launch_grid_fft_noalloc(void *grid, int G, long long nr_layers, bool forward, hipStream_t st)
+{
+    const long long planes = nr_layers * kCorr;
+    if (!grid_fft_noalloc_supported(G) || planes > 0x7fffffffLL)
+        return hipErrorInvalidValue;
+    float2 *g = static_cast<float2 *>(grid);
+    return forward ? stockham_fft2d<1>(g, g, G, G, planes, true, st, FftBox(), true)
+                   : stockham_fft2d<-1>(g, g, G, G, planes, true, st, FftBox(), true);
 }
 
 bool fft2d_supported(int nx, int ny)

@@ -173,6 +173,10 @@ hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward,
 // Zeroes the cells a boxed BACKWARD transform reads (whole 64-column tiles around [x0, x1), every row; the whole
 // grid for sizes outside the three-pass kernels' range).
 hipError_t launch_grid_clear_box(void *grid, int G, long long nr_layers, const int *box, hipStream_t st);
+// The same transform (no box) with the scratch in a file-scope array instead of a plane allocated on first use:
+// G in 1024 .. 4096 (one 4096 x 4096 plane fills the array).
+bool grid_fft_noalloc_supported(int G);
+hipError_t launch_grid_fft_noalloc(void *grid, int G, long long nr_layers, bool forward, hipStream_t st);
 bool fft2d_supported(int nx, int ny);
 // Plain 2-D DFT of `batch` planes of ny rows of nx points, out = sum in exp(sign 2 pi i (...)), unnormalised;
 // in == out is in place.

@@ -156,6 +156,7 @@ def lib():
                                                     ctypes.POINTER(AdderPlan), vp, vp]
         L.idgx_split_subgrids_layered.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
         L.idgx_grid_fft.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp]
+        L.idgx_grid_fft_noalloc.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp]
         L.idgx_grid_fft_box.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_int32), vp]
         L.idgx_grid_box.argtypes = [ctypes.POINTER(Plan), vp, ctypes.POINTER(ctypes.c_int32)]
@@ -365,6 +366,12 @@ def grid_fft(plan, d_grid, nr_layers, direction, stream=None):
     """idgx_grid_fft: in-place centred 2-D transform of a master grid complex64 [nr_layers][4][G][G]."""
     _check(lib().idgx_grid_fft(ctypes.byref(plan), _dptr(d_grid), int(nr_layers), int(direction), _stream(stream)),
            "idgx_grid_fft")
+
+
+def grid_fft_noalloc(plan, d_grid, nr_layers, direction, stream=None):
+    """idgx_grid_fft without any device allocation (G = 1024, 2048 or 4096; the scratch is a fixed array)."""
+    _check(lib().idgx_grid_fft_noalloc(ctypes.byref(plan), _dptr(d_grid), nr_layers, direction, _stream(stream)),
+           "idgx_grid_fft_noalloc")
 
 
 def grid_box(plan, metadata):
