@@ -331,10 +331,12 @@ def time_form(runner, steps, warmup, stream):
 def time_cycle(runner, dw, steps, warmup, stream):
     """Average time (ms) of each stage of an imaging cycle on resident buffers: clear the master grid, grid, subgrid
     FFT, add to the grid, master-grid FFT to the image and back (deconvolution would sit between the two), split
-    from the grid, inverse subgrid FFT, degrid (HIP events on `stream`).  The master-grid transforms are restricted
-    to the subgrids' bounding box (idgx_grid_fft_box: the uv grid is zero outside it on the way to the image, and
-    only its rows are needed on the way back), and the clear zeroes only what the transform to the image reads
-    (idgx_grid_clear_box); the full transforms are timed beside the cycle."""
+    from the grid, inverse subgrid FFT, degrid (HIP events on `stream`).  A plan with w-layers adds every subgrid to
+    the grid of its layer, transforms every layer and combines the layers into one image with the w-screens (and
+    back), all on resident [layers][4][G][G] grids.  The master-grid transforms are restricted to the subgrids'
+    bounding box (idgx_grid_fft_box: the uv grids are zero outside it on the way to the image, and only its rows are
+    needed on the way back), and the clear zeroes only what the transform to the image reads (idgx_grid_clear_box);
+    the full transforms are timed beside the cycle."""
     import torch
 
     from idgx import FFT_BACKWARD, FFT_FORWARD, grid_box, grid_clear_box, grid_fft_box
@@ -342,16 +344,24 @@ def time_cycle(runner, dw, steps, warmup, stream):
     plan = dw.plan()
     box = grid_box(plan, dw.w["metadata"])
     d_sub, _ = runner.buffers()
-    d_grid = dw.grid_buffer()
-    stages = [("grid_clear", lambda: grid_clear_box(plan, d_grid, 1, box, stream)),
+    layered = float(dw.w.get("w_step") or 0.0) > 0.0
+    z_min, nl = dw.layers() if layered else (0, 1)
+    d_grid = dw.layer_buffer(nl) if layered else dw.grid_buffer()
+    d_image = dw.grid_buffer() if layered else None
+    stages = [("grid_clear", lambda: grid_clear_box(plan, d_grid, nl, box, stream)),
               ("gridder", lambda: runner.grid(stream)),
               ("subgrid_fft", lambda: dw.fft(plan, FFT_FORWARD, d_sub, stream)),
-              ("adder", lambda: dw.add(plan, d_grid, d_sub, stream)),
-              ("grid_fft_to_image", lambda: grid_fft_box(plan, d_grid, 1, FFT_BACKWARD, box, stream)),
-              ("grid_fft_to_uv", lambda: grid_fft_box(plan, d_grid, 1, FFT_FORWARD, box, stream)),
-              ("splitter", lambda: dw.split(plan, d_grid, d_sub, stream)),
-              ("subgrid_ifft", lambda: dw.fft(plan, FFT_BACKWARD, d_sub, stream)),
-              ("degridder", lambda: runner.degrid(stream))]
+              ("adder", (lambda: dw.add_layered(plan, d_grid, d_sub, stream)) if layered else
+               (lambda: dw.add(plan, d_grid, d_sub, stream))),
+              ("grid_fft_to_image", lambda: grid_fft_box(plan, d_grid, nl, FFT_BACKWARD, box, stream))]
+    if layered:
+        stages += [("wstack_to_image", lambda: dw.to_image(plan, d_grid, d_image, stream)),
+                   ("wstack_from_image", lambda: dw.from_image(plan, d_image, d_grid, stream))]
+    stages += [("grid_fft_to_uv", lambda: grid_fft_box(plan, d_grid, nl, FFT_FORWARD, box, stream)),
+               ("splitter", (lambda: dw.split_layered(plan, d_grid, d_sub, stream)) if layered else
+                (lambda: dw.split(plan, d_grid, d_sub, stream))),
+               ("subgrid_ifft", lambda: dw.fft(plan, FFT_BACKWARD, d_sub, stream)),
+               ("degridder", lambda: runner.degrid(stream))]
     with torch.cuda.stream(stream):
         for _ in range(warmup):
             for _, f in stages:
@@ -367,9 +377,9 @@ def time_cycle(runner, dw, steps, warmup, stream):
             torch.cuda.synchronize()
             for i in range(len(stages)):
                 acc[i] += ev[i].elapsed_time(ev[i + 1])
-        # The full transforms of the same grid, for comparison (not part of the cycle).
-        full = {"to_image": lambda: dw.grid_fft(plan, d_grid, 1, FFT_BACKWARD, stream),
-                "to_uv": lambda: dw.grid_fft(plan, d_grid, 1, FFT_FORWARD, stream)}
+        # The full transforms of the same grids, for comparison (not part of the cycle).
+        full = {"to_image": lambda: dw.grid_fft(plan, d_grid, nl, FFT_BACKWARD, stream),
+                "to_uv": lambda: dw.grid_fft(plan, d_grid, nl, FFT_FORWARD, stream)}
         full_ms = {}
         for name, f in full.items():
             f()
@@ -380,8 +390,8 @@ def time_cycle(runner, dw, steps, warmup, stream):
             e1.record(stream)
             torch.cuda.synchronize()
             full_ms[name] = e0.elapsed_time(e1) / steps
-    del d_grid
-    return {name: acc[i] / steps for i, (name, _) in enumerate(stages)}, box, full_ms
+    del d_grid, d_image
+    return {name: acc[i] / steps for i, (name, _) in enumerate(stages)}, box, full_ms, nl
 
 
 def time_pcie(nvis_bytes, sub_bytes, repeats=3):
@@ -762,11 +772,11 @@ def main():
     plain = max(plain_forms, key=lambda f: f["round_trip_mvis_s"])  # the faster plain FP32 form
     cycle = None
     if args.cycle or args.config == "full_cycle":
-        cycle, box, full_ms = time_cycle(runners[-1], dw, args.steps, args.warmup, stream)
+        cycle, box, full_ms, nl = time_cycle(runners[-1], dw, args.steps, args.warmup, stream)
         total = sum(cycle.values())
         cycle = {"stages_ms": cycle, "total_ms": total, "cycle_mvis_s": nvis / (total / 1e3) / 1e6,
                  "precision": head["precision"], "path": head["path"], "grid_fft_box": list(box),
-                 "full_grid_fft_ms": full_ms}
+                 "w_layers": nl, "full_grid_fft_ms": full_ms}
         print(f"bench: imaging cycle ({head['precision']} {head['path']}): " +
               ", ".join(f"{k} {v:.2f} ms" for k, v in cycle["stages_ms"].items()) +
               f"; total {total:.2f} ms; master-grid transforms on the box {list(box)} (full transforms: to image "

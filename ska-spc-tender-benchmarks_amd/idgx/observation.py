@@ -238,12 +238,15 @@ CONFIGS = {
     "ska_low": dict(n_stations=128, radius_m=20000.0, core_fraction=0.5, core_radius_m=1000.0, n_times=256,
                     f_lo=150e6, f_hi=180e6, n_channels=64, subgrid_size=32, timesteps_per_subgrid=128,
                     grid_size=4096, image_size=0.1, dec_deg=-30.0, zero_w=False, seed=2),
-    # Full imaging cycle: the small layout with the w-term, S=64 on a 16384^2 grid, Jones A-terms per station that
-    # change every 64 timesteps (the planner cuts subgrids at the A-term boundaries).
-    "full_cycle": dict(n_stations=10, radius_m=1000.0, core_fraction=0.0, core_radius_m=0.0, n_times=128,
-                       f_lo=150e6, f_hi=165e6, n_channels=16, subgrid_size=64, timesteps_per_subgrid=128,
-                       grid_size=16384, image_size=0.1, dec_deg=-30.0, zero_w=False, aterm_interval=64,
-                       jones=True, seed=6),
+    # Full imaging cycle: the 256-station layout of the batched configuration (same layout parameters and seed: half
+    # the stations in a 1 km core, the rest in a 20 km disk) with the small configuration's sampling (128 x 1 s, 16
+    # channels 150-165 MHz), pointed at declination -41 deg so that |w| reaches 5267 wavelengths (the 10-station
+    # 1 km layout cannot exceed ~1100); S=64 on a 16384^2 grid, image 0.1 rad, Jones A-terms per station changing
+    # every 64 timesteps (the planner cuts subgrids there), w-layers 2000 wavelengths apart (6 layers).
+    "full_cycle": dict(n_stations=256, radius_m=20000.0, core_fraction=0.5, core_radius_m=1000.0, n_times=128,
+                       f_lo=150e6, f_hi=165e6, n_channels=16, subgrid_size=64, timesteps_per_subgrid=64,
+                       grid_size=16384, image_size=0.1, dec_deg=-41.0, zero_w=False, aterm_interval=64,
+                       jones=True, w_step=2000.0, seed=4),
     # W-stacking: the small layout pointed 27 degrees from zenith (|w| up to ~500 wavelengths), 32-timestep
     # subgrids, w-layers 100 wavelengths apart.
     "wstack": dict(n_stations=10, radius_m=1000.0, core_fraction=0.0, core_radius_m=0.0, n_times=128,

@@ -2,8 +2,9 @@
 
 batched_uneven_small: three subgrid classes (S = 24 / 32 / 48 by baseline-length tercile) in one persistent launch,
 ragged per-baseline time chunks, ~30 % of the visibilities flagged through selection lists.
-full_cycle: S = 64 on a 16384^2 grid, direction-dependent Jones A-terms per station changing every 64 timesteps
-(two A-term slots; the planner cuts subgrids at the boundary), full w-term.
+full_cycle: the 256-station layout, S = 64 on a 16384^2 grid, direction-dependent Jones A-terms per station changing
+every 64 timesteps (two A-term slots; the planner cuts subgrids at the boundary), |w| up to ~5300 wavelengths on six
+w-layers.
 Bar: relative L2 <= 5e-5 vs FP64 per subgrid (gridder) and per class (degridder) for the hi/lo forms.
 """
 import numpy as np
@@ -62,55 +63,118 @@ def cycle():
     return observation.make_observation("full_cycle")
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp16x3"])
-def test_full_cycle_jones_vs_fp64(cycle, precision):
+def _max_w_lambda(w):
+    """Largest |w| of any sample the plan grids, in wavelengths (each subgrid's rows at its window's top channel)."""
+    md, ch = w["metadata"], w["channels"]
+    f = w["frequencies"].astype(np.float64)
+    out = 0.0
+    for s in range(md.size):
+        r0, nt = int(md["time_offset"][s]), int(md["nr_timesteps"][s])
+        fmax = f[int(ch["channel_begin"][s]) + int(ch["nr_channels"][s]) - 1]
+        out = max(out, float(np.abs(w["uvw"][r0:r0 + nt, 2]).max()) * fmax / observation.SPEED_OF_LIGHT)
+    return out
+
+
+@pytest.fixture(scope="module")
+def cycle_subset(cycle):
+    """About 150 subgrids of the full-cycle plan spread over it, plus the 16 whose samples reach the largest |w|, so
+    the subset spans the whole w range (every w-layer) at a size the FP64 oracle checks in seconds."""
     w = cycle
+    md = w["metadata"]
+    n_times = w["config"]["n_times"]
+    wabs = np.abs(w["uvw"][:, 2]).reshape(-1, n_times)
+    bl, t0 = md["time_offset"] // n_times, md["time_offset"] % n_times
+    wmax = np.array([wabs[b, t:t + 64].max() for b, t in zip(bl, t0)])
+    idx = np.unique(np.concatenate([np.linspace(0, md.size - 1, 150).astype(np.int64), np.argsort(wmax)[-16:]]))
+    return observation.subset(w, idx)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16x3"])
+def test_full_cycle_jones_vs_fp64(cycle_subset, precision):
+    """Full-cycle subgrids (S = 64, Jones A-terms in two slots, w-layers, |w| up to ~5300 wavelengths) against the
+    FP64 oracle: the mixed form <= 5e-5 worst subgrid / whole degridder output; the plain form against the CPU
+    restatement of the reference path (<= 1e-5), its FP64 error stated."""
+    w = cycle_subset
+    wmax = _max_w_lambda(w)
     assert w["nr_aterm_slots"] == 2 and set(np.unique(w["metadata"]["aterm_index"])) == {0, 1}
+    assert wmax >= 4000.0
     dw = DeviceWorkload(w)
     plan = dw.plan(precision=precision)
     dw.grid(plan)
     got = dw.subgrids()
-    g64 = oracle_grid(w, fp64=True)
-    eg = per_subgrid_rel_l2(got, g64)
+    eg = per_subgrid_rel_l2(got, oracle_grid(w, fp64=True))
     dw.degrid(plan)
-    ev = rel_l2(dw.visibilities_out(), oracle_degrid(w, got, fp64=True))
-    print(f"full_cycle {precision}: gridder worst subgrid vs FP64 {eg.max():.3e}, degridder {ev:.3e}")
-    assert eg.max() <= BAR and ev <= BAR
+    vis = dw.visibilities_out()
+    ev = rel_l2(vis, oracle_degrid(w, got, fp64=True))
+    z = w["metadata"]["z"]
+    print(f"full_cycle {precision}: {w['nr_subgrids']} subgrids, max |w| {wmax:.0f} wavelengths, w-layers "
+          f"{int(z.min())}..{int(z.max())}: gridder worst subgrid vs FP64 {eg.max():.3e}, degridder {ev:.3e}")
+    if precision == "fp32":
+        ec = per_subgrid_rel_l2(got, oracle_grid(w))
+        evc = rel_l2(vis, oracle_degrid(w, got))
+        print(f"full_cycle fp32 vs the CPU restatement: gridder worst subgrid {ec.max():.3e}, degridder {evc:.3e}")
+        assert ec.max() <= 1e-5 and evc <= 1e-5
+        assert eg.max() <= 1e-3  # FP32 phases on 40 km baselines: stated, not held to the mixed forms' bar
+    else:
+        assert eg.max() <= BAR and ev <= BAR
+
+
+def _vdot64(a, b):
+    """<a, b> = sum conj(a) b of two complex64 device tensors (flat float32 views), accumulated in FP64 a plane at a
+    time."""
+    import torch
+
+    a, b = a.view(torch.complex64), b.view(torch.complex64)
+    acc = 0j
+    step = 1 << 28
+    for i in range(0, a.numel(), step):
+        acc += complex(torch.vdot(a[i:i + step].to(torch.complex128), b[i:i + step].to(torch.complex128)).item())
+    return acc
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp16x3"])
 def test_full_cycle_adjoint_on_the_16k_grid(cycle, precision):
-    """<degrid(fft_b(split(G))), V> = <G, add(fft_f(grid(V)))> with the 4 x 16384^2 master grid in HBM; G is
-    random on the bounding box of the subgrids (the only part the splitter reads) and zero elsewhere."""
+    """The whole full-cycle plan (102,901 subgrids, |w| to ~5300 wavelengths) through the w-stacked imaging cycle on
+    six 16384^2 w-layers in HBM, and back: with M = to_image . grid_fft(BACKWARD) . layered add . subgrid FFT .
+    gridder and M^H = degridder . inverse subgrid FFT . layered split . grid_fft(FORWARD) . from_image,
+    <M^H I, V> = <I, M V> for a random image I, inner products accumulated in FP64 on the device.  Bar: 2e-5 for the
+    mixed form (FP64 phase reduction); 2e-4 for plain FP32, whose phases round as the reference's do (a large
+    phase offset minus k (u l + v m + w n), each rounded to FP32) and so differ from FP64 by up to 1.4e-4 per
+    subgrid on this plan's 40 km baselines (test_full_cycle_jones_vs_fp64), in the gridder and degridder
+    differently: the identity cannot hold closer than that."""
     import torch
 
+    from idgx import FFT_BACKWARD, FFT_FORWARD
+
     w = cycle
-    S, Gs = int(w["subgrid_size"]), int(w["grid_size"])
-    md = w["metadata"]
-    x0, x1 = int(md["x"].min()), int(md["x"].max()) + S
-    y0, y1 = int(md["y"].min()), int(md["y"].max()) + S
     dw = DeviceWorkload(w)
     plan = dw.plan(precision=precision)
+    z_min, nl = dw.layers()
+    G = int(w["grid_size"])
+    print(f"full_cycle adjoint {precision}: {w['nr_subgrids']} subgrids, {nl} w-layers from z = {z_min}, "
+          f"G = {G}", flush=True)
     dw.grid(plan)
     dw.fft(plan, FFT_FORWARD)
-    d_grid = dw.grid_buffer()
-    dw.add(plan, d_grid)
-    g = d_grid.view(torch.complex64).view(4, Gs, Gs)
-    gV = g[:, y0:y1, x0:x1].cpu().numpy().astype(np.complex128)
-    outside = float(g.abs().sum()) - float(np.abs(gV).sum())
-    rng = np.random.default_rng(31)
-    box = (rng.standard_normal(gV.shape) + 1j * rng.standard_normal(gV.shape)).astype(np.complex64)
-    g.zero_()
-    g[:, y0:y1, x0:x1] = torch.from_numpy(box).to(g.device)
-    dw.split(plan, d_grid)
+    d_layers = dw.layer_buffer(nl)
+    dw.add_layered(plan, d_layers)
+    dw.grid_fft(plan, d_layers, nl, FFT_BACKWARD)
+    d_image = dw.grid_buffer()
+    dw.to_image(plan, d_layers, d_image)
+    gen = torch.Generator(device=d_image.device)
+    gen.manual_seed(41)
+    d_rand = torch.randn(4 * G * G * 2, generator=gen, device=d_image.device, dtype=torch.float32)
+    rhs = _vdot64(d_rand, d_image.view(torch.float32))
+    del d_image
+    dw.from_image(plan, d_rand, d_layers)
+    dw.grid_fft(plan, d_layers, nl, FFT_FORWARD)
+    dw.split_layered(plan, d_layers)
+    del d_layers
     dw.fft(plan, FFT_BACKWARD)
     dw.degrid(plan)
-    dG = dw.visibilities_out().astype(np.complex128)
-    lhs = np.vdot(dG, w["visibilities"].astype(np.complex128))
-    rhs = np.vdot(box.astype(np.complex128), gV)
-    print(f"full_cycle adjoint {precision}: rel diff {abs(lhs - rhs) / abs(rhs):.2e}; box {y1 - y0}x{x1 - x0}")
-    assert abs(outside) <= 1e-3 * float(np.abs(gV).sum()), "the adder wrote outside the subgrid footprints"
-    assert abs(lhs - rhs) <= 2e-5 * abs(rhs)
+    lhs = _vdot64(dw.d_vis_out.view(torch.float32), dw.d_visibilities.view(torch.float32))
+    rel = abs(lhs - rhs) / abs(rhs)
+    print(f"full_cycle adjoint {precision}: <M^H I, V> = {lhs:.6e}, <I, M V> = {rhs:.6e}, rel diff {rel:.2e}")
+    assert rel <= (2e-4 if precision == "fp32" else 2e-5)
 
 
 def test_device_selection_lists_match_the_host_lists(uneven):

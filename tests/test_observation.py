@@ -93,6 +93,27 @@ def test_full_cycle_plan_cuts_at_aterm_boundaries():
     w = observation.make_observation("full_cycle", visibilities=False)
     md = w["metadata"]
     t_local = md["time_offset"] % w["config"]["n_times"]
-    assert w["nr_aterm_slots"] == 2 and w["aterms"].shape == (2, 10, 64, 64, 4)
+    assert w["nr_aterm_slots"] == 2 and w["aterms"].shape == (2, 256, 64, 64, 4)
     assert (md["aterm_index"] == t_local // 64).all() and (md["nr_timesteps"] == 64).all()
     assert (md["x"] >= 0).all() and (md["x"] + 64 <= 16384).all()
+
+
+def test_full_cycle_reaches_its_w_range_with_w_layers():
+    """The full-cycle configuration's |w| reaches +-5000 wavelengths (at the top channel), every subgrid's w-layer
+    puts the middle of its w range within w_step / 2 of its phase reference (z + 0.5) w_step, and the layers span
+    the w range."""
+    w = observation.make_observation("full_cycle", visibilities=False)
+    md, ch = w["metadata"], w["channels"]
+    f = w["frequencies"].astype(np.float64)
+    w_lambda = np.abs(w["uvw"][:, 2].astype(np.float64)) * f.max() / observation.SPEED_OF_LIGHT
+    assert w_lambda.max() >= 5000.0
+    step = w["w_step"]
+    n_times = w["config"]["n_times"]
+    uvw = w["uvw"].reshape(-1, n_times, 3).astype(np.float64)
+    bl, t0 = md["time_offset"] // n_times, md["time_offset"] % n_times
+    for s in np.linspace(0, md.size - 1, 500).astype(int):
+        ws = uvw[bl[s], t0[s]:t0[s] + md["nr_timesteps"][s], 2]
+        fc = 0.5 * (f[ch["channel_begin"][s]] + f[ch["channel_begin"][s] + ch["nr_channels"][s] - 1])
+        mid = 0.5 * (ws.min() + ws.max()) * fc / observation.SPEED_OF_LIGHT
+        assert abs(mid - (md["z"][s] + 0.5) * step) <= 0.5 * step + 1e-6
+    assert md["z"].min() * step <= -5000.0 + step and (md["z"].max() + 1) * step >= 5000.0 - step
