@@ -13,9 +13,11 @@
 // for the degridder.  Items are built on the host from the plan (metadata, channel windows and visibility
 // selections give every subgrid's visibility count V_s), costed as visibilities x pixels, and sorted heaviest
 // first (longest-processing-time order), so the uneven tail (flagging, ragged timesteps) finishes together.
-// Persistent workgroups (as many as fit on the chip) take items through an atomic counter per class and
-// direction, which a one-workgroup kernel clears before the launches (no memset nodes, so the sequence can be
-// captured).
+// Persistent workgroups (as many as fit on the chip) take items through atomic counters, which a one-workgroup
+// kernel clears before the launches (no memset nodes, so the sequence can be captured).  Each queue is laid out as
+// one queue per XCD: a subgrid's items all go to one of them (subgrids balanced over the queues by cost), the
+// workgroups of an XCD take their own queue first and the others' when it is empty, so a subgrid's prepared pixels
+// (degridder) or visibilities (gridder), which every item of the subgrid stages again, are fetched into one L2.
 //
 // Degridder pixels: before the persistent degridder, one workgroup per subgrid applies taper and A-terms and
 // splits the pixels into the packed words the degridder stages (32 B per pixel slot), so the work items copy
@@ -57,7 +59,11 @@ struct idgx_batch
     int last_launches[2] = {}; // persistent launches the latest run of each direction issued
     int single_workgroups[2] = {};
     size_t single_offset[2] = {};
-    int *d_counters = nullptr; // [direction][class]
+    // Per-XCD queues: item k of queue j of a class slice (or of the merged queue) is at
+    // [qbegin[j] + k]; counters [direction][class][queue] (the merged queue uses class 0's).
+    int qbegin[2][idgx::kMaxClasses][idgx::kQueues + 1] = {};
+    int single_qbegin[2][idgx::kQueues + 1] = {};
+    int *d_counters = nullptr;
     // Degridder pixels prepared once per subgrid and pass (per class): split words, FP16 scales, slot n table.
     uint4 *d_prepared[idgx::kMaxClasses] = {};
     int *d_prepared_scale[idgx::kMaxClasses] = {};
@@ -183,8 +189,56 @@ std::vector<Item> build_items(const idgx_batch *b, bool gridder, int k, int shap
                                  static_cast<double>(std::min(per, V - v0)) * static_cast<double>(P)});
         }
     }
-    std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
     return items;
+}
+
+// Lays items (generated subgrid by subgrid) out as kQueues queues: each subgrid's items go to one queue, subgrids
+// taken heaviest first to the least loaded queue (greedy longest-processing-time), and every queue heaviest first
+// (stable: a subgrid's equal items stay together).  flat receives the queues back to back, begin their offsets.
+void layout_queues(const std::vector<Item> &items, std::vector<uint2> &flat, int begin[idgx::kQueues + 1])
+{
+    struct Group
+    {
+        size_t first, count;
+        double cost;
+    };
+    std::vector<Group> groups;
+    for (size_t i = 0; i < items.size();)
+    {
+        size_t j = i;
+        double c = 0.0;
+        while (j < items.size() && items[j].v.x == items[i].v.x)
+            c += items[j++].cost;
+        groups.push_back({i, j - i, c});
+        i = j;
+    }
+    std::vector<size_t> order(groups.size());
+    for (size_t g = 0; g < order.size(); ++g)
+        order[g] = g;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return groups[x].cost > groups[y].cost; });
+    double load[idgx::kQueues] = {};
+    std::vector<Item> queue[idgx::kQueues];
+    for (const size_t g : order)
+    {
+        int best = 0;
+        for (int q = 1; q < idgx::kQueues; ++q)
+            if (load[q] < load[best])
+                best = q;
+        load[best] += groups[g].cost;
+        queue[best].insert(queue[best].end(), items.begin() + static_cast<long>(groups[g].first),
+                           items.begin() + static_cast<long>(groups[g].first + groups[g].count));
+    }
+    flat.clear();
+    flat.reserve(items.size());
+    for (int q = 0; q < idgx::kQueues; ++q)
+    {
+        std::stable_sort(queue[q].begin(), queue[q].end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
+        begin[q] = static_cast<int>(flat.size());
+        for (const Item &it : queue[q])
+            flat.push_back(it.v);
+    }
+    begin[idgx::kQueues] = static_cast<int>(flat.size());
 }
 
 // Queue slice of class k: room for its items under any shape (the counts build_items gives).
@@ -214,9 +268,8 @@ int upload_queue(idgx_batch *b, bool gridder, int k, int shape)
     const std::vector<Item> items = build_items(b, gridder, k, shape);
     if (items.size() > b->capacity[dir][k] || items.size() > 0x7fffffffULL)
         return IDGX_ERR_INVALID_ARGUMENT;
-    std::vector<uint2> flat(items.size());
-    for (size_t i = 0; i < items.size(); ++i)
-        flat[i] = items[i].v;
+    std::vector<uint2> flat;
+    layout_queues(items, flat, b->qbegin[dir][k]);
     if (!flat.empty())
     {
         const hipError_t e = hipMemcpy(b->d_items[dir] + b->offset[dir][k], flat.data(), flat.size() * sizeof(uint2),
@@ -237,7 +290,7 @@ bool single_mode(const idgx_batch *b, int dir)
 }
 
 // Merged queue of every class under one shape (kShapeTilesPerClass: each class's items cut by its own gridder
-// shape's tiles per wave), heaviest first across classes (stable: ties keep class order).
+// shape's tiles per wave), laid out as per-XCD queues over all classes' subgrids, heaviest first.
 int upload_single_queue(idgx_batch *b, bool gridder, int shape)
 {
     const int dir = gridder ? 0 : 1;
@@ -250,15 +303,13 @@ int upload_single_queue(idgx_batch *b, bool gridder, int shape)
         if (per_class)
             b->single_tiles[k] = class_tiles(b, b->shape[0][k]);
     }
-    std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
     size_t room = 0;
     for (int k = 0; k < b->nr_classes; ++k)
         room += b->capacity[dir][k];
     if (items.size() > room || items.size() > 0x7fffffffULL)
         return IDGX_ERR_INVALID_ARGUMENT;
-    std::vector<uint2> flat(items.size());
-    for (size_t i = 0; i < items.size(); ++i)
-        flat[i] = items[i].v;
+    std::vector<uint2> flat;
+    layout_queues(items, flat, b->single_qbegin[dir]);
     if (!flat.empty())
     {
         const hipError_t e = hipMemcpy(b->d_items[dir] + b->single_offset[dir], flat.data(),
@@ -327,7 +378,8 @@ int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st, i
     int rc = make_queue_args(b, d, gridder, q);
     if (rc != IDGX_OK)
         return rc;
-    hipError_t e = idgx::launch_reset_counters(b->d_counters + dir * idgx::kMaxClasses, idgx::kMaxClasses, st);
+    constexpr int kDirCounters = idgx::kMaxClasses * idgx::kQueues;
+    hipError_t e = idgx::launch_reset_counters(b->d_counters + dir * kDirCounters, kDirCounters, st);
     b->last_launches[dir] = 0;
     if (only < 0 && single_mode(b, dir))
     {
@@ -337,8 +389,9 @@ int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st, i
         if (e != hipSuccess || b->single_items[dir] == 0)
             return to_status(e);
         q.items = b->d_items[dir] + b->single_offset[dir];
-        q.nr_items = b->single_items[dir];
-        q.counter = b->d_counters + dir * idgx::kMaxClasses;
+        std::memcpy(q.queue_begin, b->single_qbegin[dir], sizeof(q.queue_begin));
+        q.nr_queues = idgx::kQueues;
+        q.counter = b->d_counters + dir * kDirCounters;
         for (int k = 0; k < idgx::kMaxClasses; ++k)
             q.tiles[k] = b->single_tiles[k];
         e = launch_queue(b, q, gridder, b->single_shape[dir], b->single_workgroups[dir], st);
@@ -354,8 +407,9 @@ int run(idgx_batch *b, const idgx_batch_data *d, bool gridder, hipStream_t st, i
         if (e != hipSuccess)
             break;
         q.items = b->d_items[dir] + b->offset[dir][k];
-        q.nr_items = b->nr_items[dir][k];
-        q.counter = b->d_counters + dir * idgx::kMaxClasses + k;
+        std::memcpy(q.queue_begin, b->qbegin[dir][k], sizeof(q.queue_begin));
+        q.nr_queues = idgx::kQueues;
+        q.counter = b->d_counters + dir * kDirCounters + k * idgx::kQueues;
         e = launch_queue(b, q, gridder, b->shape[dir][k], b->workgroups[dir][k], st);
         b->last_launches[dir] += e == hipSuccess;
     }
@@ -426,7 +480,7 @@ extern "C" int idgx_batch_create(const idgx_batch_class *classes, int nr_classes
         e = hipMalloc(&b->d_items[dir], std::max<size_t>(1, 2 * total) * sizeof(uint2));
     }
     if (e == hipSuccess)
-        e = hipMalloc(&b->d_counters, 2 * idgx::kMaxClasses * sizeof(int));
+        e = hipMalloc(&b->d_counters, 2 * idgx::kMaxClasses * idgx::kQueues * sizeof(int));
     for (int k = 0; k < nr_classes && e == hipSuccess && !plain(b); ++k)
     {
         const int S = b->plans[k].subgrid_size;

@@ -35,31 +35,52 @@ struct KernelArgs
     const float *slot_n;
 };
 
-// Persistent-kernel work over up to kMaxClasses subgrid classes (one subgrid size each): the queue holds
-// (class << 28 | subgrid, block) items, heaviest first; workgroups take items with an atomic counter.
+// Persistent-kernel work over up to kMaxClasses subgrid classes (one subgrid size each): the items are
+// (class << 28 | subgrid, block), in nr_queues queues (items[queue_begin[k] .. queue_begin[k + 1])), each heaviest
+// first; workgroups take items with an atomic counter per queue.  Queue k is the home of workgroups
+// k, k + nr_queues, ...: with nr_queues = the XCD count, the workgroups of one XCD (the hardware dispatches them
+// round-robin) share a queue, and every item of a subgrid is in one queue, so a subgrid's data is fetched into one
+// XCD's L2 only.  A workgroup whose home queue is empty takes items from the others, so the queues drain together.
 constexpr int kMaxClasses = 4;
+constexpr int kQueues = 8; // XCDs of the MI355X
 struct QueueArgs
 {
     KernelArgs cls[kMaxClasses];
     const uint2 *items;
-    int nr_items;
-    int *counter;           // zeroed by a reset kernel before every launch
+    int queue_begin[kQueues + 1];
+    int nr_queues;
+    int *counter;           // [nr_queues], zeroed by a reset kernel before every launch
     int tiles[kMaxClasses]; // gridder tiles per wave of each class (shape kShapeTilesPerClass only)
 };
 
-// Takes the next queue item (all threads of the workgroup see the same one); returns false when the queue is
-// exhausted.  The counter only grows, so every workgroup leaves after at most nr_items + 1 attempts in total.
+// Takes the next queue item (all threads of the workgroup see the same one); returns false when every queue is
+// exhausted.  Counters only grow, and a workgroup tries each queue at most once per call, so every workgroup leaves
+// after a bounded number of calls.
 __device__ __forceinline__ bool next_queue_item(const QueueArgs &q, int &cls, int &s, int &blk)
 {
     __shared__ int s_item;
     if (threadIdx.x == 0)
-        s_item = atomicAdd(q.counter, 1);
+    {
+        int item = -1;
+        const int home = static_cast<int>(blockIdx.x % static_cast<unsigned>(q.nr_queues));
+        for (int k = 0; k < q.nr_queues && item < 0; ++k)
+        {
+            const int qi = home + k < q.nr_queues ? home + k : home + k - q.nr_queues;
+            const int size = q.queue_begin[qi + 1] - q.queue_begin[qi];
+            if (size == 0)
+                continue;
+            const int i = atomicAdd(q.counter + qi, 1);
+            if (i < size)
+                item = q.queue_begin[qi] + i;
+        }
+        s_item = item;
+    }
     __syncthreads();
     // The same value in every lane: read it into a scalar register, so that the item and the class's arguments
     // (QueueArgs::cls[cls]) are scalar loads rather than per-lane copies held in vector registers.
     const int i = __builtin_amdgcn_readfirstlane(s_item);
     __syncthreads();
-    if (i >= q.nr_items)
+    if (i < 0)
         return false;
     const uint2 it = q.items[i];
     cls = __builtin_amdgcn_readfirstlane(static_cast<int>(it.x >> 28));

@@ -612,7 +612,10 @@ __device__ __forceinline__ void gridder_item(const KernelArgs &a, int s, int pbl
 // through LDS PB at a time (taper and A-term applied, split into B fragments); the accumulators stay in
 // registers across batches, so each visibility is written once.
 // ---------------------------------------------------------------------------------------------------------------
-template <int DT, bool SPLIT, int VT, int PB, bool PM>
+// PREP: the pixels come prepared (a.prepared, a.prepared_scale, a.slot_n: the persistent kernels, whose batch
+// prepares them before every pass); otherwise the item applies taper and A-terms as it stages them (the one-shot
+// kernel).  A compile-time choice, so the persistent kernel carries no registers for the staging it never runs.
+template <int DT, bool SPLIT, int VT, int PB, bool PM, bool PREP>
 __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int vblk, DegridderShared<VT, PB> &sm)
 {
     constexpr int kChunk = kWaves * VT * 16;
@@ -663,7 +666,7 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
         acc[t] = (f32x4){0.F, 0.F, 0.F, 0.F};
     const bool wave_live = v0 + wave * VT * 16 < V; // wave-uniform
     int scale_exp = -126; // FP16: current power-of-two scale of the staged B operand
-    if (DT == kFP16 && a.prepared)
+    if constexpr (DT == kFP16 && PREP)
         scale_exp = a.prepared_scale[s];
 
     // Pixel slots: rows of Sp = S rounded up to a multiple of 4, so each lane group's 4 slots share a row.
@@ -673,7 +676,7 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
         const int np = min(PB, nslots - pb);
         __syncthreads();
         constexpr int kPer = (PB + kThreads - 1) / kThreads;
-        if (a.prepared)
+        if constexpr (PREP)
         {
             // Prepared pixels: copy the split words and the slot coordinates into LDS.
             const uint4 *src = a.prepared + (static_cast<size_t>(s) * nslots + pb) * 2;
@@ -903,7 +906,7 @@ __global__ __launch_bounds__(kThreads, (degridder_min_blocks<VT, PB>())) void de
 {
     __shared__ DegridderShared<VT, PB> sm;
     const int vblk = blockIdx.x / a.p.nr_subgrids;
-    degridder_item<DT, SPLIT, VT, PB, PM>(a, blockIdx.x - vblk * a.p.nr_subgrids, vblk, sm);
+    degridder_item<DT, SPLIT, VT, PB, PM, false>(a, blockIdx.x - vblk * a.p.nr_subgrids, vblk, sm);
 }
 
 template <int DT, bool SPLIT, int PT, int VB, bool PM>
@@ -956,7 +959,7 @@ __global__ __launch_bounds__(kThreads, (degridder_min_blocks<VT, PB>())) void de
     __shared__ DegridderShared<VT, PB> sm;
     int cls, s, blk;
     while (next_queue_item(q, cls, s, blk))
-        degridder_item<DT, SPLIT, VT, PB, PM>(q.cls[cls], s, blk, sm);
+        degridder_item<DT, SPLIT, VT, PB, PM, true>(q.cls[cls], s, blk, sm);
 }
 
 __global__ void reset_counters_kernel(int *counters, int n)
@@ -1251,6 +1254,10 @@ hipError_t launch_queue(const QueueArgs &q, int precision, bool gridder, int sha
     void *fn = queue_kernel_for(precision, gridder, shape);
     if (!fn || workgroups <= 0)
         return hipErrorInvalidValue;
+    // The persistent degridder reads prepared pixels only (launch_degrid_prepare fills them first).
+    for (int k = 0; k < kMaxClasses && !gridder; ++k)
+        if (q.cls[k].p.nr_subgrids > 0 && (!q.cls[k].prepared || !q.cls[k].prepared_scale || !q.cls[k].slot_n))
+            return hipErrorInvalidValue;
     QueueArgs arg = q;
     void *args[] = {&arg};
     return hipLaunchKernel(fn, dim3(workgroups), dim3(kThreads), args, 0, st);
