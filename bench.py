@@ -256,7 +256,10 @@ class Batched:
         else:
             db.rebatch(precision)
             self.db = db
+        t0 = time.perf_counter()
+        print(f"bench: autotuning the {precision} batch", file=sys.stderr, flush=True)
         tg, td = self.db.batch.autotune(self.db.data, repeats=2, stream=stream)
+        print(f"bench: {precision} batch autotuned in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
         self.describe(tg, td)
 
     def describe(self, tg, td):

@@ -4,6 +4,7 @@ any counters per kernel.
     python -m idgx.pmc_summary --cal-fetch DIR --cal-write DIR --fetch DIR --write DIR [--sq DIR] \
         --cal-mib 1024 --out profiles/hbm_traffic_ska_low
     python -m idgx.pmc_summary --counters DIR [DIR ...] --out profiles/pipe_utilisation_ska_low [--title ...]
+    python -m idgx.pmc_summary --stats DIR --out profiles/gridder_degridder_ska_low [--title ...] [--notes ...]
 
 FETCH_SIZE / WRITE_SIZE are read in KiB per dispatch.  The calibration runs are a plain device-to-device copy of
 --cal-mib MiB (idgx.calibrate); scale factor = true bytes / counter bytes of that copy, applied to the kernels'
@@ -17,8 +18,11 @@ import json
 import os
 import sys
 
-KERNELS = ["gridder_plain", "degridder_plain", "gridder_mfma_queue", "degridder_mfma_queue", "degrid_prepare_kernel", "gridder_mfma",
-           "degridder_mfma", "subgrid_fft_kernel", "adder_kernel", "splitter_kernel"]
+KERNELS = ["gridder_plain_queue_tiles", "gridder_plain_queue", "degridder_plain_queue", "gridder_plain",
+           "degridder_plain", "gridder_mfma_queue_tiles", "gridder_mfma_queue", "degridder_mfma_queue",
+           "degrid_prepare_kernel", "gridder_mfma", "degridder_mfma", "subgrid_fft_kernel", "adder_kernel",
+           "splitter_kernel", "fft_col_kernel", "fft_row_kernel", "copy_box_kernel", "clear_band_kernel",
+           "checkerboard_kernel", "wstack_to_image_kernel", "wstack_from_image_kernel", "reset_counters_kernel"]
 
 
 def short_name(name):
@@ -86,8 +90,38 @@ def counter_table(dirs, out, title, command):
     return 0
 
 
+def stats_table(d, out, title, command, notes=""):
+    """rocprofv3 --kernel-trace --stats output (the kernel_stats.csv of run directory d): calls, mean / min / max
+    duration and share per kernel, this library's kernels by family and template arguments, others by name."""
+    paths = glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True)
+    if not paths:
+        raise SystemExit(f"no kernel_stats.csv under {d}")
+    rows = []
+    for r in csv.DictReader(open(paths[0])):
+        name = short_name(r["Name"]) or r["Name"].split("(")[0].replace("void ", "")[:60]
+        rows.append((name, int(r["Calls"]), float(r["AverageNs"]) / 1e6, float(r["MinNs"]) / 1e6,
+                     float(r["MaxNs"]) / 1e6, float(r["Percentage"])))
+    rows.sort(key=lambda x: -x[5])
+    lines = [f"# {title}", ""]
+    if notes:
+        lines += [notes, ""]
+    if command:
+        lines += ["Command (run from /tmp with TMPDIR=/tmp, repository at $R):", "", "```", command, "```", ""]
+    lines += ["| kernel | calls | average (ms) | min (ms) | max (ms) | share |", "|---|---|---|---|---|---|"]
+    for name, calls, avg, mn, mx, pct in rows:
+        if pct < 0.01:
+            continue
+        lines.append(f"| `{name}` | {calls} | {avg:.3f} | {mn:.3f} | {mx:.3f} | {pct:.2f} % |")
+    with open(out + ".md", "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--stats", help="tabulate a rocprofv3 --kernel-trace --stats run directory")
+    ap.add_argument("--notes", default="", help="text placed under the title")
     for k in ("cal-fetch", "cal-write", "fetch", "write"):
         ap.add_argument("--" + k)
     ap.add_argument("--sq")
@@ -97,6 +131,8 @@ def main():
     ap.add_argument("--out", required=True)
     ap.add_argument("--command", default="")
     a = ap.parse_args()
+    if a.stats:
+        return stats_table(a.stats, a.out, a.title, a.command, a.notes)
     if a.counters:
         return counter_table(a.counters, a.out, a.title, a.command)
     if not (a.cal_fetch and a.cal_write and a.fetch and a.write):

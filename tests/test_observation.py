@@ -117,3 +117,57 @@ def test_full_cycle_reaches_its_w_range_with_w_layers():
         mid = 0.5 * (ws.min() + ws.max()) * fc / observation.SPEED_OF_LIGHT
         assert abs(mid - (md["z"][s] + 0.5) * step) <= 0.5 * step + 1e-6
     assert md["z"].min() * step <= -5000.0 + step and (md["z"].max() + 1) * step >= 5000.0 - step
+
+
+def test_earth_rotation_uvw_invariants():
+    """uvw of earth-rotation synthesis: a rotation of the baseline, so |uvw| = |baseline| at every hour angle; at
+    transit u = east, v = north cos(dec - lat) - up sin(dec - lat), w = north sin(dec - lat) + up cos(dec - lat)
+    (the source's direction cosines in the local frame); a baseline's (u, v) for a source at the pole turns by the
+    earth's rotation per second."""
+    rng = np.random.default_rng(11)
+    enu = rng.normal(0.0, 3000.0, (9, 3))
+    s1, s2 = observation.baseline_pairs(9)
+    b = enu[s2] - enu[s1]
+    lat = np.radians(observation.SITE_LATITUDE_DEG)
+    for dec_deg in (-30.0, -41.0, 10.0, -89.0):
+        uvw = observation.earth_rotation_uvw(enu, s1, s2, 101, dec_deg)
+        assert np.allclose(np.linalg.norm(uvw, axis=-1), np.linalg.norm(b, axis=-1)[:, None], rtol=1e-12)
+        mid = uvw[:, 50]  # hour angle 0: the middle of 101 steps centred on transit
+        d = np.radians(dec_deg) - lat
+        assert np.allclose(mid[:, 0], b[:, 0], atol=1e-9)
+        assert np.allclose(mid[:, 1], b[:, 1] * np.cos(d) - b[:, 2] * np.sin(d), atol=1e-9)
+        assert np.allclose(mid[:, 2], b[:, 1] * np.sin(d) + b[:, 2] * np.cos(d), atol=1e-9)
+    pole = observation.earth_rotation_uvw(enu, s1, s2, 2, -90.0)
+    ang = np.arctan2(pole[:, 1, 1], pole[:, 1, 0]) - np.arctan2(pole[:, 0, 1], pole[:, 0, 0])
+    ang = (ang + np.pi) % (2 * np.pi) - np.pi
+    assert np.allclose(np.abs(ang), observation.EARTH_ROTATION_RAD_S, rtol=1e-9)
+    assert np.allclose(pole[:, 0, 2], pole[:, 1, 2])  # w is constant for a source at the pole
+
+
+def test_prolate_spheroidal_taper_invariants():
+    """The taper is the outer product of a 1-D profile that is symmetric about the subgrid centre, largest at the
+    two central pixels (1 at nu = 0 by normalisation), decreasing towards the edge and non-negative; and that
+    profile is the zeroth prolate spheroidal wave function of bandwidth c = 3 pi: an eigenfunction of the finite
+    Fourier transform on [-1, 1], int exp(i c x t) psi(t) dt = lambda psi(x) (checked by Gauss-Legendre
+    quadrature on the profile's own formula)."""
+    from scipy.special import pro_ang1
+
+    for S in (16, 24, 32, 48, 64):
+        t = observation.prolate_spheroidal_taper(S).astype(np.float64)
+        prof = t[S // 2]
+        assert np.allclose(t, np.outer(prof, prof) / prof[S // 2], rtol=1e-6)
+        assert np.allclose(prof, prof[::-1], rtol=1e-6)
+        half = prof[S // 2:]
+        assert (np.diff(half) <= 1e-7).all() and (prof >= 0).all()
+        assert 0.95 < half[0] <= 1.0 and half[-1] < 0.05
+    c = 3.0 * np.pi
+    xq, wq = np.polynomial.legendre.leggauss(200)
+
+    def psi(x):
+        return pro_ang1(0, 0, c, np.clip(x, -1 + 1e-12, 1 - 1e-12))[0] / pro_ang1(0, 0, c, 0.0)[0]
+
+    x = np.linspace(-0.95, 0.95, 39)
+    ft = (np.exp(1j * c * np.outer(x, xq)) * (wq * psi(xq))).sum(axis=1)
+    lam = ft[19] / psi(x[19])
+    assert np.allclose(ft, lam * psi(x), atol=1e-8 * abs(lam))
+    assert np.isclose(lam.imag, 0.0, atol=1e-10) and lam.real > 0
