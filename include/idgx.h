@@ -271,17 +271,27 @@ int idgx_grid_clear_box(const idgx_plan *plan, void *d_grid, int nr_layers, cons
  * run by fft/src/fft_benchmark.cpp:240-345): for b < batch,
  *   d_out[b][ky][kx] = sum_{y < ny, x < nx} d_in[b][y][x] exp(sign 2 pi i (ky y / ny + kx x / nx)),
  * unnormalised, sign = -1 (heFFTe forward) or +1 (backward); complex64 planes, or complex128 with
- * precision = IDGX_FFT_FP64.  d_in == d_out transforms in place.  FP32 with nx and ny powers of two in
- * 1024 .. 16384 runs this library's three-pass kernels (the first call for a plane size on a device allocates a
- * scratch of up to 64 MB, or one plane when a plane is larger, kept until the process ends); other shapes and
- * FP64 run rocFFT (the first call for a shape creates its plan).  Calls that overlap on one device share the
- * scratch, so issue them on one stream. */
+ * precision = IDGX_FFT_FP64.  d_in == d_out transforms in place.  This library's kernels run:
+ *   - nx, ny powers of two in 16 .. 512, FP32 or FP64: LDS tiles (one pass when both sides are <= 128 and the
+ *     plane fits 132 KB of LDS, i.e. up to 128 x 128 in FP32 and 128 x 64 in FP64; else a row pass and a column
+ *     pass);
+ *   - FP32, nx and ny powers of two in 1024 .. 16384: the three-pass kernels;
+ *   - FP32, nx = ny = 3 x (a power of two in 1024 .. 16384), e.g. 24576: a de-interleaving pass into 9 sub-planes,
+ *     their three-pass transforms and a radix-3 combining pass.
+ * The first call for a plane size on a device allocates the scratch those need (up to 64 MB, or one sub-plane
+ * when larger; plus one whole plane for the 3 x 2^k sides), kept until the process ends.  Other shapes run rocFFT
+ * (the first call for a shape creates its plan).  Calls that overlap on one device share the scratch, so issue
+ * them on one stream. */
 enum
 {
     IDGX_FFT_FP32 = 0,
     IDGX_FFT_FP64 = 1
 };
 int idgx_fft2d(const void *d_in, void *d_out, int nx, int ny, int64_t batch, int sign, int precision, void *stream);
+
+/* 1 when idgx_fft2d runs this library's own kernels for the shape and precision, 0 when it runs rocFFT, negative
+ * (-IDGX_ERR_INVALID_ARGUMENT) for a bad argument.  No GPU needed. */
+int idgx_fft2d_own_kernels(int nx, int ny, int precision);
 
 /* W-stacking in the image domain, with w_z = (z + 0.5) w_step_in_lambda for layer z - z_min and
  * n = 1 - sqrt(1 - l^2 - m^2) at image pixel (X, Y):

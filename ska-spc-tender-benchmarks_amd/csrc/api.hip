@@ -413,6 +413,13 @@ extern "C" int idgx_fft2d(const void *d_in, void *d_out, int nx, int ny, int64_t
                                         static_cast<hipStream_t>(stream)));
 }
 
+extern "C" int idgx_fft2d_own_kernels(int nx, int ny, int precision)
+{
+    if (nx < 1 || ny < 1 || (precision != IDGX_FFT_FP32 && precision != IDGX_FFT_FP64))
+        return -IDGX_ERR_INVALID_ARGUMENT;
+    return idgx::fft2d_own_kernels(nx, ny, precision == IDGX_FFT_FP64) ? 1 : 0;
+}
+
 extern "C" int idgx_wstack_to_image(const idgx_plan *plan, const void *d_layers, int z_min, int nr_layers,
                                     void *d_image, void *stream)
 {

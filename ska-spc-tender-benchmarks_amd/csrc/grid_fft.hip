@@ -578,18 +578,20 @@ bool stockham_size(int G)
     return G >= 1024 && G <= 16384 && (G & (G - 1)) == 0;
 }
 
-// Scratch plane per (device, plane size), created on first use and kept for the life of the process.
+// Scratch per (device, user, size), created on first use and kept for the life of the process.  The user tells
+// apart buffers that are live at the same time: the three-pass transform's (user 0) and the radix-3 decimation's
+// sub-planes (user 1), which the three-pass transform of those sub-planes must not share.
 std::mutex g_scratch_mutex;
-std::map<std::pair<int, long long>, float2 *> g_scratch;
+std::map<std::tuple<int, int, long long>, float2 *> g_scratch;
 
-hipError_t fft_scratch(long long elements, float2 *&out)
+hipError_t fft_scratch(long long elements, float2 *&out, int user = 0)
 {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess)
         return e;
     std::lock_guard<std::mutex> lock(g_scratch_mutex);
-    const auto key = std::make_pair(dev, elements);
+    const auto key = std::make_tuple(dev, user, elements);
     auto it = g_scratch.find(key);
     if (it != g_scratch.end())
     {
@@ -910,6 +912,105 @@ hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward,
     return checkerboard(grid, planes, G, st);
 }
 
+// ---- sides 3 x 2^k (3072 .. 49152): decimation into 9 sub-planes of the three-pass sizes -------------------------
+//
+// With N = 3 M, x[3 r + i][3 c + j] = x_ij[r][c] and F_ij the M x M DFT of x_ij, the N x N DFT is
+//   X[p + a M][q + b M] = sum_i W_N^(i p) W_3^(i a) sum_j W_N^(j q) W_3^(j b) F_ij[p][q]    (W_n = exp(sign 2 pi i / n))
+// so a plane takes a de-interleaving pass into a plane-sized scratch, the three-pass transform of its 9 sub-planes,
+// and a combining pass of radix-3 butterflies with twiddles W_N^(j q), W_N^(i p) (sin / cos of exact fractions of a
+// turn: the integer product is reduced mod N first) that writes every output row contiguously.
+__global__ __launch_bounds__(kThreads) void deinterleave3_kernel(const float2 *in, float2 *sub, int N, int M,
+                                                                 long long total)
+{
+    const long long e = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
+    if (e >= total)
+        return;
+    const int row = static_cast<int>(e / N), col = static_cast<int>(e - static_cast<long long>(row) * N);
+    const int i = row % 3, r = row / 3, j = col % 3, c = col / 3;
+    sub[(static_cast<size_t>(3 * i + j) * M + r) * M + c] = in[e];
+}
+
+template <int SG>
+__global__ __launch_bounds__(kThreads) void combine3_kernel(const float2 *sub, float2 *out, int N, int M,
+                                                            long long total)
+{
+    const long long e = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
+    if (e >= total)
+        return;
+    const int p = static_cast<int>(e / M), q = static_cast<int>(e - static_cast<long long>(p) * M);
+    const size_t plane = static_cast<size_t>(M) * M;
+    float2 f[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            f[i][j] = sub[(3 * i + j) * plane + e];
+    const float inv_n = 1.F / static_cast<float>(N);
+    // W_3 = exp(SG 2 pi i / 3) = -1/2 + SG i sqrt(3)/2.
+    constexpr float kS3 = 0.866025403784438647F * SG;
+    auto dft3 = [](float2 a, float2 b, float2 c, float2 *y) {
+        const float2 s = make_float2(b.x + c.x, b.y + c.y), d = make_float2(b.x - c.x, b.y - c.y);
+        const float2 m = make_float2(a.x - 0.5F * s.x, a.y - 0.5F * s.y);
+        y[0] = make_float2(a.x + s.x, a.y + s.y);
+        y[1] = make_float2(m.x - kS3 * d.y, m.y + kS3 * d.x);
+        y[2] = make_float2(m.x + kS3 * d.y, m.y - kS3 * d.x);
+    };
+    auto twiddle = [&](float2 v, long long n) { // v * W_N^n
+        const float t = SG * static_cast<float>(n % N) * inv_n;
+        const float cs = __builtin_amdgcn_cosf(t), sn = __builtin_amdgcn_sinf(t);
+        return make_float2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
+    };
+    float2 g[3][3]; // g[i][b]
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        const float2 t1 = twiddle(f[i][1], q), t2 = twiddle(f[i][2], 2LL * q);
+        dft3(f[i][0], t1, t2, g[i]);
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+    {
+        float2 y[3];
+        dft3(g[0][b], twiddle(g[1][b], p), twiddle(g[2][b], 2LL * p), y);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            out[static_cast<size_t>(p + a * M) * N + q + b * M] = y[a];
+    }
+}
+
+bool three_times_stockham(int n)
+{
+    return n % 3 == 0 && stockham_size(n / 3);
+}
+
+template <int SG>
+hipError_t radix3_fft2d(const float2 *src, float2 *dst, int N, long long planes, hipStream_t st)
+{
+    const int M = N / 3;
+    const long long plane = static_cast<long long>(N) * N;
+    float2 *sub = nullptr;
+    hipError_t e = fft_scratch(plane, sub, 1);
+    const long long blocks_in = (plane + kThreads - 1) / kThreads;
+    const long long blocks_out = (static_cast<long long>(M) * M + kThreads - 1) / kThreads;
+    if (blocks_in > 0x7fffffffLL)
+        return hipErrorInvalidConfiguration;
+    for (long long pl = 0; pl < planes && e == hipSuccess; ++pl)
+    {
+        deinterleave3_kernel<<<static_cast<unsigned>(blocks_in), kThreads, 0, st>>>(src + pl * plane, sub, N, M,
+                                                                                    plane);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = stockham_fft2d<SG>(sub, sub, M, M, 9, false, st);
+        if (e == hipSuccess)
+        {
+            combine3_kernel<SG><<<static_cast<unsigned>(blocks_out), kThreads, 0, st>>>(
+                sub, dst + pl * plane, N, M, static_cast<long long>(M) * M);
+            e = hipGetLastError();
+        }
+    }
+    return e;
+}
+
 bool grid_fft_noalloc_supported(int G)
 {
     return stockham_size(G) && static_cast<long long>(G) * G * static_cast<long long>(sizeof(float2)) <=
@@ -926,6 +1027,12 @@ hipError_t launch_grid_fft_noalloc(void *grid, int G, long long nr_layers, bool 
                    : stockham_fft2d<-1>(g, g, G, G, planes, true, st, FftBox(), true);
 }
 
+bool fft2d_own_kernels(int nx, int ny, bool fp64)
+{
+    return small_fft_supported(nx, ny) ||
+           (!fp64 && ((stockham_size(nx) && stockham_size(ny)) || (nx == ny && three_times_stockham(nx))));
+}
+
 bool fft2d_supported(int nx, int ny)
 {
     return nx >= 1 && ny >= 1 && static_cast<long long>(nx) * ny <= 0x7fffffffLL;
@@ -936,12 +1043,20 @@ hipError_t launch_fft2d(const void *in, void *out, int nx, int ny, long long bat
 {
     if (batch < 1 || batch > 0x7fffffffLL || !fft2d_supported(nx, ny) || (sign != 1 && sign != -1))
         return hipErrorInvalidValue;
+    if (small_fft_supported(nx, ny))
+        return launch_small_fft2d(in, out, nx, ny, batch, sign, fp64, st);
     if (!fp64 && stockham_size(nx) && stockham_size(ny))
     {
         const float2 *src = static_cast<const float2 *>(in);
         float2 *dst = static_cast<float2 *>(out);
         return sign > 0 ? stockham_fft2d<1>(src, dst, nx, ny, batch, false, st)
                         : stockham_fft2d<-1>(src, dst, nx, ny, batch, false, st);
+    }
+    if (!fp64 && nx == ny && three_times_stockham(nx))
+    {
+        const float2 *src = static_cast<const float2 *>(in);
+        float2 *dst = static_cast<float2 *>(out);
+        return sign > 0 ? radix3_fft2d<1>(src, dst, nx, batch, st) : radix3_fft2d<-1>(src, dst, nx, batch, st);
     }
     hipfftHandle plan;
     hipError_t e = fft_plan(nx, ny, batch, fp64 ? HIPFFT_Z2Z : HIPFFT_C2C, plan);

@@ -199,6 +199,12 @@ hipError_t launch_grid_clear_box(void *grid, int G, long long nr_layers, const i
 bool grid_fft_noalloc_supported(int G);
 hipError_t launch_grid_fft_noalloc(void *grid, int G, long long nr_layers, bool forward, hipStream_t st);
 bool fft2d_supported(int nx, int ny);
+// Whether launch_fft2d runs this library's kernels for the shape (else rocFFT).
+bool fft2d_own_kernels(int nx, int ny, bool fp64);
+// Plain 2-D DFT of small planes (nx, ny powers of two in 16 .. 512) in LDS tiles, FP32 or FP64.
+bool small_fft_supported(int nx, int ny);
+hipError_t launch_small_fft2d(const void *in, void *out, int nx, int ny, long long batch, int sign, bool fp64,
+                              hipStream_t st);
 // Plain 2-D DFT of `batch` planes of ny rows of nx points, out = sum in exp(sign 2 pi i (...)), unnormalised;
 // in == out is in place.
 hipError_t launch_fft2d(const void *in, void *out, int nx, int ny, long long batch, int sign, bool fp64,

@@ -162,6 +162,7 @@ def lib():
         L.idgx_grid_box.argtypes = [ctypes.POINTER(Plan), vp, ctypes.POINTER(ctypes.c_int32)]
         L.idgx_grid_clear_box.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int32), vp]
         L.idgx_fft2d.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int, vp]
+        L.idgx_fft2d_own_kernels.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.idgx_plan_workspace_bytes.argtypes = [ctypes.c_int64]
         L.idgx_plan_workspace_bytes.restype = ctypes.c_size_t
         L.idgx_plan_segments.argtypes = [ctypes.POINTER(Planner)] + [vp] * 7 + [ctypes.c_int64, vp, vp, vp,
@@ -405,6 +406,14 @@ def fft2d(d_in, d_out, nx, ny, batch, sign, precision=FFT_FP32, stream=None):
     """idgx_fft2d: plain batched 2-D DFT of complex64 (or complex128) planes [batch][ny][nx], unnormalised."""
     _check(lib().idgx_fft2d(_dptr(d_in), _dptr(d_out), int(nx), int(ny), int(batch), int(sign), int(precision),
                             _stream(stream)), "idgx_fft2d")
+
+
+def fft2d_own_kernels(nx, ny, precision=FFT_FP32):
+    """True when idgx_fft2d runs this library's own kernels for the shape (False: rocFFT)."""
+    r = lib().idgx_fft2d_own_kernels(int(nx), int(ny), int(precision))
+    if r < 0:
+        raise IdgxError(-r, "idgx_fft2d_own_kernels")
+    return r == 1
 
 
 def wstack_to_image(plan, d_layers, z_min, nr_layers, d_image, stream=None):

@@ -1,7 +1,9 @@
 """The fft-benchmarks driver (C++, drop-in for the reference's FFT benchmark executable) and the plain batched 2-D
-DFT behind it (idgx_fft2d): YAML handling and error behaviour on the CPU; on the GPU the transform against numpy's
-FP64 FFT (this library's three-pass kernels and the rocFFT path, square and rectangular, in and out of place, FP32
-and FP64), and a driver run with the reference's round-trip check and CSV layout."""
+DFT behind it (idgx_fft2d): YAML handling, error behaviour and the kernel routing on the CPU; on the GPU the transform
+against numpy's FP64 FFT on every path (LDS tiles for sides 16..512 in FP32 and FP64, the three-pass kernels, the
+radix-3 decimation for sides 3 x 2^k, rocFFT for the rest; square and rectangular, in and out of place), the
+24576^2 plane through size-independent properties, and a driver run with the reference's round-trip check and CSV
+layout."""
 import os
 import subprocess
 
@@ -48,16 +50,40 @@ def test_configuration_errors(tmp_path, edit, message):
     assert r.returncode != 0 and message in r.stderr, r.stderr
 
 
+# The reference's fft/examples/gpu.yaml:3 sizes (single precision).
+GPU_YAML_SIDES = (64, 128, 512, 1024, 2048, 4096, 8192, 16384, 24576)
+
+
+def test_every_reference_gpu_size_runs_own_kernels():
+    from idgx import FFT_FP32, FFT_FP64, fft2d_own_kernels
+
+    for n in GPU_YAML_SIDES:
+        assert fft2d_own_kernels(n, n, FFT_FP32), n
+    for nx, ny in ((16, 16), (64, 16), (512, 512), (256, 32)):  # LDS tiles carry FP64 too
+        assert fft2d_own_kernels(nx, ny, FFT_FP64), (nx, ny)
+    for nx, ny in ((300, 200), (3072, 1024), (8, 8)):  # no own kernel: rocFFT
+        assert not fft2d_own_kernels(nx, ny, FFT_FP32), (nx, ny)
+    from idgx import IdgxError
+    with pytest.raises(IdgxError):
+        fft2d_own_kernels(0, 64, FFT_FP32)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("nx, ny, fp64, in_place", [
+    (16, 16, False, False), (64, 64, False, False), (128, 128, False, True), (512, 512, False, False),
+    (16, 32, False, True), (64, 16, True, False), (256, 512, True, True), (128, 128, True, False),
+    (128, 64, True, True), (512, 16, False, False), (16, 512, False, True), (256, 32, True, False),
     (1024, 1024, False, False), (2048, 1024, False, True), (1024, 4096, False, False),
-    (300, 200, False, False), (512, 256, True, False), (4096, 4096, False, True)])
+    (300, 200, False, False), (512, 256, True, False), (4096, 4096, False, True), (1024, 1024, True, False),
+    (3072, 3072, False, False), (3072, 3072, False, True), (6144, 6144, False, False)])
 def test_fft2d_matches_numpy_fp64(nx, ny, fp64, in_place):
     """Planes [batch][ny][nx]: sign -1 is numpy's fft2, sign +1 its unnormalised ifft2."""
     import torch
-    from idgx import FFT_FP32, FFT_FP64, fft2d
+    from idgx import FFT_FP32, FFT_FP64, fft2d, fft2d_own_kernels
 
-    batch = 3
+    batch = 3 if nx * ny <= (1 << 24) else 1
+    print(f"fft2d {nx}x{ny}: {'own kernels' if fft2d_own_kernels(nx, ny, FFT_FP64 if fp64 else FFT_FP32) else 'rocFFT'}",
+          flush=True)
     rng = np.random.default_rng(nx + ny)
     x = rng.standard_normal((batch, ny, nx)) + 1j * rng.standard_normal((batch, ny, nx))
     x = x.astype(np.complex128 if fp64 else np.complex64)
@@ -88,3 +114,70 @@ def test_driver_runs_and_writes_csv(tmp_path):
         assert row[2] == "OK" and int(row[5]) >= 1 and float(row[9]) > 0
         # the reference's check: the inverse of the output against the input pattern k mod 128, scaled 1/(nx ny)
         assert float(row[3]) < 2e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n, batch", [(64, 9000), (16, 140003)])
+def test_fft2d_small_planes_beyond_one_grid(n, batch):
+    """More tiles than workgroups in a launch (8192), so that every workgroup loops over several tiles; the last
+    tile of 16 x 16 planes is partial (16 planes per tile)."""
+    import torch
+    from idgx import FFT_FP32, fft2d
+
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))).astype(np.complex64)
+    d = torch.from_numpy(x).to("cuda")
+    fft2d(d, d, n, n, batch, -1, FFT_FP32)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    ref = np.fft.fft2(x.astype(np.complex128))
+    err = float(np.sqrt(np.sum(np.abs(got - ref) ** 2) / np.sum(np.abs(ref) ** 2)))
+    print(f"fft2d {n}x{n} x {batch}: {err:.2e}", flush=True)
+    assert err < 1e-6
+
+
+@pytest.mark.gpu
+def test_fft2d_24576_by_properties():
+    """The largest reference size (one 24576^2 FP32 plane, 4.8 GB) where numpy's FP64 transform is too slow for a
+    test: (1) 48 unit impulses at seeded positions with seeded complex amplitudes transform to a sum of plane waves,
+    checked in FP64 at 4096 seeded output positions per sign (catches index, sign and transposition mistakes of every
+    pass); (2) a dense seeded plane transformed forward then inverse returns N^2 times itself (rounding of the full
+    pipeline on dense data), and Parseval's sum |X|^2 = N^2 sum |x|^2 holds."""
+    import torch
+    from idgx import FFT_FP32, fft2d
+
+    N = 24576
+    rng = np.random.default_rng(24576)
+    K = 48
+    rows, cols = rng.integers(0, N, K), rng.integers(0, N, K)
+    assert len(set(zip(rows.tolist(), cols.tolist()))) == K
+    amp = rng.standard_normal(K) + 1j * rng.standard_normal(K)
+    P, Q = rng.integers(0, N, 4096), rng.integers(0, N, 4096)
+    x = torch.zeros((N, N), dtype=torch.complex64, device="cuda")
+    x[torch.from_numpy(rows).cuda(), torch.from_numpy(cols).cuda()] = torch.from_numpy(amp.astype(np.complex64)).cuda()
+    amp32 = amp.astype(np.complex64).astype(np.complex128)
+    for sign in (-1, 1):
+        y = torch.empty_like(x)
+        fft2d(x, y, N, N, 1, sign, FFT_FP32)
+        torch.cuda.synchronize()
+        got = y[torch.from_numpy(P).cuda(), torch.from_numpy(Q).cuda()].cpu().numpy().astype(np.complex128)
+        # X[p][q] = sum_k a_k exp(sign 2 pi i (p r_k + q c_k) / N), phases reduced mod N exactly in integers
+        ph = (np.outer(P, rows) + np.outer(Q, cols)) % N
+        ref = (np.exp(sign * 2j * np.pi * ph / N) * amp32).sum(axis=1)
+        err = rel_l2(got, ref)
+        print(f"fft2d {N}^2 impulses sign {sign:+d}: {err:.2e}", flush=True)
+        assert err < 1e-6
+        del y
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    x = torch.randn((N, N), dtype=torch.complex64, device="cuda", generator=gen)
+    y = torch.empty_like(x)
+    fft2d(x, y, N, N, 1, -1, FFT_FP32)
+    e_in = torch.sum(x.abs().double() ** 2).item()
+    e_out = torch.sum(y.abs().double() ** 2).item()
+    parseval = abs(e_out / (float(N) ** 2 * e_in) - 1)
+    fft2d(y, y, N, N, 1, 1, FFT_FP32)
+    torch.cuda.synchronize()
+    r = torch.linalg.vector_norm((y / (float(N) ** 2) - x).double()).item() / torch.linalg.vector_norm(x.double()).item()
+    print(f"fft2d {N}^2 dense: Parseval {parseval:.2e}, round trip {r:.2e}", flush=True)
+    # each direction to the numpy bar of a single transform (1e-6), with slack for the second
+    assert parseval < 1e-5 and r < 3e-6
