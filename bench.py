@@ -210,6 +210,22 @@ def roofline_record(config, precision, key, dominant, flops, sincos, t_dom):
     }
 
 
+def form_rates(precision, tg, td, flops, sincos, bytes_by_direction=None):
+    """Per-direction rates of one form: FP32-equivalent rate against the FP32 peak, algorithmic MFMA rate against
+    the FP16/BF16 matrix peak (128 flops per pair with the 3-product split, 64 single) and, when the algorithmic
+    bytes are known, HBM GB/s at those bytes."""
+    split = 1.0 if precision.endswith("x3") else 0.5
+    mfma_flops = 0 if precision == "fp32" else 128 * sincos * split
+    out = {}
+    for d, t in (("gridder", tg), ("degridder", td)):
+        r = {"fp32_frac": flops / t / 1e12 / FP32_PEAK_TFLOPS,
+             "mfma_frac": mfma_flops / t / 1e12 / MFMA_PEAK_TFLOPS}
+        if bytes_by_direction is not None:
+            r["hbm_gbs_algorithmic"] = bytes_by_direction[d] / t / 1e9
+        out[d] = r
+    return out
+
+
 def measured_traffic(config, key):
     """HBM bytes per launch of kernel `key` from the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE,
     scaled by the copy-kernel calibration) for this workload, or None when it was not profiled."""
@@ -424,9 +440,10 @@ def time_pcie(nvis_bytes, sub_bytes, repeats=3):
     return out
 
 
-def accuracy_vs_fp64(runner, w, host_vis, idx):
-    """Relative L2 error vs the FP64 oracle of the gridder output and the degridder output (degridding the same
-    subgrids) on the sample of subgrids idx; returns (whole-sample, worst subgrid) per direction."""
+def accuracy(runner, w, host_vis, idx):
+    """Relative L2 error of the gridder output and of the degridder output (degridding the same subgrids) on the
+    sample of subgrids idx, against the FP64 oracle and against the oracle's FP32 restatement of the reference CPU
+    path (the FP32 path) on the same inputs; (whole sample, worst subgrid) per direction and reference."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from idgx import observation
@@ -442,22 +459,24 @@ def accuracy_vs_fp64(runner, w, host_vis, idx):
     subs = subs.view(np.complex64).reshape(idx.size, 4, S, S)
     rows = torch.from_numpy(sw["source_rows"]).to(dev)
     vis = d_vis_out.view(-1, C * 4 * 8)[rows].cpu().numpy().view(np.complex64).reshape(-1, C, 4)
-    g64 = oracle_grid(sw, fp64=True)
-    v64 = oracle_degrid(sw, subs, fp64=True)
     md, ch = sw["metadata"], sw["channels"]
-    worst_v = 0.0
-    for s in range(md.size):
-        r0, nt = int(md["time_offset"][s]), int(md["nr_timesteps"][s])
-        c0, nc = int(ch["channel_begin"][s]), int(ch["nr_channels"][s])
-        worst_v = max(worst_v, rel_l2(vis[r0:r0 + nt, c0:c0 + nc], v64[r0:r0 + nt, c0:c0 + nc]))
+    windows = []
     mask = np.zeros(vis.shape[:2], bool)
     for s in range(md.size):
         r0, nt = int(md["time_offset"][s]), int(md["nr_timesteps"][s])
-        mask[r0:r0 + nt, int(ch["channel_begin"][s]):int(ch["channel_begin"][s]) + int(ch["nr_channels"][s])] = True
-    return {
-        "gridder": rel_l2(subs, g64), "gridder_worst_subgrid": float(per_subgrid_rel_l2(subs, g64).max()),
-        "degridder": rel_l2(vis[mask], v64[mask]), "degridder_worst_subgrid": worst_v,
-    }
+        c0, nc = int(ch["channel_begin"][s]), int(ch["nr_channels"][s])
+        windows.append((slice(r0, r0 + nt), slice(c0, c0 + nc)))
+        mask[r0:r0 + nt, c0:c0 + nc] = True
+    out = {}
+    for name, fp64 in (("vs_fp64", True), ("vs_fp32_path", False)):
+        g = oracle_grid(sw, fp64=fp64)
+        v = oracle_degrid(sw, subs, fp64=fp64)
+        out[name] = {
+            "gridder": rel_l2(subs, g), "gridder_worst_subgrid": float(per_subgrid_rel_l2(subs, g).max()),
+            "degridder": rel_l2(vis[mask], v[mask]),
+            "degridder_worst_subgrid": max(rel_l2(vis[r, c], v[r, c]) for r, c in windows),
+        }
+    return out
 
 
 def class_sample(bo, k, idx, d_vis, d_flags):
@@ -592,7 +611,8 @@ def run_batched_config(args):
         rec = {"precision": prec, "path": runner.path, "gridder_ms": tg * 1e3, "degridder_ms": td * 1e3,
                "step_ms": step * 1e3, "round_trip_mvis_s": nvis / (tg + td) / 1e6,
                "gridder_mvis_s": nvis / tg / 1e6, "degridder_mvis_s": nvis / td / 1e6,
-               "gridder_tflops": flops / tg / 1e12, "degridder_tflops": flops / td / 1e12}
+               "gridder_tflops": flops / tg / 1e12, "degridder_tflops": flops / td / 1e12,
+               "rates": form_rates(prec, tg, td, flops, sincos)}
         rec.update(runner.info)
         launches = runner.db.batch.last_launches()
         rec["persistent_launches_per_pass"] = {"gridder": launches[0], "degridder": launches[1]}
@@ -711,6 +731,9 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cycle", action="store_true", help="also time the imaging-cycle stages (default for full_cycle)")
     ap.add_argument("--pcie", action="store_true", help="also time the host transfers a host-buffer caller pays")
+    ap.add_argument("--all-forms", action="store_true",
+                    help="also time the other mixed forms (bf16, fp16, bf16x3 one-shot and bf16x3 batched) before "
+                         "the headline form")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the hot path is per-GPU; see the sharding design); "
@@ -750,11 +773,18 @@ def main():
     runners = [OneShot(dw, "fp32")]
     if not args.no_batch:
         runners.append(Batched([w], d_vis, "fp32", stream))
+    if args.all_forms:
+        others = [p for p in ("bf16", "fp16", "bf16x3", "fp16x3") if p != args.precision]
+        runners += [OneShot(dw, p) for p in others]
+        if not args.no_batch:
+            runners += [Batched([w], d_vis, p, stream) for p in ("bf16x3",) if p != args.precision]
     if args.precision != "fp32":
         runners.append(OneShot(dw, args.precision))
         if not args.no_batch:
             runners.append(Batched([w], d_vis, args.precision, stream))
     forms = []
+    bytes_by_direction = {d: algorithmic_bytes(md, ch, S, C, int(w["nr_aterm_slots"]), int(w["nr_stations"]), d)
+                          for d in ("gridder", "degridder")}
     for runner in runners:
         prec = runner.precision
         tg, td, step = time_form(runner, args.steps, args.warmup, stream)
@@ -763,10 +793,12 @@ def main():
                "round_trip_mvis_s": nvis / (tg + td) / 1e6, "gridder_mvis_s": nvis / tg / 1e6,
                "degridder_mvis_s": nvis / td / 1e6,
                "gridder_tflops": flops / tg / 1e12, "degridder_tflops": flops / td / 1e12}
+        rec["rates"] = form_rates(prec, tg, td, flops, sincos, bytes_by_direction)
         if hasattr(runner, "info"):
             rec.update(runner.info)
         if not args.no_check:
-            rec["rel_l2_vs_fp64"] = accuracy_vs_fp64(runner, w, host_vis, check_idx)
+            acc = accuracy(runner, w, host_vis, check_idx)
+            rec["rel_l2_vs_fp64"], rec["rel_l2_vs_fp32_path"] = acc["vs_fp64"], acc["vs_fp32_path"]
         print(f"bench: {prec} {runner.path}: gridder {tg * 1e3:.2f} ms, degridder {td * 1e3:.2f} ms, "
               f"{rec['round_trip_mvis_s']:.1f} Mvis/s", file=sys.stderr, flush=True)
         forms.append(rec)
@@ -786,7 +818,7 @@ def main():
               f"{full_ms['to_image']:.2f} ms, to uv {full_ms['to_uv']:.2f} ms)", file=sys.stderr, flush=True)
     tg, td = head["gridder_ms"] / 1e3, head["degridder_ms"] / 1e3
     dominant, t_dom = ("degridder", td) if td > tg else ("gridder", tg)
-    bytes_dom = algorithmic_bytes(md, ch, S, C, int(w["nr_aterm_slots"]), int(w["nr_stations"]), dominant)
+    bytes_dom = bytes_by_direction[dominant]
     head_runner = runners[-1]
     from idgx import plain_kernel_shape
 
@@ -848,7 +880,8 @@ def main():
         "forms": forms,
         "gflops_per_pass": flops / 1e9,
         "gsincos_per_pass": sincos / 1e9,
-        "accuracy_sample": f"every {check_every}th subgrid ({check_idx.size} subgrids) vs the FP64 oracle",
+        "accuracy_sample": f"every {check_every}th subgrid ({check_idx.size} subgrids) vs the FP64 oracle and vs "
+                           f"the FP32 path (the oracle's FP32 restatement of the reference CPU gridder and degridder)",
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
