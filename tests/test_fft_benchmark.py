@@ -177,7 +177,8 @@ def test_fft2d_24576_by_properties():
     parseval = abs(e_out / (float(N) ** 2 * e_in) - 1)
     fft2d(y, y, N, N, 1, 1, FFT_FP32)
     torch.cuda.synchronize()
-    r = torch.linalg.vector_norm((y / (float(N) ** 2) - x).double()).item() / torch.linalg.vector_norm(x.double()).item()
+    r = (torch.linalg.vector_norm((y / (float(N) ** 2) - x).to(torch.complex128)).item() /
+         torch.linalg.vector_norm(x.to(torch.complex128)).item())
     print(f"fft2d {N}^2 dense: Parseval {parseval:.2e}, round trip {r:.2e}", flush=True)
     # each direction to the numpy bar of a single transform (1e-6), with slack for the second
     assert parseval < 1e-5 and r < 3e-6
