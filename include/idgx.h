@@ -275,11 +275,15 @@ int idgx_grid_clear_box(const idgx_plan *plan, void *d_grid, int nr_layers, cons
  *   - nx, ny powers of two in 16 .. 512, FP32 or FP64: LDS tiles (one pass when both sides are <= 128 and the
  *     plane fits 132 KB of LDS, i.e. up to 128 x 128 in FP32 and 128 x 64 in FP64; else a row pass and a column
  *     pass);
+ *   - other power-of-two shapes with nx up to 16384 (FP32) / 8192 (FP64) and ny up to 16384 (FP64 every such
+ *     shape; FP32 those with a side below 1024): rows in one LDS pass, columns longer than 512 in two four-step
+ *     passes through a scratch (their column tiles, 32 .. 128 columns, must divide nx);
  *   - FP32, nx and ny powers of two in 1024 .. 16384: the three-pass kernels;
  *   - FP32, nx = ny = 3 x (a power of two in 1024 .. 16384), e.g. 24576: a de-interleaving pass into 9 sub-planes,
  *     their three-pass transforms and a radix-3 combining pass.
  * The first call for a plane size on a device allocates the scratch those need (up to 64 MB, or one sub-plane
- * when larger; plus one whole plane for the 3 x 2^k sides), kept until the process ends.  Other shapes run rocFFT
+ * when larger; one whole plane for the 3 x 2^k sides; 64 MB of planes, or one plane when larger, for the four-step
+ * columns), kept until the process ends.  Other shapes run rocFFT
  * (the first call for a shape creates its plan).  Calls that overlap on one device share the scratch, so issue
  * them on one stream. */
 enum
@@ -423,9 +427,9 @@ enum
 
 /* Single-launch mode per direction: one persistent launch over the merged queue of all classes with the given
  * shape (for the gridder also IDGX_SHAPE_TILES_PER_CLASS); -1 switches that direction to per-class launches.
- * Defaults: several classes (IDGX_SHAPE_TILES_PER_CLASS, the default degridder shape); one class (-1, -1).  The per-class shapes are kept.  For a
- * given shape the results equal the per-class launches' bit for bit (every item writes its own outputs).  get
- * reports -1 for a direction in per-class mode. */
+ * Defaults: several classes (IDGX_SHAPE_TILES_PER_CLASS, the default degridder shape); one class (-1, -1).  The
+ * per-class shapes are kept.  For a given shape the results equal the per-class launches' bit for bit (every item
+ * writes its own outputs).  get reports -1 for a direction in per-class mode. */
 int idgx_batch_set_single_launch(idgx_batch *batch, int gridder_shape, int degridder_shape);
 int idgx_batch_get_single_launch(const idgx_batch *batch, int *gridder_shape, int *degridder_shape);
 

@@ -1027,9 +1027,14 @@ hipError_t launch_grid_fft_noalloc(void *grid, int G, long long nr_layers, bool 
                    : stockham_fft2d<-1>(g, g, G, G, planes, true, st, FftBox(), true);
 }
 
+hipError_t fft_plane_scratch(long long elements, float2 *&out, int user)
+{
+    return fft_scratch(elements, out, user);
+}
+
 bool fft2d_own_kernels(int nx, int ny, bool fp64)
 {
-    return small_fft_supported(nx, ny) ||
+    return lds_fft_supported(nx, ny, fp64) ||
            (!fp64 && ((stockham_size(nx) && stockham_size(ny)) || (nx == ny && three_times_stockham(nx))));
 }
 
@@ -1044,7 +1049,7 @@ hipError_t launch_fft2d(const void *in, void *out, int nx, int ny, long long bat
     if (batch < 1 || batch > 0x7fffffffLL || !fft2d_supported(nx, ny) || (sign != 1 && sign != -1))
         return hipErrorInvalidValue;
     if (small_fft_supported(nx, ny))
-        return launch_small_fft2d(in, out, nx, ny, batch, sign, fp64, st);
+        return launch_lds_fft2d(in, out, nx, ny, batch, sign, fp64, st);
     if (!fp64 && stockham_size(nx) && stockham_size(ny))
     {
         const float2 *src = static_cast<const float2 *>(in);
@@ -1058,6 +1063,8 @@ hipError_t launch_fft2d(const void *in, void *out, int nx, int ny, long long bat
         float2 *dst = static_cast<float2 *>(out);
         return sign > 0 ? radix3_fft2d<1>(src, dst, nx, batch, st) : radix3_fft2d<-1>(src, dst, nx, batch, st);
     }
+    if (lds_fft_supported(nx, ny, fp64))
+        return launch_lds_fft2d(in, out, nx, ny, batch, sign, fp64, st);
     hipfftHandle plan;
     hipError_t e = fft_plan(nx, ny, batch, fp64 ? HIPFFT_Z2Z : HIPFFT_C2C, plan);
     if (e != hipSuccess)

@@ -201,10 +201,15 @@ hipError_t launch_grid_fft_noalloc(void *grid, int G, long long nr_layers, bool 
 bool fft2d_supported(int nx, int ny);
 // Whether launch_fft2d runs this library's kernels for the shape (else rocFFT).
 bool fft2d_own_kernels(int nx, int ny, bool fp64);
-// Plain 2-D DFT of small planes (nx, ny powers of two in 16 .. 512) in LDS tiles, FP32 or FP64.
+// Plain 2-D DFT through LDS line transforms, FP32 or FP64: small planes (both sides powers of two in 16 .. 512:
+// one or two passes), larger power-of-two sides with rows of up to 16384 (FP32) / 8192 (FP64) points in one pass and
+// columns longer than 512 in two four-step passes (the column tile must divide nx).
 bool small_fft_supported(int nx, int ny);
-hipError_t launch_small_fft2d(const void *in, void *out, int nx, int ny, long long batch, int sign, bool fp64,
-                              hipStream_t st);
+bool lds_fft_supported(int nx, int ny, bool fp64);
+hipError_t launch_lds_fft2d(const void *in, void *out, int nx, int ny, long long batch, int sign, bool fp64,
+                            hipStream_t st);
+// Scratch of the plain transforms, per (device, user, size), allocated on first use and kept for the process.
+hipError_t fft_plane_scratch(long long elements, float2 *&out, int user);
 // Plain 2-D DFT of `batch` planes of ny rows of nx points, out = sum in exp(sign 2 pi i (...)), unnormalised;
 // in == out is in place.
 hipError_t launch_fft2d(const void *in, void *out, int nx, int ny, long long batch, int sign, bool fp64,

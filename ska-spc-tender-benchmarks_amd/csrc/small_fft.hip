@@ -1,17 +1,22 @@
-// Batched 2-D DFTs of small planes for the plain FFT entry point (idgx_fft2d): sides nx, ny powers of two in
-// 16 .. 512, FP32 or FP64, out[b][ky][kx] = sum in[b][y][x] exp(sign 2 pi i (ky y / ny + kx x / nx)), unnormalised.
+// Batched 2-D DFTs through LDS line transforms for the plain FFT entry point (idgx_fft2d), FP32 or FP64:
+// out[b][ky][kx] = sum in[b][y][x] exp(sign 2 pi i (ky y / ny + kx x / nx)), unnormalised, nx and ny powers of two.
 //
 // Everything happens in LDS tiles; every thread of a workgroup holds 16 complex points of a tile at a time.
 // - A plane with both sides <= 128 whose padded copy fits 132 KB of LDS (128 x 128 in FP32, 128 x 64 in FP64, and
 //   every smaller plane) takes one pass over HBM: a workgroup loads whole planes (several when a plane has fewer
 //   than 4096 points), transforms their rows, then their columns, and writes them once.
-// - Larger planes take two passes: rows (4096 / nx whole rows per tile, contiguous loads), then columns in place (a
-//   tile of ny rows x C columns, C = max(4096 / ny, 128 bytes / point), so that every row segment is a whole 128-byte
-//   line).
-// A line of n points is transformed by decimation-in-frequency stages of radix 8 or 16 (n = 16: 16; 32: 8 x 4;
-// 64: 8 x 8; 128: 16 x 8; 256: 16 x 16; 512: 8 x 8 x 8).  A stage's small DFTs run in registers, their twiddles
-// W_L^(j k) come from a per-workgroup table of W_n^t (t < n, sincospi in FP64, rounded once for FP32), and the last
-// stage writes its outputs to their natural positions, so that no digit reversal is left for the stores.
+// - Otherwise, with ny <= 512: rows (max(1, 4096 / nx) whole rows per tile, contiguous loads; nx up to 16384 in
+//   FP32, 8192 in FP64, whose padded row fills 128 KB), then columns in place (a tile of ny rows x C columns,
+//   C = max(4096 / ny, 128 bytes / point), so that every row segment is a whole 128-byte line).
+// - With ny > 512: columns of ny = N1 N2 points in two four-step passes over tiles of C columns (A, in place: the
+//   N1-point DFTs over rows n1 N2 + n2, times W_ny^(n2 k1); B: the N2-point DFTs over rows k1 N2 + n2, into a
+//   scratch at rows k1 + N1 k2), then the rows from the scratch back into the output; 64 MB of planes (or one plane)
+//   at a time.
+// A line of n points is transformed by decimation-in-frequency stages of radix 4 to 16 (n = 16: 16; 32: 8 x 4;
+// 64: 8 x 8; 128: 16 x 8; 256: 16 x 16; 512: 8 x 8 x 8; 1024: 16 x 8 x 8; ... 16384: 16 x 16 x 8 x 8).  A stage's
+// small DFTs run in registers, their twiddles W_L^(j k) come from a per-workgroup table of W_n^t (sincospi in FP64,
+// rounded once for FP32; above 512 points two tables, W_n^(64 a) and W_n^b, multiplied), and the last stage writes
+// its outputs to their natural positions, so that no digit reversal is left for the stores.
 // Bank conflicts: consecutive lanes take consecutive lines; rows are padded to nx + 1 points, so that lanes on
 // consecutive rows fall in different banks, and lanes on consecutive columns read consecutive addresses.
 // Workgroups loop over tiles (grid-stride), so that the twiddle table is computed once per workgroup; barriers are
@@ -20,6 +25,7 @@
 // HBM traffic either way.)
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <utility>
 
 namespace idgx
@@ -124,51 +130,83 @@ template <int R, int SG, class V> __device__ __forceinline__ void dft(V *a)
     }
 }
 
-// Radix sequence of an N-point line.
+// Radix sequence of an N-point line (count stages; r[count..3] = 1).
+template <int C, int R0, int R1 = 1, int R2 = 1, int R3 = 1> struct RadixList
+{
+    static constexpr int count = C;
+    static constexpr int r[4] = {R0, R1, R2, R3};
+};
 template <int N> struct Radices;
-template <> struct Radices<16>
-{
-    static constexpr int count = 1, r0 = 16, r1 = 1, r2 = 1;
-};
-template <> struct Radices<32>
-{
-    static constexpr int count = 2, r0 = 8, r1 = 4, r2 = 1;
-};
-template <> struct Radices<64>
-{
-    static constexpr int count = 2, r0 = 8, r1 = 8, r2 = 1;
-};
-template <> struct Radices<128>
-{
-    static constexpr int count = 2, r0 = 16, r1 = 8, r2 = 1;
-};
-template <> struct Radices<256>
-{
-    static constexpr int count = 2, r0 = 16, r1 = 16, r2 = 1;
-};
-template <> struct Radices<512>
-{
-    static constexpr int count = 3, r0 = 8, r1 = 8, r2 = 8;
-};
+template <> struct Radices<16> : RadixList<1, 16> {};
+template <> struct Radices<32> : RadixList<2, 8, 4> {};
+template <> struct Radices<64> : RadixList<2, 8, 8> {};
+template <> struct Radices<128> : RadixList<2, 16, 8> {};
+template <> struct Radices<256> : RadixList<2, 16, 16> {};
+template <> struct Radices<512> : RadixList<3, 8, 8, 8> {};
+template <> struct Radices<1024> : RadixList<3, 16, 8, 8> {};
+template <> struct Radices<2048> : RadixList<3, 16, 16, 8> {};
+template <> struct Radices<4096> : RadixList<3, 16, 16, 16> {};
+template <> struct Radices<8192> : RadixList<4, 16, 16, 8, 4> {};
+template <> struct Radices<16384> : RadixList<4, 16, 16, 8, 8> {};
 
 // Natural frequency of output 0 of the last stage's block blk.  After stages of radix r0, r1, ... the position
-// d0 (N / r0) + d1 (N / r0 r1) + ... + ds holds frequency d0 + r0 d1 + r0 r1 d2 + ..., and blk = position / rs.
+// d0 (N / r0) + d1 (N / r0 r1) + ... + ds holds frequency d0 + r0 d1 + r0 r1 d2 + ..., and blk = position / rs, whose
+// digits are d0 .. d(s-1), d0 the most significant.
 template <int N> __device__ __forceinline__ int natural_base(int blk)
 {
     using Rs = Radices<N>;
-    if constexpr (Rs::count == 1)
-        return 0;
-    else if constexpr (Rs::count == 2)
-        return blk;
-    else
-        return blk / Rs::r1 + Rs::r0 * (blk % Rs::r1);
+    int k = 0, mult = 1;
+#pragma unroll
+    for (int i = 0; i + 1 < Rs::count; ++i)
+    {
+        int span = 1;
+#pragma unroll
+        for (int j = i + 1; j + 1 < Rs::count; ++j)
+            span *= Rs::r[j];
+        const int d = blk / span;
+        blk -= d * span;
+        k += d * mult;
+        mult *= Rs::r[i];
+    }
+    return k;
+}
+
+// Twiddles W_N^t (t < N) of a line: one table of N values for N <= 512; above, W_N^(64 a) W_N^b from two tables of
+// N / 64 and 64 values (one complex multiply per twiddle, in T).
+template <int N> constexpr int twiddle_words()
+{
+    return N <= 512 ? N : N / 64 + 64;
+}
+template <int N, class V> struct Twiddles
+{
+    const V *tab;
+    __device__ __forceinline__ V operator()(int t) const
+    {
+        if constexpr (N <= 512)
+            return tab[t];
+        else
+            return cmul(tab[t >> 6], tab[N / 64 + (t & 63)]);
+    }
+};
+
+// Fills the twiddle words of an N-point line: exp(SG 2 pi i t / N), computed in FP64 and rounded once.
+template <int N, int SG, int THREADS, class V> __device__ __forceinline__ void twiddle_table(V *tw)
+{
+    using T = decltype(tw->x);
+    for (int i = threadIdx.x; i < twiddle_words<N>(); i += THREADS)
+    {
+        const int t = N <= 512 ? i : i < N / 64 ? 64 * i : i - N / 64;
+        double s, c;
+        sincospi(2.0 * t / N, &s, &c);
+        tw[i] = V{static_cast<T>(c), static_cast<T>(SG * s)};
+    }
 }
 
 // One non-final DIF stage of radix R on blocks of L points, in place: for each line, block and j < Q = L / R, the R
-// points at j + m Q become their DFT y_k times W_L^(j k) (tw[t] = W_N^t, so W_L^(j k) = tw[j k N / L]).
-// Consecutive threads take consecutive lines.
+// points at j + m Q become their DFT y_k times W_L^(j k) = W_N^(j k N / L).  Consecutive threads take consecutive
+// lines.
 template <int N, int R, int L, int NLINES, int THREADS, int SG, class V, class Addr>
-__device__ __forceinline__ void dif_stage(V *lds, const V *tw, Addr addr)
+__device__ __forceinline__ void dif_stage(V *lds, Twiddles<N, V> tw, Addr addr)
 {
     constexpr int Q = L / R, BUTTERFLIES = NLINES * (N / R);
     static_assert(BUTTERFLIES % THREADS == 0, "whole butterflies per thread");
@@ -185,7 +223,7 @@ __device__ __forceinline__ void dif_stage(V *lds, const V *tw, Addr addr)
         dft<R, SG>(a);
 #pragma unroll
         for (int k = 1; k < R; ++k)
-            a[k] = cmul(a[k], tw[j * k * (N / L)]);
+            a[k] = cmul(a[k], tw(j * k * (N / L)));
 #pragma unroll
         for (int k = 0; k < R; ++k)
             lds[addr(line, base + k * Q)] = a[k];
@@ -225,37 +263,26 @@ __device__ __forceinline__ void last_stage(V *lds, Addr addr)
     lds_barrier();
 }
 
-// Every stage on NLINES lines of N points; element e of line l is lds[addr(l, e)].  Natural order in and out; the
-// caller has synchronised after filling the lines, and every stage ends with a barrier.
-template <int N, int NLINES, int THREADS, int SG, class V, class Addr>
-__device__ __forceinline__ void fft_lines(V *lds, const V *tw, Addr addr)
+// Stages STAGE .. count-1 on blocks of L points.
+template <int N, int STAGE, int L, int NLINES, int THREADS, int SG, class V, class Addr>
+__device__ __forceinline__ void stages_from(V *lds, Twiddles<N, V> tw, Addr addr)
 {
     using Rs = Radices<N>;
-    if constexpr (Rs::count == 1)
-        last_stage<N, Rs::r0, NLINES, THREADS, SG>(lds, addr);
-    else if constexpr (Rs::count == 2)
-    {
-        dif_stage<N, Rs::r0, N, NLINES, THREADS, SG>(lds, tw, addr);
-        last_stage<N, Rs::r1, NLINES, THREADS, SG>(lds, addr);
-    }
+    if constexpr (STAGE + 1 == Rs::count)
+        last_stage<N, Rs::r[STAGE], NLINES, THREADS, SG>(lds, addr);
     else
     {
-        dif_stage<N, Rs::r0, N, NLINES, THREADS, SG>(lds, tw, addr);
-        dif_stage<N, Rs::r1, N / Rs::r0, NLINES, THREADS, SG>(lds, tw, addr);
-        last_stage<N, Rs::r2, NLINES, THREADS, SG>(lds, addr);
+        dif_stage<N, Rs::r[STAGE], L, NLINES, THREADS, SG>(lds, tw, addr);
+        stages_from<N, STAGE + 1, L / Rs::r[STAGE], NLINES, THREADS, SG>(lds, tw, addr);
     }
 }
 
-// tw[t] = exp(SG 2 pi i t / n) for t < n, computed in FP64 and rounded once.
-template <int SG, int THREADS, class V> __device__ __forceinline__ void twiddle_table(V *tw, int n)
+// Every stage on NLINES lines of N points; element e of line l is lds[addr(l, e)].  Natural order in and out; the
+// caller has synchronised after filling the lines (and the twiddle words), and every stage ends with a barrier.
+template <int N, int NLINES, int THREADS, int SG, class V, class Addr>
+__device__ __forceinline__ void fft_lines(V *lds, const V *tw, Addr addr)
 {
-    using T = decltype(tw->x);
-    for (int t = threadIdx.x; t < n; t += THREADS)
-    {
-        double s, c;
-        sincospi(2.0 * t / n, &s, &c);
-        tw[t] = V{static_cast<T>(c), static_cast<T>(SG * s)};
-    }
+    stages_from<N, 0, N, NLINES, THREADS, SG>(lds, Twiddles<N, V>{tw}, addr);
 }
 
 template <int NX, int NY> constexpr int planes_per_tile()
@@ -278,8 +305,8 @@ __global__ __launch_bounds__((plane_threads<NX, NY>())) void small_fft_planes_ke
     constexpr int PITCH = NX + 1, PLANE = NX * NY;
     __shared__ V lds[P * NY * PITCH];
     __shared__ V tw[NX + NY];
-    twiddle_table<SG, THREADS>(tw, NX);
-    twiddle_table<SG, THREADS>(tw + NX, NY);
+    twiddle_table<NX, SG, THREADS>(tw);
+    twiddle_table<NY, SG, THREADS>(tw + NX);
     const long long tiles = (planes + P - 1) / P;
     for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x)
     {
@@ -299,17 +326,26 @@ __global__ __launch_bounds__((plane_threads<NX, NY>())) void small_fft_planes_ke
     }
 }
 
-// Row pass: kRowTile / NX whole rows (of any plane) per tile, in -> out.
+// Row pass: max(1, kRowTile / NX) whole rows (of any plane) per tile, in -> out; NX up to 16384 (FP32) or 8192
+// (FP64), whose padded row fills 128 KB of LDS.
+template <int NX> constexpr int rows_per_tile()
+{
+    return NX >= kRowTile ? 1 : kRowTile / NX;
+}
+template <int NX> constexpr int row_threads()
+{
+    return rows_per_tile<NX>() * NX / kPointsPerThread;
+}
+
 template <int NX, int SG, class T>
-__global__ __launch_bounds__(kRowTile / kPointsPerThread) void small_fft_rows_kernel(const typename Vec<T>::type *in,
-                                                                                    typename Vec<T>::type *out,
-                                                                                    long long rows)
+__global__ __launch_bounds__((row_threads<NX>())) void small_fft_rows_kernel(const typename Vec<T>::type *in,
+                                                                           typename Vec<T>::type *out, long long rows)
 {
     using V = typename Vec<T>::type;
-    constexpr int ROWS = kRowTile / NX, THREADS = kRowTile / kPointsPerThread, PITCH = NX + 1;
+    constexpr int ROWS = rows_per_tile<NX>(), THREADS = row_threads<NX>(), PITCH = NX + 1;
     __shared__ V lds[ROWS * PITCH];
-    __shared__ V tw[NX];
-    twiddle_table<SG, THREADS>(tw, NX);
+    __shared__ V tw[twiddle_words<NX>()];
+    twiddle_table<NX, SG, THREADS>(tw);
     const long long tiles = (rows + ROWS - 1) / ROWS;
     for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x)
     {
@@ -342,7 +378,7 @@ __global__ __launch_bounds__((NY * column_tile<NY, T>() / kPointsPerThread)) voi
     constexpr int C = column_tile<NY, T>(), THREADS = NY * C / kPointsPerThread;
     __shared__ V lds[NY * C];
     __shared__ V tw[NY];
-    twiddle_table<SG, THREADS>(tw, NY);
+    twiddle_table<NY, SG, THREADS>(tw);
     const int per_plane = nx / C;
     for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x)
     {
@@ -355,6 +391,68 @@ __global__ __launch_bounds__((NY * column_tile<NY, T>() / kPointsPerThread)) voi
         fft_lines<NY, C, THREADS, SG>(lds, tw, [](int line, int e) { return e * C + line; });
         for (int i = threadIdx.x; i < NY * C; i += THREADS)
             p[static_cast<long long>(i / C) * nx + i % C] = lds[i];
+        lds_barrier();
+    }
+}
+
+// Columns of NY = N1 N2 > 512 points in two passes over HBM (four-step), on tiles of C consecutive columns; row
+// index n = n1 N2 + n2 in, k = k1 + N1 k2 out.
+// Pass A, src -> dst at the same positions (in place when equal): for each n2, the N1-point DFT over n1, times
+// W_NY^(n2 k1), stored at row k1 N2 + n2.
+template <int N1, int N2, int SG, class T>
+__global__ __launch_bounds__((N1 * column_tile<N1, T>() / kPointsPerThread)) void fft_columns_a_kernel(
+    const typename Vec<T>::type *src, typename Vec<T>::type *dst, int nx, long long tiles)
+{
+    using V = typename Vec<T>::type;
+    constexpr int C = column_tile<N1, T>(), THREADS = N1 * C / kPointsPerThread, NY = N1 * N2;
+    __shared__ V lds[N1 * C];
+    __shared__ V tw[twiddle_words<N1>()];
+    __shared__ V step[N1];
+    twiddle_table<N1, SG, THREADS>(tw);
+    const int per_row = nx / C;
+    for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    {
+        const long long plane = tile / (static_cast<long long>(N2) * per_row);
+        const int rest = static_cast<int>(tile - plane * N2 * per_row), n2 = rest / per_row;
+        const long long base = plane * NY * static_cast<long long>(nx) + (rest - n2 * per_row) * C;
+        for (int i = threadIdx.x; i < N1 * C; i += THREADS)
+            lds[i] = src[base + static_cast<long long>((i / C) * N2 + n2) * nx + i % C];
+        for (int k1 = threadIdx.x; k1 < N1; k1 += THREADS)
+        {
+            double sn, cs;
+            sincospi(2.0 * (n2 * k1) / NY, &sn, &cs);
+            step[k1] = V{static_cast<T>(cs), static_cast<T>(SG * sn)};
+        }
+        lds_barrier();
+        fft_lines<N1, C, THREADS, SG>(lds, tw, [](int line, int e) { return e * C + line; });
+        for (int i = threadIdx.x; i < N1 * C; i += THREADS)
+            dst[base + static_cast<long long>((i / C) * N2 + n2) * nx + i % C] = cmul(lds[i], step[i / C]);
+        lds_barrier();
+    }
+}
+
+// Pass B, src -> dst (distinct): for each k1, the N2-point DFT over the rows k1 N2 + n2, stored at rows k1 + N1 k2.
+template <int N1, int N2, int SG, class T>
+__global__ __launch_bounds__((N2 * column_tile<N2, T>() / kPointsPerThread)) void fft_columns_b_kernel(
+    const typename Vec<T>::type *src, typename Vec<T>::type *dst, int nx, long long tiles)
+{
+    using V = typename Vec<T>::type;
+    constexpr int C = column_tile<N2, T>(), THREADS = N2 * C / kPointsPerThread, NY = N1 * N2;
+    __shared__ V lds[N2 * C];
+    __shared__ V tw[twiddle_words<N2>()];
+    twiddle_table<N2, SG, THREADS>(tw);
+    const int per_row = nx / C;
+    for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    {
+        const long long plane = tile / (static_cast<long long>(N1) * per_row);
+        const int rest = static_cast<int>(tile - plane * N1 * per_row), k1 = rest / per_row;
+        const long long base = plane * NY * static_cast<long long>(nx) + (rest - k1 * per_row) * C;
+        for (int i = threadIdx.x; i < N2 * C; i += THREADS)
+            lds[i] = src[base + static_cast<long long>(k1 * N2 + i / C) * nx + i % C];
+        lds_barrier();
+        fft_lines<N2, C, THREADS, SG>(lds, tw, [](int line, int e) { return e * C + line; });
+        for (int i = threadIdx.x; i < N2 * C; i += THREADS)
+            dst[base + static_cast<long long>(k1 + N1 * (i / C)) * nx + i % C] = lds[i];
         lds_barrier();
     }
 }
@@ -377,8 +475,8 @@ hipError_t planes_pass(const void *in, void *out, long long batch, hipStream_t s
 template <int NX, int SG, class T> hipError_t rows_pass(const void *in, void *out, long long rows, hipStream_t st)
 {
     using V = typename Vec<T>::type;
-    constexpr int ROWS = kRowTile / NX;
-    small_fft_rows_kernel<NX, SG, T><<<grid_for((rows + ROWS - 1) / ROWS), kRowTile / kPointsPerThread, 0, st>>>(
+    constexpr int ROWS = rows_per_tile<NX>();
+    small_fft_rows_kernel<NX, SG, T><<<grid_for((rows + ROWS - 1) / ROWS), row_threads<NX>(), 0, st>>>(
         static_cast<const V *>(in), static_cast<V *>(out), rows);
     return hipGetLastError();
 }
@@ -411,6 +509,19 @@ template <int SG, class T> hipError_t rows_any(int nx, const void *in, void *out
         return rows_pass<256, SG, T>(in, out, rows, st);
     case 512:
         return rows_pass<512, SG, T>(in, out, rows, st);
+    case 1024:
+        return rows_pass<1024, SG, T>(in, out, rows, st);
+    case 2048:
+        return rows_pass<2048, SG, T>(in, out, rows, st);
+    case 4096:
+        return rows_pass<4096, SG, T>(in, out, rows, st);
+    case 8192:
+        return rows_pass<8192, SG, T>(in, out, rows, st);
+    case 16384:
+        if constexpr (sizeof(T) == 4) // a padded FP64 row of 16384 points does not fit LDS
+            return rows_pass<16384, SG, T>(in, out, rows, st);
+        else
+            return hipErrorInvalidValue;
     default:
         return hipErrorInvalidValue;
     }
@@ -435,6 +546,57 @@ template <int SG, class T> hipError_t columns_any(int ny, void *data, int nx, lo
     default:
         return hipErrorInvalidValue;
     }
+}
+
+template <int N1, int N2, int SG, class T>
+hipError_t four_step_columns(const void *src, void *dst, void *scratch, int nx, long long planes, hipStream_t st)
+{
+    using V = typename Vec<T>::type;
+    constexpr int C1 = column_tile<N1, T>(), C2 = column_tile<N2, T>();
+    if (nx % C1 != 0 || nx % C2 != 0)
+        return hipErrorInvalidValue;
+    const long long ta = planes * N2 * (nx / C1), tb = planes * N1 * (nx / C2);
+    fft_columns_a_kernel<N1, N2, SG, T><<<grid_for(ta), N1 * C1 / kPointsPerThread, 0, st>>>(
+        static_cast<const V *>(src), static_cast<V *>(dst), nx, ta);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    fft_columns_b_kernel<N1, N2, SG, T><<<grid_for(tb), N2 * C2 / kPointsPerThread, 0, st>>>(
+        static_cast<const V *>(dst), static_cast<V *>(scratch), nx, tb);
+    return hipGetLastError();
+}
+
+// Splits NY = N1 x N2 of the four-step column passes.
+template <int SG, class T>
+hipError_t long_columns(int ny, const void *src, void *dst, void *scratch, int nx, long long planes, hipStream_t st)
+{
+    switch (ny)
+    {
+    case 1024:
+        return four_step_columns<32, 32, SG, T>(src, dst, scratch, nx, planes, st);
+    case 2048:
+        return four_step_columns<64, 32, SG, T>(src, dst, scratch, nx, planes, st);
+    case 4096:
+        return four_step_columns<64, 64, SG, T>(src, dst, scratch, nx, planes, st);
+    case 8192:
+        return four_step_columns<128, 64, SG, T>(src, dst, scratch, nx, planes, st);
+    case 16384:
+        return four_step_columns<128, 128, SG, T>(src, dst, scratch, nx, planes, st);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+int column_tile_of(int n, bool fp64)
+{
+    const int by_points = kRowTile / n, by_line = fp64 ? 8 : 16;
+    return by_points > by_line ? by_points : by_line;
+}
+
+void column_split(int ny, int &n1, int &n2)
+{
+    n1 = ny >= 8192 ? 128 : ny >= 2048 ? 64 : 32;
+    n2 = ny / n1;
 }
 
 // One-pass plane shapes: both sides <= 128 and the padded tile within kOnePassLds.
@@ -475,36 +637,72 @@ bool planes_any(int nx, int ny, const void *in, void *out, long long b, hipStrea
     return (try_row<SG, T, (16 << LX)>(nx, ny, in, out, b, st, e, std::make_integer_sequence<int, 4>()) || ...);
 }
 
-bool small_side(int n)
+bool pow2_in(int n, int lo, int hi)
 {
-    return n >= 16 && n <= 512 && (n & (n - 1)) == 0;
+    return n >= lo && n <= hi && (n & (n - 1)) == 0;
 }
 
-template <int SG, class T> hipError_t small_fft(const void *in, void *out, int nx, int ny, long long batch, hipStream_t st)
+// Planes per launch group of the four-step path: the scratch holds one group (64 MB, or one plane when larger).
+constexpr long long kGroupBytes = 64LL << 20;
+
+template <int SG, class T>
+hipError_t lds_fft(const void *in, void *out, int nx, int ny, long long batch, hipStream_t st)
 {
+    using V = typename Vec<T>::type;
     hipError_t e = hipSuccess;
     if (planes_any<SG, T>(nx, ny, in, out, batch, st, e, std::make_integer_sequence<int, 4>()))
         return e;
-    e = rows_any<SG, T>(nx, in, out, batch * ny, st);
-    if (e == hipSuccess)
-        e = columns_any<SG, T>(ny, out, nx, batch, st);
+    if (ny <= 512)
+    {
+        // rows in -> out, then the columns in place
+        e = rows_any<SG, T>(nx, in, out, batch * ny, st);
+        if (e == hipSuccess)
+            e = columns_any<SG, T>(ny, out, nx, batch, st);
+        return e;
+    }
+    // Four-step columns (in -> out, out -> scratch), then the rows scratch -> out, a group of planes at a time.
+    const long long plane = static_cast<long long>(nx) * ny;
+    const long long plane_bytes = plane * static_cast<long long>(sizeof(V));
+    const long long group = std::max(1LL, std::min(batch, kGroupBytes / plane_bytes));
+    float2 *scratch = nullptr;
+    e = fft_plane_scratch(group * plane * static_cast<long long>(sizeof(V) / sizeof(float2)), scratch, 2);
+    const V *src = static_cast<const V *>(in);
+    V *dst = static_cast<V *>(out);
+    for (long long p = 0; p < batch && e == hipSuccess; p += group)
+    {
+        const long long n = std::min(group, batch - p);
+        e = long_columns<SG, T>(ny, src + p * plane, dst + p * plane, scratch, nx, n, st);
+        if (e == hipSuccess)
+            e = rows_any<SG, T>(nx, scratch, dst + p * plane, n * ny, st);
+    }
     return e;
 }
 } // namespace
 
-bool small_fft_supported(int nx, int ny)
+bool lds_fft_supported(int nx, int ny, bool fp64)
 {
-    return small_side(nx) && small_side(ny);
+    if (!pow2_in(nx, 16, fp64 ? 8192 : 16384) || !pow2_in(ny, 16, 16384))
+        return false;
+    if (ny <= 512)
+        return true; // one pass, or a column tile of at most 256 columns, which divides nx
+    int n1, n2;
+    column_split(ny, n1, n2);
+    return nx % column_tile_of(n1, fp64) == 0 && nx % column_tile_of(n2, fp64) == 0;
 }
 
-hipError_t launch_small_fft2d(const void *in, void *out, int nx, int ny, long long batch, int sign, bool fp64,
-                              hipStream_t st)
+bool small_fft_supported(int nx, int ny)
 {
-    if (!small_fft_supported(nx, ny) || batch < 1 || (sign != 1 && sign != -1))
+    return pow2_in(nx, 16, 512) && pow2_in(ny, 16, 512);
+}
+
+hipError_t launch_lds_fft2d(const void *in, void *out, int nx, int ny, long long batch, int sign, bool fp64,
+                            hipStream_t st)
+{
+    if (!lds_fft_supported(nx, ny, fp64) || batch < 1 || (sign != 1 && sign != -1))
         return hipErrorInvalidValue;
     if (fp64)
-        return sign > 0 ? small_fft<1, double>(in, out, nx, ny, batch, st)
-                        : small_fft<-1, double>(in, out, nx, ny, batch, st);
-    return sign > 0 ? small_fft<1, float>(in, out, nx, ny, batch, st) : small_fft<-1, float>(in, out, nx, ny, batch, st);
+        return sign > 0 ? lds_fft<1, double>(in, out, nx, ny, batch, st)
+                        : lds_fft<-1, double>(in, out, nx, ny, batch, st);
+    return sign > 0 ? lds_fft<1, float>(in, out, nx, ny, batch, st) : lds_fft<-1, float>(in, out, nx, ny, batch, st);
 }
 } // namespace idgx

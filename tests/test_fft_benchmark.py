@@ -59,10 +59,16 @@ def test_every_reference_gpu_size_runs_own_kernels():
 
     for n in GPU_YAML_SIDES:
         assert fft2d_own_kernels(n, n, FFT_FP32), n
-    for nx, ny in ((16, 16), (64, 16), (512, 512), (256, 32)):  # LDS tiles carry FP64 too
+    # FP64: every power-of-two plane with rows of up to 8192 points (LDS tiles, four-step columns)
+    for nx, ny in ((16, 16), (64, 16), (512, 512), (256, 32), (1024, 1024), (8192, 8192), (512, 16384)):
         assert fft2d_own_kernels(nx, ny, FFT_FP64), (nx, ny)
-    for nx, ny in ((300, 200), (3072, 1024), (8, 8)):  # no own kernel: rocFFT
-        assert not fft2d_own_kernels(nx, ny, FFT_FP32), (nx, ny)
+    # FP32 power-of-two shapes outside the three-pass sizes (one side below 1024)
+    for nx, ny in ((512, 4096), (64, 8192), (16384, 64), (16384, 16)):
+        assert fft2d_own_kernels(nx, ny, FFT_FP32), (nx, ny)
+    # rocFFT: other shapes, FP64 rows of 16384, column tiles wider than the plane
+    for nx, ny, prec in ((300, 200, FFT_FP32), (3072, 1024, FFT_FP32), (8, 8, FFT_FP32), (16384, 16384, FFT_FP64),
+                         (16, 2048, FFT_FP32)):
+        assert not fft2d_own_kernels(nx, ny, prec), (nx, ny)
     from idgx import IdgxError
     with pytest.raises(IdgxError):
         fft2d_own_kernels(0, 64, FFT_FP32)
@@ -75,7 +81,9 @@ def test_every_reference_gpu_size_runs_own_kernels():
     (128, 64, True, True), (512, 16, False, False), (16, 512, False, True), (256, 32, True, False),
     (1024, 1024, False, False), (2048, 1024, False, True), (1024, 4096, False, False),
     (300, 200, False, False), (512, 256, True, False), (4096, 4096, False, True), (1024, 1024, True, False),
-    (3072, 3072, False, False), (3072, 3072, False, True), (6144, 6144, False, False)])
+    (3072, 3072, False, False), (3072, 3072, False, True), (6144, 6144, False, False),
+    (2048, 1024, True, True), (4096, 4096, True, False), (8192, 2048, True, False), (8192, 8192, True, True),
+    (512, 4096, False, False), (64, 8192, False, True), (16384, 64, False, False), (16384, 16, True, False)])
 def test_fft2d_matches_numpy_fp64(nx, ny, fp64, in_place):
     """Planes [batch][ny][nx]: sign -1 is numpy's fft2, sign +1 its unnormalised ifft2."""
     import torch
