@@ -32,9 +32,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <vector>
+
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include "idgx.h"
 #include "kernels.hpp"
@@ -233,7 +236,8 @@ void layout_queues(const std::vector<Item> &items, std::vector<uint2> &flat, int
     flat.reserve(items.size());
     for (int q = 0; q < idgx::kQueues; ++q)
     {
-        std::stable_sort(queue[q].begin(), queue[q].end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
+        std::stable_sort(queue[q].begin(), queue[q].end(),
+                         [](const Item &x, const Item &y) { return x.cost > y.cost; });
         begin[q] = static_cast<int>(flat.size());
         for (const Item &it : queue[q])
             flat.push_back(it.v);
@@ -670,9 +674,17 @@ extern "C" int idgx_batch_autotune(idgx_batch *b, const idgx_batch_data *data, i
             int best_shape = b->shape[dir][k];
             for (int sh = 0; sh < shape_count(b) && rc == IDGX_OK; ++sh)
             {
+                // One roctx range per trial (each ends synchronised on its event), for rocprofv3 --marker-trace.
+                char name[64];
+                std::snprintf(name, sizeof(name), "idgx autotune %s class %d shape %d",
+                              gridder ? "gridder" : "degridder", k, sh);
+                roctxRangePushA(name);
                 rc = upload_queue(b, gridder, k, sh);
                 if (rc != IDGX_OK)
+                {
+                    roctxRangePop();
                     break;
+                }
                 rc = run(b, data, gridder, st, k); // warm-up
                 if (rc != IDGX_OK)
                     break;
@@ -685,6 +697,7 @@ extern "C" int idgx_batch_autotune(idgx_batch *b, const idgx_batch_data *data, i
                                       hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
                     rc = IDGX_ERR_DEVICE;
                 ms /= repeats;
+                roctxRangePop();
                 if (rc == IDGX_OK && (best < 0.F || ms < best))
                 {
                     best = ms;
