@@ -125,6 +125,36 @@ def test_driver_runs_and_writes_csv(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("fp64", [False, True])
+def test_fft2d_every_power_of_two_shape_up_to_2048(fp64):
+    """Every power-of-two shape 16 .. 2048 per side (64 shapes, batch 2, sign -1 out of place) against numpy's FP64
+    FFT, whichever path runs it (LDS one- or two-pass, four-step columns, three-pass kernels, rocFFT)."""
+    import torch
+    from idgx import FFT_FP32, FFT_FP64, fft2d, fft2d_own_kernels
+
+    prec = FFT_FP64 if fp64 else FFT_FP32
+    sides = [16 << k for k in range(8)]
+    worst, own = 0.0, 0
+    for ny in sides:
+        for nx in sides:
+            rng = np.random.default_rng(nx * 7 + ny)
+            x = rng.standard_normal((2, ny, nx)) + 1j * rng.standard_normal((2, ny, nx))
+            x = x.astype(np.complex128 if fp64 else np.complex64)
+            d_in = torch.from_numpy(x).to("cuda")
+            d_out = torch.empty_like(d_in)
+            fft2d(d_in, d_out, nx, ny, 2, -1, prec)
+            torch.cuda.synchronize()
+            ref = np.fft.fft2(x.astype(np.complex128))
+            got = d_out.cpu().numpy()
+            err = max(rel_l2(got[b], ref[b]) for b in range(2))
+            assert err < (1e-13 if fp64 else 1e-6), (nx, ny, err)
+            worst = max(worst, err)
+            own += fft2d_own_kernels(nx, ny, prec)
+    print(f"fft2d {'fp64' if fp64 else 'fp32'}: 64 shapes 16..2048, {own} on own kernels, worst {worst:.2e}",
+          flush=True)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("n, batch", [(64, 9000), (16, 140003)])
 def test_fft2d_small_planes_beyond_one_grid(n, batch):
     """More tiles than workgroups in a launch (8192), so that every workgroup loops over several tiles; the last
