@@ -596,7 +596,7 @@ def run_batched_config(args):
     # The mixed form first, then plain FP32 on the same HBM buffers (the mixed batch released first: its prepared
     # pixels are as large as the subgrids).  Both run the persistent kernels over the same selection lists, so both
     # skip the flagged samples and the ratio compares equal work.
-    precisions = ([] if args.precision == "fp32" else [args.precision]) + ["fp32"]
+    precisions = ([] if args.precision == "fp32" else [args.precision]) + ([] if args.no_plain else ["fp32"])
     forms = []
     runners = []
     db = None
@@ -639,7 +639,7 @@ def run_batched_config(args):
         forms.append(rec)
         runners.append(runner)
     head, runner = forms[0], runners[0]
-    plain = forms[-1]
+    plain = forms[-1] if len(forms) > 1 and forms[-1]["precision"] == "fp32" else None
     tg, td = head["gridder_ms"] / 1e3, head["degridder_ms"] / 1e3
     dominant, t_dom = ("degridder", td) if td > tg else ("gridder", tg)
     dom_key = kernel_key(head["precision"], "batched", dominant, runner.shapes[dominant])
@@ -701,9 +701,9 @@ def run_batched_config(args):
             "classes": [{"subgrid_size": int(c["subgrid_size"]), "subgrids": int(c["nr_subgrids"]),
                          "visibilities": int(c["nr_visibilities"])} for c in bo["classes"]],
         },
-        "speedup_vs_plain_fp32": head["round_trip_mvis_s"] / plain["round_trip_mvis_s"],
+        "speedup_vs_plain_fp32": None if plain is None else head["round_trip_mvis_s"] / plain["round_trip_mvis_s"],
         "speedup_note": "both forms batched over the same selection lists (flagged samples skipped by both): equal "
-                        "work",
+                        "work" if plain is not None else "plain form not timed (--no-plain)",
         "forms": forms,
         "planner": dict(planner, note="device time includes uploading the segments and downloading the plan"),
         "gflops_per_pass": flops / 1e9,
@@ -731,6 +731,8 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cycle", action="store_true", help="also time the imaging-cycle stages (default for full_cycle)")
     ap.add_argument("--pcie", action="store_true", help="also time the host transfers a host-buffer caller pays")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched configurations: time only the mixed form (counter runs of its kernels)")
     ap.add_argument("--all-forms", action="store_true",
                     help="also time the other mixed forms (bf16, fp16, bf16x3 one-shot and bf16x3 batched) before "
                          "the headline form")

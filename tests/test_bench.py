@@ -50,6 +50,15 @@ def test_kernel_keys_resolve_in_the_committed_profiles(bench):
     assert bench.measured_traffic("ska_low", key) is not None
 
 
+def test_batched_uneven_keys_resolve_in_the_committed_profiles(bench):
+    # the default single launch of the batched uneven plan: the gridder with each class's own tiles per wave
+    key = bench.kernel_key("fp16x3", "batched", "gridder", "tiles per class")
+    assert key == "gridder_mfma_queue_tiles<1, true>"
+    assert bench.measured_traffic("batched_uneven", key) is not None
+    pipes = bench.pipe_profile("batched_uneven", key)
+    assert pipes is not None and pipes["VALUBusy"] > 90
+
+
 def test_roofline_record_is_valu_bound_with_counters(bench):
     key = "degridder_mfma_queue<1, true, 4, 256, false>"
     sincos = 136.3e9
