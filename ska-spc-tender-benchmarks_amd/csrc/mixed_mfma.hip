@@ -639,26 +639,47 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
     const float2 *sub = reinterpret_cast<const float2 *>(a.subgrids) + static_cast<size_t>(s) * kCorr * P;
     uint32_t *s_bw = reinterpret_cast<uint32_t *>(sm.b);
 
-    // Reduced phase coefficients of visibility lane%16 of each tile; for PM only component lane/16 of them (the B
-    // operand of the phase MFMA).
-    float coef[VT], qu[VT], qv[VT], qw[VT];
+    // The FP64 reduction of each of the chunk's visibilities, once: thread tid takes visibilities tid, tid + 256, ...
+    // (the ones it writes in the epilogue, so their output rotations stay in registers) and shares qu, qv, dw
+    // through sm.vis, which the epilogue alone writes.  Each lane then reads visibility lane%16 of each of its tiles;
+    // for PM only component lane/16 of them (the B operand of the phase MFMA).
+    constexpr int kPerThread = (kChunk + kThreads - 1) / kThreads;
+    float rot[kPerThread];
 #pragma unroll
-    for (int t = 0; t < VT; ++t)
+    for (int i = 0; i < kPerThread; ++i)
     {
-        const int v = v0 + (wave * VT + t) * 16 + (lane & 15);
-        coef[t] = qu[t] = qv[t] = qw[t] = 0.F;
+        const int q = tid + i * kThreads;
+        rot[i] = 0.F;
+        if (q >= kChunk)
+            break;
+        const int v = v0 + q;
+        float ru = 0.F, rv = 0.F, rw = 0.F;
         int tt = 0, c = 0;
         if (v < V)
             sel.at(v, tt, c);
         if (v < V && entry_ok(tt, c, m.nr_timesteps, nc))
         {
             const Reduced rd = reduce_visibility(a.uvw + 3 * (row0 + tt), a.wavenumbers[c0 + c], off, delta, 1.F);
-            const int kk = lane >> 4;
-            qu[t] = rd.qu;
-            qv[t] = rd.qv;
-            qw[t] = rd.dw;
-            coef[t] = kk == 0 ? rd.qu : kk == 1 ? rd.qv : kk == 2 ? rd.dw : 0.F;
+            ru = rd.qu;
+            rv = rd.qv;
+            rw = rd.dw;
+            rot[i] = rd.rot;
         }
+        sm.vis[3 * q] = ru;
+        sm.vis[3 * q + 1] = rv;
+        sm.vis[3 * q + 2] = rw;
+    }
+    __syncthreads();
+    float coef[VT], qu[VT], qv[VT], qw[VT];
+#pragma unroll
+    for (int t = 0; t < VT; ++t)
+    {
+        const int q = (wave * VT + t) * 16 + (lane & 15);
+        qu[t] = sm.vis[3 * q];
+        qv[t] = sm.vis[3 * q + 1];
+        qw[t] = sm.vis[3 * q + 2];
+        const int kk = lane >> 4;
+        coef[t] = kk == 0 ? qu[t] : kk == 1 ? qv[t] : kk == 2 ? qw[t] : 0.F;
     }
     f32x4 acc[VT];
 #pragma unroll
@@ -854,10 +875,11 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
     }
     __syncthreads();
     float4 *vout = reinterpret_cast<float4 *>(a.visibilities_out);
-    for (int q = tid; q < kChunk; q += kThreads)
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i)
     {
-        const int v = v0 + q;
-        if (v >= V)
+        const int q = tid + i * kThreads, v = v0 + q;
+        if (q >= kChunk || v >= V)
             break;
         int tt, c;
         sel.at(v, tt, c);
@@ -866,7 +888,7 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
         const size_t o = ((row0 + tt) * C + c0 + c) * 2;
         const float *x = sm.vis + q * 8;
         cfloat val[4] = {{x[0], x[1]}, {x[2], x[3]}, {x[4], x[5]}, {x[6], x[7]}};
-        rotate4(val, reduce_visibility(a.uvw + 3 * (row0 + tt), a.wavenumbers[c0 + c], off, delta, 1.F).rot);
+        rotate4(val, rot[i]);
         vout[o] = make_float4(val[0].re, val[0].im, val[1].re, val[1].im);
         vout[o + 1] = make_float4(val[2].re, val[2].im, val[3].re, val[3].im);
     }
