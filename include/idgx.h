@@ -279,8 +279,9 @@ int idgx_grid_clear_box(const idgx_plan *plan, void *d_grid, int nr_layers, cons
  *     shape; FP32 those with a side below 1024): rows in one LDS pass, columns longer than 512 in two four-step
  *     passes through a scratch (their column tiles, 32 .. 128 columns, must divide nx);
  *   - FP32, nx and ny powers of two in 1024 .. 16384: the three-pass kernels;
- *   - FP32, nx = ny = 3 x (a power of two in 1024 .. 16384), e.g. 24576: a de-interleaving pass into 9 sub-planes,
- *     their three-pass transforms and a radix-3 combining pass.
+ *   - FP32, nx = ny = 3 x (a power of two in 1024 .. 8192), e.g. 24576: three passes as well, the two column passes
+ *     each opening with a radix-3 step along their axis and the row pass writing the interleaved output columns.
+ * Planes hold at most 2^31 - 1 points.
  * The first call for a plane size on a device allocates the scratch those need (up to 64 MB, or one sub-plane
  * when larger; one whole plane for the 3 x 2^k sides; 64 MB of planes, or one plane when larger, for the four-step
  * columns), kept until the process ends.  Other shapes run rocFFT

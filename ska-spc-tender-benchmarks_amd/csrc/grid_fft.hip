@@ -257,7 +257,10 @@ template <int L> constexpr int fft_row_waves_per_simd()
     return L / 2048 > 2 ? L / 2048 : 2;
 }
 
-template <int L, int SG>
+// R3: the first pass of the N x N planes of fft_col3_kernel (N = 3 L = A.gx, not centred, dense): the L-point
+// transforms of the sub-rows x = 3 x' + b (b = 0, 1, 2) of each input row (A.scratch), stored as segment b
+// (columns [b L, b L + L)) of that row in A.grid.
+template <int L, int SG, bool R3 = false>
 __global__ __launch_bounds__(L / 16, fft_row_waves_per_simd<L>()) void fft_row_kernel(FftArgs A)
 {
     constexpr int kLog = ilog2(L);
@@ -288,31 +291,42 @@ __global__ __launch_bounds__(L / 16, fft_row_waves_per_simd<L>()) void fft_row_k
 
     const bool flip = A.centred != 0;
     // Work item i is row ry0 + i % nry of plane i / nry; planes are contiguous, so its global row is
-    // plane * gy + row.  Scratch columns outside [cx0, cx1) are taken as zero without being read.
-    const int rows = A.nplanes * A.nry;
+    // plane * gy + row.  Scratch columns outside [cx0, cx1) are taken as zero without being read.  R3: item i is
+    // sub-row b of row g (below).
+    const int rows = R3 ? 3 * A.nplanes * A.gx : A.nplanes * A.nry;
+    constexpr int kIn = R3 ? 3 : 1; // input element stride
     auto global_row = [&](int i) { return (i / A.nry) * A.gy + A.ry0 + i % A.nry; };
     const bool sign_in = flip && A.rows_first; // (-1)^(x + y) of the centred transform's input
-    auto load = [&](int gr, int col) {
-        float2 v = col >= A.cx0 && col < A.cx1 ? A.scratch[static_cast<size_t>(gr) * L + col] : make_float2(0.F, 0.F);
+    // R3 addresses a row as a workgroup-uniform row pointer plus the thread's 32-bit element offset (fewer VGPRs
+    // than 64-bit element indices there); the plain rows keep 64-bit indices, whose constant parts fold into the
+    // instructions' offsets.
+    auto elem = [](int k) {
+        if constexpr (R3)
+            return static_cast<unsigned>(k);
+        else
+            return static_cast<size_t>(k);
+    };
+    auto load = [&](int gr, const float2 *in_row, int col) {
+        float2 v = col >= A.cx0 && col < A.cx1 ? in_row[elem(kIn * col)] : make_float2(0.F, 0.F);
         if (sign_in && ((col + gr) & 1)) // gy is even: the row's parity within its plane is that of gr
             v = make_float2(-v.x, -v.y);
         return v;
     };
-    for (int i = blockIdx.x; i < rows; i += gridDim.x)
-    {
-        const int y = global_row(i);
+    auto tw = [&](int qq) {
+        qq &= L - 1;
+        return f2mul(t_hi[qq >> 6], t_lo[qq & 63]);
+    };
+    // The L-point transform of the row at in_row (global row gr, for the input signs); output element k, the idx-th
+    // (0 .. 15, a compile-time index after unrolling) of this thread's, goes to emit(idx, k, value).
+    auto row_fft = [&](int gr, const float2 *in_row, auto &&emit) {
         // An opaque copy of the thread index per row: what derives from it (twiddles, slots) is formed where it is
         // used instead of being hoisted out of the row loop and held in registers for every row.
         int j0 = threadIdx.x;
         asm volatile("" : "+v"(j0));
-        auto tw = [&](int qq) {
-            qq &= L - 1;
-            return f2mul(t_hi[qq >> 6], t_lo[qq & 63]);
-        };
         float2 v[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r)
-            v[r] = load(y, j0 + r * kQ);
+            v[r] = load(gr, in_row, j0 + r * kQ);
         int Ns = 1;
 #pragma unroll
         for (int st = 0; st < kStages16; ++st)
@@ -329,11 +343,7 @@ __global__ __launch_bounds__(L / 16, fft_row_waves_per_simd<L>()) void fft_row_k
             {
 #pragma unroll
                 for (int r = 0; r < 16; ++r)
-                {
-                    const int k = j0 + r * Ns; // k odd <=> j0 odd
-                    A.grid[static_cast<size_t>(y) * L + k] =
-                        (flip && (k & 1)) ? make_float2(-v[r].x, -v[r].y) : v[r];
-                }
+                    emit(r, j0 + r * Ns, v[r]);
             }
             else
             {
@@ -381,12 +391,32 @@ __global__ __launch_bounds__(L / 16, fft_row_waves_per_simd<L>()) void fft_row_k
                 dft<kRl, SG>(u);
 #pragma unroll
                 for (int r = 0; r < kRl; ++r)
-                {
-                    const int k = j + r * Ns;
-                    A.grid[static_cast<size_t>(y) * L + k] =
-                        (flip && (k & 1)) ? make_float2(-u[r].x, -u[r].y) : u[r];
-                }
+                    emit(b * kRl + r, j + r * Ns, u[r]);
             }
+        }
+    };
+    for (int i = blockIdx.x; i < rows; i += gridDim.x)
+    {
+        if constexpr (R3)
+        {
+            // Item i is sub-row b = (i % 24) / 8 of row g = 8 (i / 24) + i % 8: with a grid that is a multiple of
+            // 24, the three sub-rows of a row go to workgroups w, w + 8, w + 16, which share an XCD (workgroups are
+            // dispatched to the 8 XCDs in turn) and run in step, so each cache line of the input row, a third of
+            // which each sub-row reads, is fetched into one L2 once.
+            const int N = A.gx;
+            const int grp = i / 24, rem = i - 24 * grp;
+            const int g = 8 * grp + (rem & 7), b = rem >> 3;
+            float2 *out_row = A.grid + static_cast<size_t>(g) * N + static_cast<size_t>(b) * L;
+            row_fft(g, A.scratch + static_cast<size_t>(g) * N + b,
+                    [&](int, int k, float2 val) { out_row[elem(k)] = val; });
+        }
+        else
+        {
+            const int y = global_row(i);
+            float2 *out_row = A.grid + static_cast<size_t>(y) * L;
+            row_fft(y, A.scratch + static_cast<size_t>(y) * L, [&](int, int k, float2 val) {
+                out_row[elem(k)] = (flip && (k & 1)) ? make_float2(-val.x, -val.y) : val; // k odd <=> j0 odd
+            });
         }
     }
 }
@@ -497,6 +527,162 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
                     if (static_cast<unsigned>(row - A.ry0) < static_cast<unsigned>(A.nry))
                         A.scratch[base + static_cast<size_t>(row) * G + x] =
                             (A.centred && (row & 1)) ? make_float2(-u[r].x, -u[r].y) : u[r];
+                }
+            }
+        }
+    }
+}
+
+// y[r] = sum_q W_3^(r q) x_q, W_3 = exp(SG 2 pi i / 3) = -1/2 + SG i sqrt(3) / 2.
+template <int SG> __device__ __forceinline__ void dft3(float2 a, float2 b, float2 c, float2 *y)
+{
+    constexpr float kS3 = 0.866025403784438647F * SG;
+    const float2 s = f2add(b, c), d = f2sub(b, c);
+    const float2 m = make_float2(a.x - 0.5F * s.x, a.y - 0.5F * s.y);
+    y[0] = f2add(a, s);
+    y[1] = make_float2(m.x - kS3 * d.y, m.y + kS3 * d.x);
+    y[2] = make_float2(m.x + kS3 * d.y, m.y - kS3 * d.x);
+}
+
+// Column passes of N x N planes with N = 3 M (M = N1 N2 a power of two), after the R3 row pass, which leaves in
+// segment b (columns [b M, b M + M)) of row y the M-point transforms R_b of the sub-rows x = 3 x' + b.  Along x
+// (decimation in time),
+//   X_x[y][k + M q] = sum_{b < 3} W_3^(b q) W_N^(b k) R_b[y][k]        (W_n = exp(SG 2 pi i / n)),
+// and along y (decimation in frequency),
+//   X[3 k + r][.] = sum_{y < M} W_M^(k y) t_r[y][.],   t_r[y] = W_N^(r y) sum_{q < 3} W_3^(r q) X_x[y + M q][.].
+// The x-axis combination acts on whole columns and the y-axis steps on each column alone, so they commute: column
+// pass 1 (in place) forms t_0, t_1, t_2 from rows y, y + M, y + 2 M of the row pass's output and runs the first
+// four-step stage of their M-point transforms (as fft_col_kernel's C1), storing t_r's rows in row block r (rows
+// [r M, r M + M)): a tile writes the rows it read.  Column pass 2 reads columns k, k + M, k + 2 M of a row block's
+// rows, forms the x-axis combination, runs the second stage (C2) on each of its three results q and stores them at
+// columns [q M, q M + M) of output row 3 (k1 + N1 k2) + r.  Every pass reads and writes whole cache lines.  No
+// prefetch of the next tile: a thread already holds three sequences.
+template <int PASS, int L, int SG>
+__global__ __launch_bounds__(kColTile * L / 16) void fft_col3_kernel(FftArgs A, int nr_tiles)
+{
+    constexpr int kLog = ilog2(L);
+    constexpr int kRl = 1 << (kLog % 4);
+    constexpr int kQ = L / 16;
+    static_assert(kLog / 4 == 1 && kRl > 1, "L = 32, 64 or 128");
+    __shared__ float buf[L * kColTile];
+    const int c = threadIdx.x & (kColTile - 1);
+    const int N = A.gx, M = A.gy;
+    const size_t block_elems = static_cast<size_t>(M) * N; // one row block: M rows of N
+    const int log_q = PASS == kPassCol1 ? A.log_n2 : A.log_n1;
+    const float inv_n = 1.F / static_cast<float>(N);
+    auto rot = [&](float2 v, int n) { // v W_N^n for 0 <= n < N, from sin / cos of the fraction n / N of a turn
+        const float t = SG * static_cast<float>(n) * inv_n;
+        return f2mul(v, make_float2(__builtin_amdgcn_cosf(t), __builtin_amdgcn_sinf(t)));
+    };
+    for (int tile = blockIdx.x; tile < nr_tiles; tile += gridDim.x)
+    {
+        int j0 = threadIdx.x / kColTile; // opaque per tile: keeps twiddles and slots out of registers across tiles
+        asm volatile("" : "+v"(j0));
+        // Tile index = (blk x Q + q) x nct + column tile: C1 blk = plane, q = n2, nct = N / kColTile; C2 blk =
+        // 3 plane + row block, q = k1, nct = M / kColTile.
+        const int rest = tile / A.nct;
+        const int x0 = (tile - rest * A.nct) * kColTile; // the tile's first column
+        const int x = x0 + c;
+        const int q = rest & ((1 << log_q) - 1);
+        const int blk = rest >> log_q;
+        // Element r of sequence qq: C1 row N2 (j0 + r kQ) + q + M qq, column x; C2 row N2 q + j0 + r kQ, column
+        // x + M qq.  Addresses: a pointer that is uniform over the workgroup (tile, r, qq) plus the thread's 32-bit
+        // offset (j0 rows of the pass's row step, c columns).
+        const float2 *in = PASS == kPassCol1 ? A.src + 3 * static_cast<size_t>(blk) * block_elems
+                                                   + static_cast<size_t>(q) * N + x0
+                                             : A.grid + static_cast<size_t>(blk) * block_elems
+                                                   + static_cast<size_t>(A.N2) * q * N + x0;
+        const int row_step = PASS == kPassCol1 ? A.N2 : 1; // rows between consecutive elements of a sequence
+        const unsigned toff = static_cast<unsigned>(j0 * row_step * N + c);
+        float2 t[3][16];
+#pragma unroll
+        for (int qq = 0; qq < 3; ++qq)
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+            {
+                const size_t u = PASS == kPassCol1 ? static_cast<size_t>(row_step * r * kQ + M * qq) * N
+                                                   : static_cast<size_t>(r * kQ) * N + static_cast<size_t>(M) * qq;
+                t[qq][r] = (in + u)[toff];
+            }
+        // The radix-3 steps: C1 twiddles the butterfly's outputs by W_N^(r y), C2 its inputs by W_N^(b k) (y, k < M,
+        // so 2 y, 2 k < N).
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+        {
+            float2 y3[3];
+            if constexpr (PASS == kPassCol1)
+            {
+                const int n = A.N2 * (j0 + r * kQ) + q;
+                dft3<SG>(t[0][r], t[1][r], t[2][r], y3);
+                t[0][r] = y3[0];
+                t[1][r] = rot(y3[1], n);
+                t[2][r] = rot(y3[2], 2 * n);
+            }
+            else
+            {
+                dft3<SG>(t[0][r], rot(t[1][r], x), rot(t[2][r], 2 * x), y3);
+                t[0][r] = y3[0];
+                t[1][r] = y3[1];
+                t[2][r] = y3[2];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+        {
+            float2 *v = t[s];
+            dft<16, SG>(v);
+            // As fft_col_kernel: output r of butterfly j0 is element 16 j0 + r; the radix-kRl stage takes elements
+            // j0 + b kQ + r (L / kRl).
+            auto rslot = [&](int e) { return (j0 + (e / kRl) * kQ + (e % kRl) * (L / kRl)) * kColTile + c; };
+            float re[16];
+            lds_barrier(); // the previous sequence's reads are done
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                buf[(16 * j0 + r) * kColTile + c] = v[r].x;
+            lds_barrier();
+#pragma unroll
+            for (int e = 0; e < 16; ++e)
+                re[e] = buf[rslot(e)];
+            lds_barrier();
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                buf[(16 * j0 + r) * kColTile + c] = v[r].y;
+            lds_barrier();
+#pragma unroll
+            for (int e = 0; e < 16; ++e)
+                v[e] = make_float2(re[e], buf[rslot(e)]);
+#pragma unroll
+            for (int b = 0; b < 16 / kRl; ++b)
+            {
+                const int j = j0 + b * kQ;
+                float2 u[kRl];
+#pragma unroll
+                for (int r = 0; r < kRl; ++r)
+                    u[r] = v[b * kRl + r];
+#pragma unroll
+                for (int r = 1; r < kRl; ++r)
+                    u[r] = f2mul(u[r], twiddle<SG>(j * r, kLog));
+                dft<kRl, SG>(u);
+#pragma unroll
+                for (int r = 0; r < kRl; ++r)
+                {
+                    // k = j0 + (b kQ + r L / kRl): the thread's j0 rows of the pass's output row step plus a
+                    // uniform part.
+                    const int k = j + r * (L / kRl);
+                    const int ku = b * kQ + r * (L / kRl);
+                    if constexpr (PASS == kPassCol1)
+                    {
+                        float2 *o = A.grid + (3 * static_cast<size_t>(blk) * M + s * M + A.N2 * ku + q) * N + x0;
+                        o[static_cast<unsigned>(j0 * A.N2 * N + c)] = f2mul(u[r], twiddle<SG>(q * k, A.log_gy));
+                    }
+                    else
+                    {
+                        // Row block blk % 3 of plane blk / 3, k1 = q: output row 3 (q + N1 k) + blk % 3.
+                        const int plane = blk / 3, rb = blk - 3 * plane;
+                        float2 *o = A.scratch + (static_cast<size_t>(plane) * N + rb + 3 * (q + A.N1 * ku)) * N +
+                                    s * M + x0;
+                        o[static_cast<unsigned>(3 * j0 * A.N1 * N + c)] = u[r];
+                    }
                 }
             }
         }
@@ -912,75 +1098,73 @@ hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward,
     return checkerboard(grid, planes, G, st);
 }
 
-// ---- sides 3 x 2^k (3072 .. 49152): decimation into 9 sub-planes of the three-pass sizes -------------------------
+// ---- sides 3 x 2^k (3072 .. 24576): one radix-3 step per axis folded into the three passes ---------------------
 //
-// With N = 3 M, x[3 r + i][3 c + j] = x_ij[r][c] and F_ij the M x M DFT of x_ij, the N x N DFT is
-//   X[p + a M][q + b M] = sum_i W_N^(i p) W_3^(i a) sum_j W_N^(j q) W_3^(j b) F_ij[p][q]    (W_n = exp(sign 2 pi i / n))
-// so a plane takes a de-interleaving pass into a plane-sized scratch, the three-pass transform of its 9 sub-planes,
-// and a combining pass of radix-3 butterflies with twiddles W_N^(j q), W_N^(i p) (sin / cos of exact fractions of a
-// turn: the integer product is reduced mod N first) that writes every output row contiguously.
-__global__ __launch_bounds__(kThreads) void deinterleave3_kernel(const float2 *in, float2 *sub, int N, int M,
-                                                                 long long total)
+// N = 3 M: the R3 row pass (M-point transforms of the sub-rows x = 3 x' + b, input -> scratch), column pass 1 of
+// fft_col3_kernel (a radix-3 step along y, then the first four-step stage; in place) and column pass 2 (the radix-3
+// step along x, then the second stage; scratch -> output).  Three passes over HBM, as for the power-of-two sides;
+// the scratch is one plane.
+template <int PASS, int L, int SG> hipError_t launch_cols3(const FftArgs &A, int nr_tiles, hipStream_t st)
 {
-    const long long e = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
-    if (e >= total)
-        return;
-    const int row = static_cast<int>(e / N), col = static_cast<int>(e - static_cast<long long>(row) * N);
-    const int i = row % 3, r = row / 3, j = col % 3, c = col / 3;
-    sub[(static_cast<size_t>(3 * i + j) * M + r) * M + c] = in[e];
+    constexpr int threads = kColTile * L / 16;
+    int blocks = 0;
+    const hipError_t e = resident_blocks(reinterpret_cast<const void *>(&fft_col3_kernel<PASS, L, SG>), threads,
+                                         nr_tiles, blocks);
+    if (e != hipSuccess)
+        return e;
+    fft_col3_kernel<PASS, L, SG><<<blocks, threads, 0, st>>>(A, nr_tiles);
+    return hipGetLastError();
 }
 
-template <int SG>
-__global__ __launch_bounds__(kThreads) void combine3_kernel(const float2 *sub, float2 *out, int N, int M,
-                                                            long long total)
+template <int PASS, int SG> hipError_t launch_col3_pass(const FftArgs &A, int L, int nr_tiles, hipStream_t st)
 {
-    const long long e = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
-    if (e >= total)
-        return;
-    const int p = static_cast<int>(e / M), q = static_cast<int>(e - static_cast<long long>(p) * M);
-    const size_t plane = static_cast<size_t>(M) * M;
-    float2 f[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            f[i][j] = sub[(3 * i + j) * plane + e];
-    const float inv_n = 1.F / static_cast<float>(N);
-    // W_3 = exp(SG 2 pi i / 3) = -1/2 + SG i sqrt(3)/2.
-    constexpr float kS3 = 0.866025403784438647F * SG;
-    auto dft3 = [](float2 a, float2 b, float2 c, float2 *y) {
-        const float2 s = make_float2(b.x + c.x, b.y + c.y), d = make_float2(b.x - c.x, b.y - c.y);
-        const float2 m = make_float2(a.x - 0.5F * s.x, a.y - 0.5F * s.y);
-        y[0] = make_float2(a.x + s.x, a.y + s.y);
-        y[1] = make_float2(m.x - kS3 * d.y, m.y + kS3 * d.x);
-        y[2] = make_float2(m.x + kS3 * d.y, m.y - kS3 * d.x);
-    };
-    auto twiddle = [&](float2 v, long long n) { // v * W_N^n
-        const float t = SG * static_cast<float>(n % N) * inv_n;
-        const float cs = __builtin_amdgcn_cosf(t), sn = __builtin_amdgcn_sinf(t);
-        return make_float2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
-    };
-    float2 g[3][3]; // g[i][b]
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
+    switch (L)
     {
-        const float2 t1 = twiddle(f[i][1], q), t2 = twiddle(f[i][2], 2LL * q);
-        dft3(f[i][0], t1, t2, g[i]);
-    }
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-    {
-        float2 y[3];
-        dft3(g[0][b], twiddle(g[1][b], p), twiddle(g[2][b], 2LL * p), y);
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-            out[static_cast<size_t>(p + a * M) * N + q + b * M] = y[a];
+    case 32:
+        return launch_cols3<PASS, 32, SG>(A, nr_tiles, st);
+    case 64:
+        return launch_cols3<PASS, 64, SG>(A, nr_tiles, st);
+    case 128:
+        return launch_cols3<PASS, 128, SG>(A, nr_tiles, st);
+    default:
+        return hipErrorInvalidValue;
     }
 }
 
+template <int L, int SG> hipError_t launch_rows3(const FftArgs &A, hipStream_t st)
+{
+    const int items = 3 * A.nplanes * A.gx; // a multiple of 24: M >= 1024
+    int blocks = 0;
+    const hipError_t e =
+        resident_blocks(reinterpret_cast<const void *>(&fft_row_kernel<L, SG, true>), L / 16, items, blocks);
+    if (e != hipSuccess)
+        return e;
+    blocks = std::max(24, blocks / 24 * 24); // the three sub-rows of a row on one XCD, in step
+    fft_row_kernel<L, SG, true><<<blocks, L / 16, 0, st>>>(A);
+    return hipGetLastError();
+}
+
+template <int SG> hipError_t launch_row3_pass(const FftArgs &A, hipStream_t st)
+{
+    switch (A.gy)
+    {
+    case 1024:
+        return launch_rows3<1024, SG>(A, st);
+    case 2048:
+        return launch_rows3<2048, SG>(A, st);
+    case 4096:
+        return launch_rows3<4096, SG>(A, st);
+    case 8192:
+        return launch_rows3<8192, SG>(A, st);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+// 3 M with M a three-pass size whose N x N plane stays within 2^31 - 1 points (M <= 8192).
 bool three_times_stockham(int n)
 {
-    return n % 3 == 0 && stockham_size(n / 3);
+    return n % 3 == 0 && stockham_size(n / 3) && n / 3 <= 8192;
 }
 
 template <int SG>
@@ -988,25 +1172,43 @@ hipError_t radix3_fft2d(const float2 *src, float2 *dst, int N, long long planes,
 {
     const int M = N / 3;
     const long long plane = static_cast<long long>(N) * N;
-    float2 *sub = nullptr;
-    hipError_t e = fft_scratch(plane, sub, 1);
-    const long long blocks_in = (plane + kThreads - 1) / kThreads;
-    const long long blocks_out = (static_cast<long long>(M) * M + kThreads - 1) / kThreads;
-    if (blocks_in > 0x7fffffffLL)
-        return hipErrorInvalidConfiguration;
+    FftArgs A;
+    hipError_t e = fft_scratch(plane, A.scratch, 1);
+    if (e != hipSuccess)
+        return e;
+    A.gx = N;
+    A.gy = M;
+    A.log_gy = ilog2(M);
+    A.log_n1 = A.log_gy / 2;
+    A.log_n2 = A.log_gy - A.log_n1;
+    A.N1 = 1 << A.log_n1;
+    A.N2 = 1 << A.log_n2;
+    A.log_tiles = 0;
+    A.nplanes = 1;
+    A.centred = 0;
+    A.ct0 = 0;
+    A.cx0 = 0;
+    A.cx1 = M; // the row pass reads whole segments
+    A.ry0 = 0;
+    A.nry = M;
+    A.rows_first = 0;
+    float2 *sc = A.scratch;
     for (long long pl = 0; pl < planes && e == hipSuccess; ++pl)
     {
-        deinterleave3_kernel<<<static_cast<unsigned>(blocks_in), kThreads, 0, st>>>(src + pl * plane, sub, N, M,
-                                                                                    plane);
-        e = hipGetLastError();
+        FftArgs R = A; // rows: input -> scratch
+        R.scratch = const_cast<float2 *>(src + pl * plane);
+        R.grid = sc;
+        e = launch_row3_pass<SG>(R, st);
+        FftArgs C = A; // column pass 1: scratch, in place
+        C.src = sc;
+        C.grid = sc;
+        C.nct = N / kColTile;
         if (e == hipSuccess)
-            e = stockham_fft2d<SG>(sub, sub, M, M, 9, false, st);
+            e = launch_col3_pass<kPassCol1, SG>(C, A.N1, A.N2 * C.nct, st);
+        C.scratch = dst + pl * plane; // column pass 2: scratch -> output
+        C.nct = M / kColTile;
         if (e == hipSuccess)
-        {
-            combine3_kernel<SG><<<static_cast<unsigned>(blocks_out), kThreads, 0, st>>>(
-                sub, dst + pl * plane, N, M, static_cast<long long>(M) * M);
-            e = hipGetLastError();
-        }
+            e = launch_col3_pass<kPassCol2, SG>(C, A.N2, 3 * A.N1 * C.nct, st);
     }
     return e;
 }
