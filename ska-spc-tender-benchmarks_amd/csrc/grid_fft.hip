@@ -835,8 +835,8 @@ hipError_t static_scratch(float2 *&out)
 constexpr long long kScratchBytes = 64LL << 20;
 
 // Copies rows [y0, y0 + ny) x columns [c0, c0 + nc) of every plane from src to dst (planes of G x G).
-__global__ __launch_bounds__(kThreads) void copy_box_kernel(const float2 *src, float2 *dst, int G, int y0, int ny, int c0,
-                                                            int nc, long long total)
+__global__ __launch_bounds__(kThreads) void copy_box_kernel(const float2 *src, float2 *dst, int G, int y0, int ny,
+                                                            int c0, int nc, long long total)
 {
     const long long i = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
     if (i >= total)
@@ -874,7 +874,8 @@ hipError_t stockham_fft2d(const float2 *src, float2 *dst, int gx, int gy, long l
 {
     FftArgs A;
     const long long plane = static_cast<long long>(gx) * gy;
-    const long long group = std::max(1LL, std::min(planes, kScratchBytes / (plane * static_cast<long long>(sizeof(float2)))));
+    const long long plane_bytes = plane * static_cast<long long>(sizeof(float2));
+    const long long group = std::max(1LL, std::min(planes, kScratchBytes / plane_bytes));
     hipError_t e = hipSuccess;
     if (no_alloc)
     {

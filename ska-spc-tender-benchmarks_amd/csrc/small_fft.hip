@@ -7,7 +7,8 @@
 //   than 4096 points), transforms their rows, then their columns, and writes them once.
 // - Otherwise, with ny <= 512: rows (max(1, 4096 / nx) whole rows per tile, contiguous loads; nx up to 16384 in
 //   FP32, 8192 in FP64, whose padded row fills 128 KB), then columns in place (a tile of ny rows x C columns,
-//   C = max(4096 / ny, 128 bytes / point), so that every row segment is a whole 128-byte line).
+//   C = max(4096 / ny, 128 bytes / point), so that every row segment is a whole 128-byte line); 64 MB of planes at
+//   a time, so that the column pass reads the row pass's output from the memory-side cache.
 // - With ny > 512: columns of ny = N1 N2 points in two four-step passes over tiles of C columns (A, in place: the
 //   N1-point DFTs over rows n1 N2 + n2, times W_ny^(n2 k1); B: the N2-point DFTs over rows k1 N2 + n2, into a
 //   scratch at rows k1 + N1 k2), then the rows from the scratch back into the output; 64 MB of planes (or one plane)
@@ -652,18 +653,25 @@ hipError_t lds_fft(const void *in, void *out, int nx, int ny, long long batch, h
     hipError_t e = hipSuccess;
     if (planes_any<SG, T>(nx, ny, in, out, batch, st, e, std::make_integer_sequence<int, 4>()))
         return e;
-    if (ny <= 512)
-    {
-        // rows in -> out, then the columns in place
-        e = rows_any<SG, T>(nx, in, out, batch * ny, st);
-        if (e == hipSuccess)
-            e = columns_any<SG, T>(ny, out, nx, batch, st);
-        return e;
-    }
-    // Four-step columns (in -> out, out -> scratch), then the rows scratch -> out, a group of planes at a time.
     const long long plane = static_cast<long long>(nx) * ny;
     const long long plane_bytes = plane * static_cast<long long>(sizeof(V));
     const long long group = std::max(1LL, std::min(batch, kGroupBytes / plane_bytes));
+    if (ny <= 512)
+    {
+        // Rows in -> out, then the columns in place, a group of planes at a time: the column pass finds the row
+        // pass's output of its group in the memory-side cache instead of HBM.
+        const V *src = static_cast<const V *>(in);
+        V *dst = static_cast<V *>(out);
+        for (long long p = 0; p < batch && e == hipSuccess; p += group)
+        {
+            const long long n = std::min(group, batch - p);
+            e = rows_any<SG, T>(nx, src + p * plane, dst + p * plane, n * ny, st);
+            if (e == hipSuccess)
+                e = columns_any<SG, T>(ny, dst + p * plane, nx, n, st);
+        }
+        return e;
+    }
+    // Four-step columns (in -> out, out -> scratch), then the rows scratch -> out, a group of planes at a time.
     float2 *scratch = nullptr;
     e = fft_plane_scratch(group * plane * static_cast<long long>(sizeof(V) / sizeof(float2)), scratch, 2);
     const V *src = static_cast<const V *>(in);

@@ -175,12 +175,38 @@ def test_fft2d_small_planes_beyond_one_grid(n, batch):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("nx, ny, fp64", [(512, 256, False), (256, 256, True)])
+def test_fft2d_rows_then_columns_in_groups(nx, ny, fp64):
+    """The row-then-column path runs 64 MB of planes at a time: 150 planes of 1 MB take groups of 64, 64 and 22
+    planes; every plane against numpy's FP64 transform, out of place and both signs."""
+    import torch
+    from idgx import FFT_FP32, FFT_FP64, fft2d
+
+    batch = 150
+    rng = np.random.default_rng(nx * ny + batch)
+    x = rng.standard_normal((batch, ny, nx)) + 1j * rng.standard_normal((batch, ny, nx))
+    x = x.astype(np.complex128 if fp64 else np.complex64)
+    for sign in (-1, 1):
+        d_in = torch.from_numpy(x.copy()).to("cuda")
+        d_out = torch.empty_like(d_in)
+        fft2d(d_in, d_out, nx, ny, batch, sign, FFT_FP64 if fp64 else FFT_FP32)
+        torch.cuda.synchronize()
+        got = d_out.cpu().numpy()
+        xd = x.astype(np.complex128)
+        ref = np.fft.fft2(xd) if sign < 0 else np.fft.ifft2(xd) * (nx * ny)
+        err = max(rel_l2(got[b], ref[b]) for b in range(batch))
+        print(f"fft2d {nx}x{ny} x {batch} {'fp64' if fp64 else 'fp32'} sign {sign:+d}: {err:.2e}", flush=True)
+        assert err < (1e-13 if fp64 else 1e-6)
+
+
+@pytest.mark.gpu
 def test_fft2d_24576_by_properties():
-    """The largest reference size (one 24576^2 FP32 plane, 4.8 GB; also the largest 3 x 2^k side a plane of at most
-    2^31 - 1 points admits), where numpy's FP64 transform is too slow for a test: (1) 48 unit impulses at seeded positions with seeded complex
-    amplitudes transform to a sum of plane waves, checked in FP64 at 4096 seeded output positions per sign (catches
-    index, sign and transposition mistakes of every pass); (2) a dense seeded plane transformed forward then inverse returns N^2 times itself (rounding of the full
-    pipeline on dense data), and Parseval's sum |X|^2 = N^2 sum |x|^2 holds."""
+    """The largest reference size (one 24576^2 FP32 plane, 4.8 GB; also the largest 3 x 2^k side that a plane of at
+    most 2^31 - 1 points admits), where numpy's FP64 transform is too slow for a test: (1) 48 unit impulses at seeded
+    positions with seeded complex amplitudes transform to a sum of plane waves, checked in FP64 at 4096 seeded output
+    positions per sign (catches index, sign and transposition mistakes of every pass); (2) a dense seeded plane
+    transformed forward then inverse returns N^2 times itself (rounding of the full pipeline on dense data), and
+    Parseval's sum |X|^2 = N^2 sum |x|^2 holds."""
     import torch
     from idgx import FFT_FP32, fft2d
 
