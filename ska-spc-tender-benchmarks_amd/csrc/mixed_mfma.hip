@@ -1029,7 +1029,7 @@ __global__ __launch_bounds__(kThreads) void degrid_prepare_kernel(KernelArgs a)
     const float2 *sub = reinterpret_cast<const float2 *>(a.subgrids) + static_cast<size_t>(s) * kCorr * P;
     uint4 *dst = const_cast<uint4 *>(a.prepared) + static_cast<size_t>(s) * nslots * 2;
     // Slots per thread held in registers between the FP16 scale and the split, so that the subgrid is read once:
-    // 4 covers S <= 32, 9 covers S <= 48; larger subgrids read it twice.
+    // 4 covers S <= 32, 9 covers S <= 48, 16 covers S <= 64; larger subgrids read it twice.
     constexpr int kCached = CACHED;
     if (nslots <= kCached * kThreads)
     {
@@ -1391,17 +1391,20 @@ hipError_t launch_degrid_prepare(const KernelArgs &a, int precision, hipStream_t
     if (!a.prepared || !a.prepared_scale || !a.slot_n)
         return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>(a.p.nr_subgrids));
-    // FP16 needs the subgrid's maximum before the split: hold up to 9 slots per thread for S <= 48 (BF16 has no
-    // scale and streams the slots in one pass either way).
+    // FP16 needs the subgrid's maximum before the split: hold up to 4, 9 or 16 slots per thread, so that subgrids
+    // up to S = 32, 48 or 64 and their A-terms are read once (BF16 has no scale and streams the slots in one pass
+    // either way).
     const int nslots = a.p.S * degrid_row_stride(a.p.S);
-    const bool wide = nslots > 4 * kThreads && nslots <= 9 * kThreads;
+    const int cached = nslots <= 4 * kThreads ? 4 : nslots <= 9 * kThreads ? 9 : 16;
     switch (precision)
     {
     case IDGX_MIXED_BF16X3:
         degrid_prepare_kernel<kBF16, true, 4><<<grid, kThreads, 0, st>>>(a);
         break;
     case IDGX_MIXED_FP16X3:
-        if (wide)
+        if (cached == 16)
+            degrid_prepare_kernel<kFP16, true, 16><<<grid, kThreads, 0, st>>>(a);
+        else if (cached == 9)
             degrid_prepare_kernel<kFP16, true, 9><<<grid, kThreads, 0, st>>>(a);
         else
             degrid_prepare_kernel<kFP16, true, 4><<<grid, kThreads, 0, st>>>(a);
@@ -1410,7 +1413,9 @@ hipError_t launch_degrid_prepare(const KernelArgs &a, int precision, hipStream_t
         degrid_prepare_kernel<kBF16, false, 4><<<grid, kThreads, 0, st>>>(a);
         break;
     case IDGX_MIXED_FP16:
-        if (wide)
+        if (cached == 16)
+            degrid_prepare_kernel<kFP16, false, 16><<<grid, kThreads, 0, st>>>(a);
+        else if (cached == 9)
             degrid_prepare_kernel<kFP16, false, 9><<<grid, kThreads, 0, st>>>(a);
         else
             degrid_prepare_kernel<kFP16, false, 4><<<grid, kThreads, 0, st>>>(a);
