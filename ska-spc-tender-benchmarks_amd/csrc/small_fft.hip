@@ -2,9 +2,10 @@
 // out[b][ky][kx] = sum in[b][y][x] exp(sign 2 pi i (ky y / ny + kx x / nx)), unnormalised, nx and ny powers of two.
 //
 // Everything happens in LDS tiles; every thread of a workgroup holds 16 complex points of a tile at a time.
-// - A plane with both sides <= 128 whose padded copy fits 132 KB of LDS (128 x 128 in FP32, 128 x 64 in FP64, and
-//   every smaller plane) takes one pass over HBM: a workgroup loads whole planes (several when a plane has fewer
-//   than 4096 points), transforms their rows, then their columns, and writes them once.
+// - FP32 64 x 64 and 128 x 128 planes: one pass with the plane in registers (plane_regs_kernel below).
+// - Other planes with both sides <= 128 whose padded copy fits 132 KB of LDS (128 x 64 in FP64, and every smaller
+//   plane) take one pass over HBM: a workgroup loads whole planes (several when a plane has fewer than 4096
+//   points), transforms their rows, then their columns, and writes them once.
 // - Otherwise, with ny <= 512: rows (max(1, 4096 / nx) whole rows per tile, contiguous loads; nx up to 16384 in
 //   FP32, 8192 in FP64, whose padded row fills 128 KB), then columns in place (a tile of ny rows x C columns,
 //   C = max(4096 / ny, 128 bytes / point), so that every row segment is a whole 128-byte line); 64 MB of planes at
@@ -327,6 +328,102 @@ __global__ __launch_bounds__((plane_threads<NX, NY>())) void small_fft_planes_ke
     }
 }
 
+// One pass for FP32 N x N planes, N = 64 or 128, with the plane in registers (16 points per thread) instead of in
+// LDS.  Thread t is lane j = t % R of line a = t / R (R = N / 16): a row in the row phase, a column in the column
+// phase.  Each phase is X[k + 16 q] = sum_j W_R^(j q) W_N^(j k) sum_m W_16^(m k) x[j + R m]: a radix-16 DFT over
+// the thread's points j + R m, the twiddles W_N^(j k), then for k = 16 j / R + kk (kk < 16 / R) a radix-R DFT over
+// the R lanes of the line, whose values an LDS exchange brings together.  Between the phases a transpose through
+// LDS gives thread (a, j) the points j + R m of column a.  LDS holds one float per point (the real parts, then the
+// imaginary parts, through the same array): 69 KB at N = 128, so two 1024-thread workgroups share a CU, where the
+// complex plane in LDS (small_fft_planes_kernel) takes 132 KB and leaves one.  Line exchanges pad one slot per 16
+// points (lanes j of 32-lane halves of a wave land in distinct banks both ways); the transpose pitch N + 32 / R
+// makes its column reads conflict-free.  At N = 128 the 64-VGPR limit leaves 24 bytes per lane in scratch; a form
+// that recomputed the line indices in each phase spilled less and measured slower (71.0 against 64.7 ns per plane).
+template <int N> constexpr int plane_regs_waves_per_simd()
+{
+    return N * N / 16 / 64 / 4 * 2; // two workgroups of N^2 / 16 threads per CU
+}
+
+template <int N, int SG>
+__global__ __launch_bounds__(N * N / 16, plane_regs_waves_per_simd<N>()) void plane_regs_kernel(const float2 *in,
+                                                                                               float2 *out,
+                                                                                               long long planes)
+{
+    constexpr int THREADS = N * N / 16, R = N / 16, KPT = 16 / R;
+    constexpr int LP = N + N / 16;      // line pitch of the line exchanges (one pad slot per 16 points)
+    constexpr int TP = N + 32 / R;      // row pitch of the transpose
+    constexpr int WORDS = N * (LP > TP ? LP : TP);
+    static_assert(N == 64 || N == 128, "N = 64 or 128");
+    __shared__ float buf[WORDS];
+    // tw[16 j + k] = W_N^(j k) (j < R, k < 16): a lane's 15 twiddles are one base address plus constant offsets.
+    __shared__ float2 tw[N];
+    for (int i = threadIdx.x; i < N; i += THREADS)
+    {
+        double sn, cs;
+        sincospi(2.0 * ((i / 16) * (i % 16)) / N, &sn, &cs);
+        tw[i] = make_float2(static_cast<float>(cs), static_cast<float>(SG * sn));
+    }
+    lds_barrier();
+    auto phys = [](int p) { return p + p / 16; };
+    // The new real parts go straight into v[].x (the old ones are in LDS by then) while v[].y still holds the
+    // imaginary parts to send: no temporary array.
+    auto exchange = [&](float2 *v, auto widx, auto ridx) {
+        lds_barrier(); // the previous exchange's reads are done
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            buf[widx(r)] = v[r].x;
+        lds_barrier();
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            v[r].x = buf[ridx(r)];
+        lds_barrier();
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            buf[widx(r)] = v[r].y;
+        lds_barrier();
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            v[r].y = buf[ridx(r)];
+    };
+    // One phase on the thread's line: points j + R m in v[m] -> X[KPT j + kk + 16 q] in v[kk R + q].
+    auto line_fft = [&](float2 *v, int a, int j) {
+        dft<16, SG>(v);
+#pragma unroll
+        for (int k = 1; k < 16; ++k)
+            v[k] = cmul(v[k], tw[16 * j + k]);
+        // y[k] of lane j at line position k R + j; lane j' takes positions 16 j' .. 16 j' + 15 = (k = KPT j' + kk,
+        // lane jj) for kk < KPT, jj < R.
+        exchange(
+            v, [&](int k) { return a * LP + phys(k * R + j); }, [&](int e) { return a * LP + phys(16 * j + e); });
+#pragma unroll
+        for (int kk = 0; kk < KPT; ++kk)
+            dft<R, SG>(v + kk * R);
+    };
+    const int nplanes = static_cast<int>(planes); // the entry point admits batches below 2^31
+    for (int p = blockIdx.x; p < nplanes; p += gridDim.x)
+    {
+        int t = threadIdx.x; // opaque per plane: indices are formed where they are used
+        asm volatile("" : "+v"(t));
+        const int a = t / R, j = t % R;
+        const float2 *src = in + static_cast<size_t>(p) * N * N;
+        float2 v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            v[m] = src[a * N + j + R * m];
+        line_fft(v, a, j); // rows
+        // Transpose: X of row a at column K = KPT j + kk + 16 q (in v[kk R + q]) -> thread (c, j) takes rows j + R m
+        // of column c = its line a.
+        exchange(
+            v, [&](int e) { return a * TP + KPT * j + e / R + 16 * (e % R); },
+            [&](int m) { return (j + R * m) * TP + a; });
+        line_fft(v, a, j); // columns
+        float2 *dst = out + static_cast<size_t>(p) * N * N;
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+            dst[(KPT * j + e / R + 16 * (e % R)) * N + a] = v[e];
+    }
+}
+
 // Row pass: max(1, kRowTile / NX) whole rows (of any plane) per tile, in -> out; NX up to 16384 (FP32) or 8192
 // (FP64), whose padded row fills 128 KB of LDS.
 template <int NX> constexpr int rows_per_tile()
@@ -461,6 +558,13 @@ __global__ __launch_bounds__((N2 * column_tile<N2, T>() / kPointsPerThread)) voi
 unsigned grid_for(long long tiles)
 {
     return static_cast<unsigned>(tiles < kMaxGrid ? tiles : kMaxGrid);
+}
+
+template <int N, int SG> hipError_t plane_regs_pass(const void *in, void *out, long long batch, hipStream_t st)
+{
+    plane_regs_kernel<N, SG><<<grid_for(batch), N * N / 16, 0, st>>>(static_cast<const float2 *>(in),
+                                                                      static_cast<float2 *>(out), batch);
+    return hipGetLastError();
 }
 
 template <int NX, int NY, int SG, class T>
@@ -651,6 +755,13 @@ hipError_t lds_fft(const void *in, void *out, int nx, int ny, long long batch, h
 {
     using V = typename Vec<T>::type;
     hipError_t e = hipSuccess;
+    if constexpr (sizeof(T) == sizeof(float))
+    {
+        if (nx == ny && nx == 64)
+            return plane_regs_pass<64, SG>(in, out, batch, st);
+        if (nx == ny && nx == 128)
+            return plane_regs_pass<128, SG>(in, out, batch, st);
+    }
     if (planes_any<SG, T>(nx, ny, in, out, batch, st, e, std::make_integer_sequence<int, 4>()))
         return e;
     const long long plane = static_cast<long long>(nx) * ny;
