@@ -272,19 +272,20 @@ int idgx_grid_clear_box(const idgx_plan *plan, void *d_grid, int nr_layers, cons
  *   d_out[b][ky][kx] = sum_{y < ny, x < nx} d_in[b][y][x] exp(sign 2 pi i (ky y / ny + kx x / nx)),
  * unnormalised, sign = -1 (heFFTe forward) or +1 (backward); complex64 planes, or complex128 with
  * precision = IDGX_FFT_FP64.  d_in == d_out transforms in place.  This library's kernels run:
- *   - nx, ny powers of two in 16 .. 512, FP32 or FP64: LDS tiles (one pass when both sides are <= 128 and the
- *     plane fits 132 KB of LDS, i.e. up to 128 x 128 in FP32 and 128 x 64 in FP64; else a row pass and a column
- *     pass);
+ *   - FP32 64 x 64 and 128 x 128: one pass with the plane in registers;
+ *   - other nx, ny powers of two in 16 .. 512, FP32 or FP64: LDS tiles (one pass when both sides are <= 128 and the
+ *     plane fits 132 KB of LDS, i.e. up to 128 x 64 in FP64; else a row pass and a column pass, 64 MB of planes at
+ *     a time);
  *   - other power-of-two shapes with nx up to 16384 (FP32) / 8192 (FP64) and ny up to 16384 (FP64 every such
  *     shape; FP32 those with a side below 1024): rows in one LDS pass, columns longer than 512 in two four-step
  *     passes through a scratch (their column tiles, 32 .. 128 columns, must divide nx);
  *   - FP32, nx and ny powers of two in 1024 .. 16384: the three-pass kernels;
- *   - FP32, nx = ny = 3 x (a power of two in 1024 .. 8192), e.g. 24576: three passes as well, the two column passes
- *     each opening with a radix-3 step along their axis and the row pass writing the interleaved output columns.
+ *   - FP32, nx = ny = 3 x (a power of two in 1024 .. 8192), e.g. 24576: three passes as well, the transforms of the
+ *     sub-rows x = 3 x' + b, then two column passes each opening with a radix-3 step (along y, then along x).
  * Planes hold at most 2^31 - 1 points.
- * The first call for a plane size on a device allocates the scratch those need (up to 64 MB, or one sub-plane
- * when larger; one whole plane for the 3 x 2^k sides; 64 MB of planes, or one plane when larger, for the four-step
- * columns), kept until the process ends.  Other shapes run rocFFT
+ * The first call for a plane size on a device allocates the scratch those need (the three-pass kernels and the
+ * four-step columns: 64 MB of planes, or one plane when larger; the 3 x 2^k sides: one plane), kept until the
+ * process ends.  Other shapes run rocFFT
  * (the first call for a shape creates its plan).  Calls that overlap on one device share the scratch, so issue
  * them on one stream. */
 enum
