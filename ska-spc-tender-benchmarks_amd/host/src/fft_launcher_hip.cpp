@@ -1,15 +1,15 @@
-// HIP backend of the FFT benchmark: the GPU launch of fft/src/fft_benchmark.cpp:240-345 over libidgx's
-// idgx_fft2d (include/idgx.h): this library's kernels for every plane with power-of-two sides in 16 .. 512 (FP32
-// and FP64; one pass up to 128 x 128, in registers for FP32 64 x 64 and 128 x 128), FP32 planes with power-of-two
-// sides in 1024 .. 16384 (three passes), FP32 square planes of side 3 x 2^k, 3072 .. 24576 (three passes, each
-// column pass opening with a radix-3 step) and the other power-of-two shapes with rows of up to 16384 (FP32) / 8192
-// (FP64) points (one-pass rows, four-step columns); rocFFT for the rest (idgx_fft2d_own_kernels tells which).  The sequence is the reference's: as many planes as fit in
-// memorysize (input + output planes; the reference also counts heFFTe's workspace), a first transform of the batch
-// timed as initialisation (plan or scratch creation; heFFTe's plan does not depend on the batch, rocFFT's does),
-// niterations uploads of the batch, niterations transforms of the batch, niterations downloads, then the inverse
-// transform of the output and the maximum error of the first plane against its input, the pattern k mod 128
-// (fft/src/fft_benchmark.cpp:82-108).  Times are per transform
-// (total / (batch x niterations)), bandwidths in bytes per microsecond.  HIP events time the device work.
+// HIP backend of the FFT benchmark: the GPU launch of fft/src/fft_benchmark.cpp:240-345 over libidgx's idgx_fft2d
+// (include/idgx.h): this library's kernels for every plane with power-of-two sides in 16 .. 512 (FP32 and FP64; one
+// pass up to 128 x 128, in registers for FP32 64 x 64 and 128 x 128), FP32 planes with power-of-two sides in 1024 ..
+// 16384 (three passes), FP32 square planes of side 3 x 2^k, 3072 .. 24576 (three passes, each column pass opening with
+// a radix-3 step) and the other power-of-two shapes with rows of up to 16384 (FP32) / 8192 (FP64) points (one-pass
+// rows, four-step columns); rocFFT for the rest (idgx_fft2d_own_kernels tells which).  The sequence is the reference's:
+// as many planes as fit in memorysize (input + output planes; the reference also counts heFFTe's workspace), a first
+// transform of the batch timed as initialisation (plan or scratch creation; heFFTe's plan does not depend on the batch,
+// rocFFT's does), niterations uploads of the batch, niterations transforms of the batch, niterations downloads, then
+// the inverse transform of the output and the maximum error of the first plane against its input, the pattern k mod 128
+// (fft/src/fft_benchmark.cpp:82-108).  Times are per transform (total / (batch x niterations)), bandwidths in bytes per
+// microsecond.  HIP events time the device work.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
