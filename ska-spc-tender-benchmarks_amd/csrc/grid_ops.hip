@@ -39,14 +39,22 @@ __host__ __device__ inline int adder_rows(int T)
 }
 
 __global__ __launch_bounds__(kThreads) void adder_kernel(const idgx_metadata *md, const int32_t *order,
-                                                         const int32_t *tile_start, int tile_begin, int S, int T,
-                                                         int G, int z_min, int nr_layers, const float2 *subgrids,
-                                                         float2 *grid)
+                                                         const int32_t *tile_start, int tile_begin, int ntiles, int S,
+                                                         int T, int G, int z_min, int nr_layers,
+                                                         const float2 *subgrids, float2 *grid)
 {
     const int R = adder_rows(T), nrb = (T + R - 1) / R;
     const int per_tile = 16 * nrb;
-    const int tile = tile_begin + static_cast<int>(blockIdx.x) / per_tile;
-    const int sub_b = static_cast<int>(blockIdx.x) % per_tile;
+    // Workgroup b -> (tile, sub-block): workgroups reach the 8 XCDs in turn (b % 8), so XCD x takes tiles x, x + 8,
+    // x + 16, ... of the colour (heaviest first), each with all its sub-blocks.  The rows of a subgrid that straddle
+    // the boundary between two quadrants are then read by workgroups of one XCD, so each cache line they share is
+    // fetched into one L2 once.  The grid is padded to a multiple of 8 tiles; the padding exits.
+    const int b = static_cast<int>(blockIdx.x), k = b >> 3;
+    const int tl = (k / per_tile) * 8 + (b & 7);
+    if (tl >= ntiles)
+        return; // uniform
+    const int tile = tile_begin + tl;
+    const int sub_b = k % per_tile;
     const int pol = sub_b / (4 * nrb), quad = (sub_b / nrb) & 3, rb = sub_b % nrb;
     const int first = tile_start[tile], last = tile_start[tile + 1];
     const idgx_metadata m0 = md[order[first]];
@@ -167,11 +175,12 @@ hipError_t launch_adder(const idgx_metadata *md, const int32_t *order, const int
         const long long n = color_begin[c + 1] - color_begin[c];
         if (n <= 0)
             continue;
-        if (n * per_tile > 0x7fffffffLL)
+        const long long padded = (n + 7) / 8 * 8; // whole groups of 8 tiles, one per XCD
+        if (padded * per_tile > 0x7fffffffLL)
             return hipErrorInvalidConfiguration;
-        adder_kernel<<<static_cast<unsigned>(n * per_tile), kThreads, 0, st>>>(
-            md, order, tile_start, color_begin[c], S, T, G, z_min, nr_layers, static_cast<const float2 *>(subgrids),
-            static_cast<float2 *>(grid));
+        adder_kernel<<<static_cast<unsigned>(padded * per_tile), kThreads, 0, st>>>(
+            md, order, tile_start, color_begin[c], static_cast<int>(n), S, T, G, z_min, nr_layers,
+            static_cast<const float2 *>(subgrids), static_cast<float2 *>(grid));
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess)
             return e;
