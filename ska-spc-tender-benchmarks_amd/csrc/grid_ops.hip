@@ -141,6 +141,8 @@ __global__ __launch_bounds__(kThreads) void adder_kernel(const idgx_metadata *md
 __global__ __launch_bounds__(kThreads) void splitter_kernel(const idgx_metadata *md, int S, int G, int z_min,
                                                             int nr_layers, const float2 *grid, float2 *subgrids)
 {
+    // Subgrids in launch order, spread over the XCDs: an XCD-contiguous order (neighbours in uv on one XCD) read
+    // 1.52 GB instead of 3.13 GB on C2 but took 0.98 ms instead of 0.84 ms.
     const long long s = blockIdx.x;
     const int x0 = md[s].x, y0 = md[s].y;
     const int P = S * S;

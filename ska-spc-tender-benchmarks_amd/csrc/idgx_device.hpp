@@ -114,6 +114,15 @@ template <int MODE> __device__ __forceinline__ void sincos_rad(float x, float &s
     }
 }
 
+// Workgroup b of n -> item, XCD-contiguous: workgroups reach the 8 XCDs round-robin (b % 8), so XCD x takes the
+// items [x per + min(x, rem), ...) in order.  A bijection for any n (and any XCD count: only locality changes).
+__device__ __forceinline__ int xcd_contiguous(int b, int n)
+{
+    constexpr int kXcds = 8;
+    const int per = n / kXcds, rem = n % kXcds, x = b % kXcds, k = b / kXcds;
+    return x * per + min(x, rem) + k;
+}
+
 // LDS-only workgroup barrier: this wave's LDS accesses are complete, then the workgroup meets.  Unlike
 // __syncthreads() it does not wait for outstanding global loads, so a persistent kernel's prefetch of its next
 // item stays in flight across the barriers of the current one (global memory is not exchanged through it).

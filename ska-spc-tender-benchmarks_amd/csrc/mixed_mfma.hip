@@ -990,15 +990,6 @@ __global__ void reset_counters_kernel(int *counters, int n)
         counters[threadIdx.x] = 0;
 }
 
-// Workgroup b of n -> item, XCD-contiguous: workgroups reach the 8 XCDs round-robin (b % 8), so XCD x takes the
-// items [x per + min(x, rem), ...) in order.  A bijection for any n (and any XCD count: only locality changes).
-__device__ __forceinline__ int xcd_contiguous(int b, int n)
-{
-    constexpr int kXcds = 8;
-    const int per = n / kXcds, rem = n % kXcds, x = b % kXcds, k = b / kXcds;
-    return x * per + min(x, rem) + k;
-}
-
 // One workgroup per subgrid: the taper x A-term-applied pixels of every slot, split into the packed words the
 // degridder stages (FP16: scaled by 2^-scale so that the largest value is about 2^14).  The workgroup of subgrid 0
 // also writes the slot n table.  Subgrids outside the arrays are skipped (the degridder skips them too).  Each XCD
