@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kThreads) void splitter_kernel(const idgx_metadata 
                                                             int nr_layers, const float2 *grid, float2 *subgrids)
 {
     // Subgrids in launch order, spread over the XCDs: an XCD-contiguous order (neighbours in uv on one XCD) read
-    // 1.52 GB instead of 3.13 GB on C2 but took 0.98 ms instead of 0.84 ms.
+    // 1.52 GB instead of 3.13 GB for 76,865 subgrids of 32^2 on a 4096^2 grid, but took 0.98 ms instead of 0.84 ms.
     const long long s = blockIdx.x;
     const int x0 = md[s].x, y0 = md[s].y;
     const int P = S * S;

@@ -190,10 +190,10 @@ def test_device_selection_lists_match_the_host_lists(uneven):
         assert np.array_equal(d_vi.cpu().numpy().view(np.uint32), c["vis_index"])
 
 
-# ---- config 4 on its own workload: the full 256-station plan, every k-th subgrid of each class ----------------------
+# ---- the batched uneven workload itself: the full 256-station plan, every k-th subgrid of each class -------------
 
 @pytest.fixture(scope="module")
-def c4():
+def uneven_plan():
     """The full batched_uneven plan (256 stations, 512 x 1 s, 128 channels 150-200 MHz, S = 24 / 32 / 48 by
     baseline-length tercile, time chunks log-uniform in [16, 256], G = 8192), and a subset of about 60 subgrids per
     class spread evenly over each class's plan, with seeded visibilities and per-baseline flags on their rows."""
@@ -203,7 +203,7 @@ def c4():
     return bo, sb
 
 
-def _c4_describe(sb):
+def _describe_classes(sb):
     for c in sb["classes"]:
         ch = c["channels"]
         print(f"  S={c['subgrid_size']}: {c['nr_subgrids']} subgrids, T {int(c['metadata']['nr_timesteps'].min())}.."
@@ -212,11 +212,11 @@ def _c4_describe(sb):
 
 
 @pytest.mark.parametrize("precision", ["fp16x3", "bf16x3"])
-def test_c4_subset_single_launch_vs_fp64(c4, precision):
+def test_batched_uneven_subset_single_launch_vs_fp64(uneven_plan, precision):
     """The default single persistent launch per direction over the three classes of the real plan against the FP64
     oracle: worst subgrid <= 5e-5 (gridder), per class <= 5e-5 (degridder); flagged samples left unwritten."""
-    _, sb = c4
-    _c4_describe(sb)
+    _, sb = uneven_plan
+    _describe_classes(sb)
     flags, C = sb["flags"], sb["wavenumbers"].shape[0]
     vis0 = np.where(flags[..., None], 0, sb["visibilities"]).astype(np.complex64)
     db = DeviceBatch(sb["classes"], precision=precision)
@@ -231,15 +231,15 @@ def test_c4_subset_single_launch_vs_fp64(c4, precision):
         v64 = oracle_degrid(dict(c, visibilities=vis0), got, fp64=True)
         sel = _window_mask(c, sb["nr_rows"], C) & ~flags
         ev = rel_l2(vis[sel], v64[sel])
-        print(f"C4 {precision} S={c['subgrid_size']}: gridder worst subgrid vs FP64 {eg.max():.3e}, degridder {ev:.3e}")
+        print(f"batched_uneven {precision} S={c['subgrid_size']}: gridder worst subgrid vs FP64 {eg.max():.3e}, degridder {ev:.3e}")
         assert eg.max() <= BAR and ev <= BAR
 
 
-def test_c4_subset_plain_vs_cpu_restatement(c4):
+def test_batched_uneven_subset_plain_vs_cpu_restatement(uneven_plan):
     """Plain FP32 on the real plan's classes, against the CPU restatement of the reference path with the flagged
     samples zeroed (<= 1e-5 per subgrid / per class): the plain batch (one launch per direction, flagged samples
     skipped) and the one-shot kernels per class (every sample of each window)."""
-    _, sb = c4
+    _, sb = uneven_plan
     flags, C = sb["flags"], sb["wavenumbers"].shape[0]
     vis0 = np.where(flags[..., None], 0, sb["visibilities"]).astype(np.complex64)
     db = DeviceBatch(sb["classes"], precision="fp32")
@@ -261,17 +261,17 @@ def test_c4_subset_plain_vs_cpu_restatement(c4):
         eo = per_subgrid_rel_l2(one, ref)
         dw.degrid(plan)
         eov = rel_l2(dw.visibilities_out()[sel], oracle_degrid(cw, one)[sel])
-        print(f"C4 plain S={c['subgrid_size']}: batch gridder worst subgrid vs CPU {eg.max():.3e}, degridder "
+        print(f"batched_uneven plain S={c['subgrid_size']}: batch gridder worst subgrid vs CPU {eg.max():.3e}, degridder "
               f"{ev:.3e}; one-shot {eo.max():.3e} / {eov:.3e}")
         assert max(eg.max(), ev, eo.max(), eov) <= 1e-5
 
 
-def test_c4_device_planner_matches_the_host_planner(c4):
-    """idgx_plan_segments on the full C4 geometry (every ragged time segment of every class) gives the host
+def test_batched_uneven_device_planner_matches_the_host_planner(uneven_plan):
+    """idgx_plan_segments on the full batched_uneven geometry (every ragged time segment of every class) gives the host
     planner's metadata and channel windows bit for bit."""
     from idgx import plan_segments_device
 
-    bo, _ = c4
+    bo, _ = uneven_plan
     cfg = bo["config"]
     total = 0
     for c in bo["classes"]:
@@ -281,4 +281,4 @@ def test_c4_device_planner_matches_the_host_planner(c4):
         assert md_d.size == c["metadata"].size
         assert md_d.tobytes() == c["metadata"].tobytes() and ch_d.tobytes() == c["channels"].tobytes()
         total += md_d.size
-    print(f"C4 device plan: {total} subgrids in {len(bo['classes'])} classes, bit-identical to the host plan")
+    print(f"batched_uneven device plan: {total} subgrids in {len(bo['classes'])} classes, bit-identical to the host plan")
