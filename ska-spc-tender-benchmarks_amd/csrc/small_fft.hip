@@ -7,7 +7,8 @@
 //   plane) take one pass over HBM: a workgroup loads whole planes (several when a plane has fewer than 4096
 //   points), transforms their rows, then their columns, and writes them once.
 // - Otherwise, with ny <= 512: rows (max(1, 4096 / nx) whole rows per tile, contiguous loads; nx up to 16384 in
-//   FP32, 8192 in FP64, whose padded row fills 128 KB), then columns in place (a tile of ny rows x C columns,
+//   FP32, 8192 in FP64, whose padded row fills 128 KB; FP32 rows of 256 and 512 points in registers,
+//   rows_regs_kernel below), then columns in place (a tile of ny rows x C columns,
 //   C = max(4096 / ny, 128 bytes / point), so that every row segment is a whole 128-byte line); 64 MB of planes at
 //   a time, so that the column pass reads the row pass's output from the memory-side cache.
 // - With ny > 512: columns of ny = N1 N2 points in two four-step passes over tiles of C columns (A, in place: the
@@ -424,6 +425,106 @@ __global__ __launch_bounds__(N * N / 16, plane_regs_waves_per_simd<N>()) void pl
     }
 }
 
+// Rows of N = 256 or 512 points (FP32) with the row in registers: 256 threads, 4096 / N rows per tile; thread
+// (r, j) of the R = N / 16 lanes of row r takes its points j + R m.  A radix-16 DFT over m and the twiddles
+// W_N^(j k) give y_j[k], and an exchange through LDS within the row gives
+//   N = 256: lane k the values y_j[k] (j < 16), whose radix-16 DFT over j is X[k + 16 q];
+//   N = 512: lane 2 kk + j1 the values y_{j1 + 2 j2}[kk] (j2 < 16); their radix-16 DFT over j2 is Z_j1[q], and
+//            X[kk + 16 q + 256 j1] = Z_0[q] + (-1)^j1 W_32^q Z_1[q], a butterfly between neighbouring lanes (a DPP
+//            lane swap).
+// LDS holds one float per point (real parts, then imaginary parts) with one pad slot per 16 points.
+template <int N, int SG> __global__ __launch_bounds__(256) void rows_regs_kernel(const float2 *in, float2 *out, long long rows)
+{
+    constexpr int THREADS = 256, R = N / 16, ROWS = THREADS / R, LP = N + N / 16;
+    static_assert(N == 256 || N == 512, "N = 256 or 512");
+    __shared__ float buf[ROWS * LP];
+    // tw[16 j + k] = W_N^(j k) for j < R, k < 16 (at N = 512, j = 16 gives W_32^k).
+    __shared__ float2 tw[N];
+    for (int i = threadIdx.x; i < N; i += THREADS)
+    {
+        double sn, cs;
+        sincospi(2.0 * ((i / 16) * (i % 16)) / N, &sn, &cs);
+        tw[i] = make_float2(static_cast<float>(cs), static_cast<float>(SG * sn));
+    }
+    lds_barrier();
+    auto phys = [](int p) { return p + p / 16; };
+    const long long tiles = (rows + ROWS - 1) / ROWS;
+    for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    {
+        int t = threadIdx.x; // opaque per tile: indices are formed where they are used
+        asm volatile("" : "+v"(t));
+        const int r = t / R, j = t % R;
+        const long long row = tile * ROWS + r;
+        const bool live = row < rows; // lanes of a missing last-tile row still join the barriers
+        const float2 *src = in + (live ? row : 0) * N;
+        float2 v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            v[m] = live ? src[j + R * m] : make_float2(0.F, 0.F);
+        dft<16, SG>(v);
+#pragma unroll
+        for (int k = 1; k < 16; ++k)
+            v[k] = cmul(v[k], tw[16 * j + k]);
+        // y_j[k] at row position k R + j; the reads gather 16 values per lane (see above).
+        auto widx = [&](int k) { return r * LP + phys(k * R + j); };
+        auto ridx = [&](int e) {
+            if constexpr (N == 256)
+                return r * LP + phys(16 * j + e);
+            else
+                return r * LP + phys(32 * (j >> 1) + (j & 1) + 2 * e);
+        };
+        lds_barrier(); // the previous tile's reads are done
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            buf[widx(k)] = v[k].x;
+        lds_barrier();
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+            v[e].x = buf[ridx(e)];
+        lds_barrier();
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            buf[widx(k)] = v[k].y;
+        lds_barrier();
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+            v[e].y = buf[ridx(e)];
+        dft<16, SG>(v);
+        float2 *dst = out + (live ? row : 0) * N;
+        if constexpr (N == 256)
+        {
+            if (live)
+            {
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+                    dst[j + 16 * q] = v[q];
+            }
+        }
+        else
+        {
+            const int kk = j >> 1, j1 = j & 1;
+            // Neighbouring lanes 2 kk and 2 kk + 1 swap values: DPP quad_perm [1, 0, 3, 2].
+            auto swap = [](float x) {
+                return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
+            };
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+            {
+                if (j1)
+                    v[q] = cmul(v[q], tw[256 + q]);
+                const float2 o = make_float2(swap(v[q].x), swap(v[q].y));
+                v[q] = j1 ? make_float2(o.x - v[q].x, o.y - v[q].y) : make_float2(v[q].x + o.x, v[q].y + o.y);
+            }
+            if (live)
+            {
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+                    dst[kk + 16 * q + 256 * j1] = v[q];
+            }
+        }
+    }
+}
+
 // Row pass: max(1, kRowTile / NX) whole rows (of any plane) per tile, in -> out; NX up to 16384 (FP32) or 8192
 // (FP64), whose padded row fills 128 KB of LDS.
 template <int NX> constexpr int rows_per_tile()
@@ -560,6 +661,14 @@ unsigned grid_for(long long tiles)
     return static_cast<unsigned>(tiles < kMaxGrid ? tiles : kMaxGrid);
 }
 
+template <int N, int SG> hipError_t rows_regs_pass(const void *in, void *out, long long rows, hipStream_t st)
+{
+    constexpr int ROWS = 256 / (N / 16);
+    rows_regs_kernel<N, SG><<<grid_for((rows + ROWS - 1) / ROWS), 256, 0, st>>>(static_cast<const float2 *>(in),
+                                                                                static_cast<float2 *>(out), rows);
+    return hipGetLastError();
+}
+
 template <int N, int SG> hipError_t plane_regs_pass(const void *in, void *out, long long batch, hipStream_t st)
 {
     plane_regs_kernel<N, SG><<<grid_for(batch), N * N / 16, 0, st>>>(static_cast<const float2 *>(in),
@@ -611,9 +720,15 @@ template <int SG, class T> hipError_t rows_any(int nx, const void *in, void *out
     case 128:
         return rows_pass<128, SG, T>(in, out, rows, st);
     case 256:
-        return rows_pass<256, SG, T>(in, out, rows, st);
+        if constexpr (sizeof(T) == sizeof(float))
+            return rows_regs_pass<256, SG>(in, out, rows, st);
+        else
+            return rows_pass<256, SG, T>(in, out, rows, st);
     case 512:
-        return rows_pass<512, SG, T>(in, out, rows, st);
+        if constexpr (sizeof(T) == sizeof(float))
+            return rows_regs_pass<512, SG>(in, out, rows, st);
+        else
+            return rows_pass<512, SG, T>(in, out, rows, st);
     case 1024:
         return rows_pass<1024, SG, T>(in, out, rows, st);
     case 2048:
