@@ -433,7 +433,8 @@ __global__ __launch_bounds__(N * N / 16, plane_regs_waves_per_simd<N>()) void pl
 //            X[kk + 16 q + 256 j1] = Z_0[q] + (-1)^j1 W_32^q Z_1[q], a butterfly between neighbouring lanes (a DPP
 //            lane swap).
 // LDS holds one float per point (real parts, then imaginary parts) with one pad slot per 16 points.
-template <int N, int SG> __global__ __launch_bounds__(256) void rows_regs_kernel(const float2 *in, float2 *out, long long rows)
+template <int N, int SG>
+__global__ __launch_bounds__(256) void rows_regs_kernel(const float2 *in, float2 *out, long long rows)
 {
     constexpr int THREADS = 256, R = N / 16, ROWS = THREADS / R, LP = N + N / 16;
     static_assert(N == 256 || N == 512, "N = 256 or 512");
