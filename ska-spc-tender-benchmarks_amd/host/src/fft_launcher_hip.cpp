@@ -166,6 +166,8 @@ namespace fft_benchmark
     {
         if (c.htype != benchmarks_common::hardware_type::gpu)
             return benchmark_result{};
+        // double runs FP64 planes; the reference's GPU branch for double instantiates its single-precision
+        // transform instead (fft/src/fft_benchmark.cpp:62-65).
         return c.ftype == float_type::double_precision ? run<double>(c) : run<float>(c);
     }
 } // namespace fft_benchmark
