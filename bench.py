@@ -352,13 +352,16 @@ def time_cycle(runner, dw, steps, warmup, stream):
     FFT, add to the grid, master-grid FFT to the image and back (deconvolution would sit between the two), split
     from the grid, inverse subgrid FFT, degrid (HIP events on `stream`).  A plan with w-layers adds every subgrid to
     the grid of its layer, transforms every layer and combines the layers into one image with the w-screens (and
-    back), all on resident [layers][4][G][G] grids.  The master-grid transforms are restricted to the subgrids'
-    bounding box (idgx_grid_fft_box: the uv grids are zero outside it on the way to the image, and only its rows are
-    needed on the way back), and the clear zeroes only what the transform to the image reads (idgx_grid_clear_box);
-    the full transforms are timed beside the cycle."""
+    back), all on resident [layers][4][G][G] grids, in one call each way (idgx_grid_to_image, idgx_image_to_grid:
+    the screens and the sum over layers run inside the row pass).  The master-grid transforms are restricted to the
+    subgrids' bounding box (the uv grids are zero outside it on the way to the image, and only its rows are needed
+    on the way back), and the clear zeroes only what the transform to the image reads (idgx_grid_clear_box).  Timed
+    beside the cycle: the full transforms and, with w-layers, the separate calls (idgx_grid_fft_box and
+    idgx_wstack_to_image, and back)."""
     import torch
 
-    from idgx import FFT_BACKWARD, FFT_FORWARD, grid_box, grid_clear_box, grid_fft_box
+    from idgx import FFT_BACKWARD, FFT_FORWARD, grid_box, grid_clear_box, grid_fft_box, wstack_from_image, \
+        wstack_to_image
 
     plan = dw.plan()
     box = grid_box(plan, dw.w["metadata"])
@@ -371,13 +374,14 @@ def time_cycle(runner, dw, steps, warmup, stream):
               ("gridder", lambda: runner.grid(stream)),
               ("subgrid_fft", lambda: dw.fft(plan, FFT_FORWARD, d_sub, stream)),
               ("adder", (lambda: dw.add_layered(plan, d_grid, d_sub, stream)) if layered else
-               (lambda: dw.add(plan, d_grid, d_sub, stream))),
-              ("grid_fft_to_image", lambda: grid_fft_box(plan, d_grid, nl, FFT_BACKWARD, box, stream))]
+               (lambda: dw.add(plan, d_grid, d_sub, stream)))]
     if layered:
-        stages += [("wstack_to_image", lambda: dw.to_image(plan, d_grid, d_image, stream)),
-                   ("wstack_from_image", lambda: dw.from_image(plan, d_image, d_grid, stream))]
-    stages += [("grid_fft_to_uv", lambda: grid_fft_box(plan, d_grid, nl, FFT_FORWARD, box, stream)),
-               ("splitter", (lambda: dw.split_layered(plan, d_grid, d_sub, stream)) if layered else
+        stages += [("grid_to_image", lambda: dw.grid_to_image(plan, d_grid, d_image, box, stream)),
+                   ("image_to_grid", lambda: dw.image_to_grid(plan, d_image, d_grid, box, stream))]
+    else:
+        stages += [("grid_fft_to_image", lambda: grid_fft_box(plan, d_grid, nl, FFT_BACKWARD, box, stream)),
+                   ("grid_fft_to_uv", lambda: grid_fft_box(plan, d_grid, nl, FFT_FORWARD, box, stream))]
+    stages += [("splitter", (lambda: dw.split_layered(plan, d_grid, d_sub, stream)) if layered else
                 (lambda: dw.split(plan, d_grid, d_sub, stream))),
                ("subgrid_ifft", lambda: dw.fft(plan, FFT_BACKWARD, d_sub, stream)),
                ("degridder", lambda: runner.degrid(stream))]
@@ -399,6 +403,16 @@ def time_cycle(runner, dw, steps, warmup, stream):
         # The full transforms of the same grids, for comparison (not part of the cycle).
         full = {"to_image": lambda: dw.grid_fft(plan, d_grid, nl, FFT_BACKWARD, stream),
                 "to_uv": lambda: dw.grid_fft(plan, d_grid, nl, FFT_FORWARD, stream)}
+        if layered:
+            def separate_to_image():
+                grid_fft_box(plan, d_grid, nl, FFT_BACKWARD, box, stream)
+                wstack_to_image(plan, d_grid, z_min, nl, d_image, stream)
+
+            def separate_to_uv():
+                wstack_from_image(plan, d_image, z_min, nl, d_grid, stream)
+                grid_fft_box(plan, d_grid, nl, FFT_FORWARD, box, stream)
+
+            full.update(separate_to_image=separate_to_image, separate_to_uv=separate_to_uv)
         full_ms = {}
         for name, f in full.items():
             f()
@@ -813,11 +827,11 @@ def main():
         total = sum(cycle.values())
         cycle = {"stages_ms": cycle, "total_ms": total, "cycle_mvis_s": nvis / (total / 1e3) / 1e6,
                  "precision": head["precision"], "path": head["path"], "grid_fft_box": list(box),
-                 "w_layers": nl, "full_grid_fft_ms": full_ms}
+                 "w_layers": nl, "beside_the_cycle_ms": full_ms}
         print(f"bench: imaging cycle ({head['precision']} {head['path']}): " +
               ", ".join(f"{k} {v:.2f} ms" for k, v in cycle["stages_ms"].items()) +
-              f"; total {total:.2f} ms; master-grid transforms on the box {list(box)} (full transforms: to image "
-              f"{full_ms['to_image']:.2f} ms, to uv {full_ms['to_uv']:.2f} ms)", file=sys.stderr, flush=True)
+              f"; total {total:.2f} ms; master-grid transforms on the box {list(box)} (beside the cycle: " +
+              ", ".join(f"{k} {v:.2f} ms" for k, v in full_ms.items()) + ")", file=sys.stderr, flush=True)
     tg, td = head["gridder_ms"] / 1e3, head["degridder_ms"] / 1e3
     dominant, t_dom = ("degridder", td) if td > tg else ("gridder", tg)
     bytes_dom = bytes_by_direction[dominant]

@@ -309,6 +309,23 @@ int idgx_wstack_to_image(const idgx_plan *plan, const void *d_layers, int z_min,
 int idgx_wstack_from_image(const idgx_plan *plan, const void *d_image, int z_min, int nr_layers, void *d_layers,
                            void *stream);
 
+/* The image side of an imaging cycle in one call: the boxed master-grid transform and the w-screens together.
+ *   idgx_grid_to_image: d_image = idgx_wstack_to_image(idgx_grid_fft_box(BACKWARD, d_grid)), the grid's layers
+ *     zero outside the box as idgx_grid_fft_box requires; d_grid is left undefined (it serves as scratch);
+ *   idgx_image_to_grid: d_grid = idgx_grid_fft_box(FORWARD, idgx_wstack_from_image(d_image)) in the box's cells
+ *     (the rest of the grid undefined, as for idgx_grid_fft_box FORWARD).
+ * For the three-pass sizes (G = 1024 ... 16384) the screens and the sum over layers run inside the row pass: each
+ * image row is transformed for every layer while n(l, m) stays in registers, so neither the layers' image-domain
+ * planes nor a separate w-screen pass touch HBM.  The screens there take v_sin / v_cos of the FP64-reduced fraction
+ * of a turn (about 1e-7 absolute), not FP64 sincospi.  idgx_grid_to_image keeps a scratch of
+ * nr_layers x G x (box width rounded out to 64-column tiles) cells per device, allocated on first use and grown
+ * (after a device synchronisation) when a call needs more.  Other sizes run the two calls one after the other.
+ * No counterpart in the reference (its benchmark stops at the subgrids).  d_image must not alias d_grid. */
+int idgx_grid_to_image(const idgx_plan *plan, void *d_grid, int z_min, int nr_layers, const int32_t box[4],
+                       void *d_image, void *stream);
+int idgx_image_to_grid(const idgx_plan *plan, const void *d_image, int z_min, int nr_layers, const int32_t box[4],
+                       void *d_grid, void *stream);
+
 /* ---- Device-side planner (no reference counterpart: the reference draws random metadata,
  * gridding/src/gridding_initialization.cpp:158-182) -------------------------------------------------------------
  *

@@ -444,6 +444,38 @@ extern "C" int idgx_wstack_from_image(const idgx_plan *plan, const void *d_image
                                          plan->w_step_in_lambda, false, static_cast<hipStream_t>(stream)));
 }
 
+namespace
+{
+int grid_wstack(const idgx_plan *plan, void *d_grid, void *d_image, int z_min, int nr_layers, const int32_t *box,
+                bool to_image, void *stream)
+{
+    const int rc = check_plan_struct(plan);
+    if (rc != IDGX_OK)
+        return rc;
+    const int G = plan->grid_size;
+    if (nr_layers < 1 || !d_grid || !d_image || d_grid == d_image || !box || box[0] < 0 || box[0] > box[1] ||
+        box[1] > G || box[2] < 0 || box[2] > box[3] || box[3] > G)
+        return IDGX_ERR_INVALID_ARGUMENT;
+    if (!idgx::grid_fft_supported(G))
+        return IDGX_ERR_UNSUPPORTED;
+    const int b[4] = {box[0], box[1], box[2], box[3]};
+    return to_status(idgx::launch_grid_wstack(d_grid, d_image, G, z_min, nr_layers, plan->image_size,
+                                              plan->w_step_in_lambda, to_image, b, static_cast<hipStream_t>(stream)));
+}
+} // namespace
+
+extern "C" int idgx_grid_to_image(const idgx_plan *plan, void *d_grid, int z_min, int nr_layers, const int32_t box[4],
+                                  void *d_image, void *stream)
+{
+    return grid_wstack(plan, d_grid, d_image, z_min, nr_layers, box, true, stream);
+}
+
+extern "C" int idgx_image_to_grid(const idgx_plan *plan, const void *d_image, int z_min, int nr_layers,
+                                  const int32_t box[4], void *d_grid, void *stream)
+{
+    return grid_wstack(plan, d_grid, const_cast<void *>(d_image), z_min, nr_layers, box, false, stream);
+}
+
 extern "C" size_t idgx_plan_workspace_bytes(int64_t nr_segments)
 {
     return nr_segments < 0 ? 0 : idgx::plan_workspace_bytes(nr_segments);

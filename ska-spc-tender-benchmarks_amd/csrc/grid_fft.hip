@@ -21,14 +21,17 @@
 //   pass 3 (scratch -> grid): the G-point transform of each row, the whole row in LDS.
 // The column passes take tiles of 64 columns (one wave reads a 512-byte row segment), so every pass reads and
 // writes whole cache lines: 3 x (read + write) of the grid, against 2 checkerboard passes plus the library
-// transform otherwise.  All three kernels are persistent (as many workgroups as are resident): each loads its next
-// tile or row into registers while it transforms the current one, and its LDS barriers do not wait for those
-// loads.  Other sizes use rocFFT (hipFFT C2C, batched over correlations and layers) between two checkerboard passes.
+// transform otherwise.  All three kernels are persistent (as many workgroups as are resident); the column kernels
+// load their next tile into registers while they transform the current one (their LDS barriers do not wait for
+// those loads), the row kernel holds only its row, which keeps two workgroups on a CU.  Other sizes use rocFFT
+// (hipFFT C2C, batched over correlations and layers) between two checkerboard passes.
 //
 // W-stacking.  A subgrid with w-layer z is gridded relative to w_z = (z + 0.5) w_step (in wavelengths), so layer
 // z of the dirty image carries exp(+2 pi i w_z n(l, m)) relative to the true one, n = 1 - sqrt(1 - l^2 - m^2)
 // written as (l^2 + m^2) / (1 + sqrt(1 - l^2 - m^2)) (gridding/inc/math.hpp:29-37).  to_image sums the layers
-// after multiplying each by exp(-2 pi i w_z n); from_image is its adjoint.  The phases are reduced in FP64.
+// after multiplying each by exp(-2 pi i w_z n); from_image is its adjoint.  The phases are reduced in FP64.  Either
+// as a pass of its own over the image-domain layers, or (three-pass sizes) inside the row pass, which then
+// transforms an image row of every layer and applies the screens there (wstack_row_kernel).
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
@@ -221,6 +224,10 @@ struct FftArgs
     int cx0, cx1;      // row pass: scratch columns outside [cx0, cx1) are read as zero (0, gx: dense)
     int ry0, nry;      // row pass: rows [ry0, ry0 + nry) of each plane (0, gy: dense)
     int rows_first;    // 1: the row pass runs first, in place (scratch == grid), and takes the input signs
+    // Column pass 2 stores row y, column x of plane pl at scratch[pl splane + y spitch + x - sx0]: (gx gy, gx, 0)
+    // for whole planes, (gy nc, nc, first column) for a box of nc columns stored compactly.
+    long long splane;
+    int spitch, sx0;
 };
 
 constexpr int ilog2(int n)
@@ -251,32 +258,20 @@ template <int SG> __device__ __forceinline__ float2 twiddle(int q, int log_g)
 // 3-6 % at 16384, and 10 % on the boxed transform to the image, whose row reads are mostly skipped.
 // The second __launch_bounds__ argument is HIP's minimum number of waves per SIMD (amdgpu-waves-per-eu): two
 // workgroups of L / 16 threads per CU put max(2, L / 2048) waves on each SIMD, which caps the kernel at 512 / that
-// many VGPRs (64 at L = 16384), so the two-workgroup occupancy described above is enforced, not incidental.
-template <int L> constexpr int fft_row_waves_per_simd()
+// many VGPRs (64 at L = 16384), so the two-workgroup occupancy described above is enforced, not incidental.  The
+// plain rows of 8192 points take four workgroups per CU (8 waves per SIMD, 64 VGPRs), which they fit without
+// spilling; the R3 form there would spill and keeps two.
+template <int L, bool R3> constexpr int fft_row_waves_per_simd()
 {
-    return L / 2048 > 2 ? L / 2048 : 2;
+    return L == 8192 && !R3 ? 8 : L / 2048 > 2 ? L / 2048 : 2;
 }
 
-// R3: the first pass of the N x N planes of fft_col3_kernel (N = 3 L = A.gx, not centred, dense): the L-point
-// transforms of the sub-rows x = 3 x' + b (b = 0, 1, 2) of each input row (A.scratch), stored as segment b
-// (columns [b L, b L + L)) of that row in A.grid.
-template <int L, int SG, bool R3 = false>
-__global__ __launch_bounds__(L / 16, fft_row_waves_per_simd<L>()) void fft_row_kernel(FftArgs A)
+// Twiddles W_L^q = exp(SG 2 pi i q / L) of the row transforms as t_hi[q / 64] t_lo[q % 64]: two LDS reads and a
+// complex product instead of a sine and a cosine at quarter rate.  The L / 64 + 64 entries are rounded from FP64
+// values once per workgroup of L / 16 threads (the caller synchronises before use).
+template <int L, int SG> __device__ __forceinline__ void row_twiddle_tables(float2 (&t_hi)[L / 64], float2 (&t_lo)[64])
 {
-    constexpr int kLog = ilog2(L);
-    constexpr int kStages16 = kLog / 4;
-    constexpr int kRl = 1 << (kLog % 4);
-    constexpr int kQ = L / 16;
-    static_assert(kStages16 >= 2, "L = 16^a x {1, 2, 4, 8}, a >= 2");
-    // LDS slot of element i is i + i / 16.  Every access below is a per-thread base plus a compile-time offset:
-    // for the elements j + t kQ (kQ a multiple of 16) the slot is slot(j) + t (kQ + kQ / 16).
-    constexpr int kQs = kQ + kQ / 16;
-    constexpr int kRs = L / kRl + L / (16 * kRl);
-    __shared__ float buf[L + L / 16];
-    // Twiddles W_L^q = T_hi[q / 64] T_lo[q % 64]: two LDS reads and a complex product instead of a sine and a cosine
-    // at quarter rate.  The L / 64 + 64 entries are rounded from FP64 values once per workgroup.
-    __shared__ float2 t_hi[L / 64], t_lo[64];
-    for (int i = threadIdx.x; i < L / 64 + 64; i += kQ)
+    for (int i = threadIdx.x; i < L / 64 + 64; i += L / 16)
     {
         const int qq = i < L / 64 ? 64 * i : i - L / 64;
         double sn, cs;
@@ -287,6 +282,109 @@ __global__ __launch_bounds__(L / 16, fft_row_waves_per_simd<L>()) void fft_row_k
         else
             t_lo[i - L / 64] = w;
     }
+}
+
+// The L-point transform of one row held by the workgroup's L / 16 threads, 16 values each: thread j0 holds elements
+// j0 + r L / 16 in v[r].  Stages: radix 16 (L = 16^a x {1, 2, 4, 8}, a >= 2) with an LDS exchange through buf
+// (L + L / 16 floats) after each, then a radix-kRl stage when L is not a power of 16.  Output element k, the idx-th
+// (0 .. 15, a compile-time index after unrolling) of this thread's, goes to emit(idx, k, value).  The first exchange
+// waits for every thread's reads of buf from the previous call.
+template <int L, int SG, class Emit>
+__device__ __forceinline__ void row_transform(float2 (&v)[16], int j0, float (&buf)[L + L / 16],
+                                              const float2 (&t_hi)[L / 64], const float2 (&t_lo)[64], Emit &&emit)
+{
+    constexpr int kLog = ilog2(L);
+    constexpr int kStages16 = kLog / 4;
+    constexpr int kRl = 1 << (kLog % 4);
+    constexpr int kQ = L / 16;
+    static_assert(kStages16 >= 2, "L = 16^a x {1, 2, 4, 8}, a >= 2");
+    // LDS slot of element i is i + i / 16.  Every access below is a per-thread base plus a compile-time offset:
+    // for the elements j + t kQ (kQ a multiple of 16) the slot is slot(j) + t (kQ + kQ / 16).
+    constexpr int kQs = kQ + kQ / 16;
+    constexpr int kRs = L / kRl + L / (16 * kRl);
+    auto tw = [&](int qq) {
+        qq &= L - 1;
+        return f2mul(t_hi[qq >> 6], t_lo[qq & 63]);
+    };
+    int Ns = 1;
+#pragma unroll
+    for (int st = 0; st < kStages16; ++st)
+    {
+        if (st > 0)
+        {
+            const int m = (j0 & (Ns - 1)) * (L / (16 * Ns));
+#pragma unroll
+            for (int r = 1; r < 16; ++r)
+                v[r] = f2mul(v[r], tw(m * r));
+        }
+        dft<16, SG>(v);
+        if (st == kStages16 - 1 && kRl == 1)
+        {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                emit(r, j0 + r * Ns, v[r]);
+        }
+        else
+        {
+            // Output r of butterfly j0 goes to element d + r Ns: slot wb + r (Ns + Ns / 16), or 17 j0 + r when
+            // Ns = 1.  The next stage reads elements j0 + t kQ (slot rb + t kQs), or for the radix-kRl stage
+            // j0 + b kQ + r (L / kRl) (slot rb + b kQs + r kRs).
+            const int d = (j0 / Ns) * Ns * 16 + (j0 & (Ns - 1));
+            const int wb = d + (d >> 4), ws = Ns == 1 ? 1 : Ns + Ns / 16;
+            const int rb = j0 + (j0 >> 4);
+            const bool next16 = st + 1 < kStages16;
+            auto rslot = [&](int t) { return next16 ? rb + t * kQs : rb + (t / kRl) * kQs + (t % kRl) * kRs; };
+            float re[16];
+            lds_barrier(); // the previous exchange's reads are done
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                buf[wb + r * ws] = v[r].x;
+            lds_barrier();
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                re[t] = buf[rslot(t)];
+            lds_barrier();
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                buf[wb + r * ws] = v[r].y;
+            lds_barrier();
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                v[t] = make_float2(re[t], buf[rslot(t)]);
+        }
+        Ns *= 16;
+    }
+    if constexpr (kRl > 1)
+    {
+#pragma unroll
+        for (int b = 0; b < 16 / kRl; ++b)
+        {
+            const int j = j0 + b * kQ;
+            float2 u[kRl];
+#pragma unroll
+            for (int r = 0; r < kRl; ++r)
+                u[r] = v[b * kRl + r];
+#pragma unroll
+            for (int r = 1; r < kRl; ++r)
+                u[r] = f2mul(u[r], tw(j * r));
+            dft<kRl, SG>(u);
+#pragma unroll
+            for (int r = 0; r < kRl; ++r)
+                emit(b * kRl + r, j + r * Ns, u[r]);
+        }
+    }
+}
+
+// R3: the first pass of the N x N planes of fft_col3_kernel (N = 3 L = A.gx, not centred, dense): the L-point
+// transforms of the sub-rows x = 3 x' + b (b = 0, 1, 2) of each input row (A.scratch), stored as segment b
+// (columns [b L, b L + L)) of that row in A.grid.
+template <int L, int SG, bool R3 = false>
+__global__ __launch_bounds__(L / 16, (fft_row_waves_per_simd<L, R3>())) void fft_row_kernel(FftArgs A)
+{
+    constexpr int kQ = L / 16;
+    __shared__ float buf[L + L / 16];
+    __shared__ float2 t_hi[L / 64], t_lo[64];
+    row_twiddle_tables<L, SG>(t_hi, t_lo);
     __syncthreads();
 
     const bool flip = A.centred != 0;
@@ -312,12 +410,7 @@ __global__ __launch_bounds__(L / 16, fft_row_waves_per_simd<L>()) void fft_row_k
             v = make_float2(-v.x, -v.y);
         return v;
     };
-    auto tw = [&](int qq) {
-        qq &= L - 1;
-        return f2mul(t_hi[qq >> 6], t_lo[qq & 63]);
-    };
-    // The L-point transform of the row at in_row (global row gr, for the input signs); output element k, the idx-th
-    // (0 .. 15, a compile-time index after unrolling) of this thread's, goes to emit(idx, k, value).
+    // The L-point transform of the row at in_row (global row gr, for the input signs).
     auto row_fft = [&](int gr, const float2 *in_row, auto &&emit) {
         // An opaque copy of the thread index per row: what derives from it (twiddles, slots) is formed where it is
         // used instead of being hoisted out of the row loop and held in registers for every row.
@@ -327,73 +420,7 @@ __global__ __launch_bounds__(L / 16, fft_row_waves_per_simd<L>()) void fft_row_k
 #pragma unroll
         for (int r = 0; r < 16; ++r)
             v[r] = load(gr, in_row, j0 + r * kQ);
-        int Ns = 1;
-#pragma unroll
-        for (int st = 0; st < kStages16; ++st)
-        {
-            if (st > 0)
-            {
-                const int m = (j0 & (Ns - 1)) * (L / (16 * Ns));
-#pragma unroll
-                for (int r = 1; r < 16; ++r)
-                    v[r] = f2mul(v[r], tw(m * r));
-            }
-            dft<16, SG>(v);
-            if (st == kStages16 - 1 && kRl == 1)
-            {
-#pragma unroll
-                for (int r = 0; r < 16; ++r)
-                    emit(r, j0 + r * Ns, v[r]);
-            }
-            else
-            {
-                // Output r of butterfly j0 goes to element d + r Ns: slot wb + r (Ns + Ns / 16), or 17 j0 + r when
-                // Ns = 1.  The next stage reads elements j0 + t kQ (slot rb + t kQs), or for the radix-kRl stage
-                // j0 + b kQ + r (L / kRl) (slot rb + b kQs + r kRs).
-                const int d = (j0 / Ns) * Ns * 16 + (j0 & (Ns - 1));
-                const int wb = d + (d >> 4), ws = Ns == 1 ? 1 : Ns + Ns / 16;
-                const int rb = j0 + (j0 >> 4);
-                const bool next16 = st + 1 < kStages16;
-                auto rslot = [&](int t) { return next16 ? rb + t * kQs : rb + (t / kRl) * kQs + (t % kRl) * kRs; };
-                float re[16];
-                lds_barrier(); // the previous exchange's reads are done
-#pragma unroll
-                for (int r = 0; r < 16; ++r)
-                    buf[wb + r * ws] = v[r].x;
-                lds_barrier();
-#pragma unroll
-                for (int t = 0; t < 16; ++t)
-                    re[t] = buf[rslot(t)];
-                lds_barrier();
-#pragma unroll
-                for (int r = 0; r < 16; ++r)
-                    buf[wb + r * ws] = v[r].y;
-                lds_barrier();
-#pragma unroll
-                for (int t = 0; t < 16; ++t)
-                    v[t] = make_float2(re[t], buf[rslot(t)]);
-            }
-            Ns *= 16;
-        }
-        if constexpr (kRl > 1)
-        {
-#pragma unroll
-            for (int b = 0; b < 16 / kRl; ++b)
-            {
-                const int j = j0 + b * kQ;
-                float2 u[kRl];
-#pragma unroll
-                for (int r = 0; r < kRl; ++r)
-                    u[r] = v[b * kRl + r];
-#pragma unroll
-                for (int r = 1; r < kRl; ++r)
-                    u[r] = f2mul(u[r], tw(j * r));
-                dft<kRl, SG>(u);
-#pragma unroll
-                for (int r = 0; r < kRl; ++r)
-                    emit(b * kRl + r, j + r * Ns, u[r]);
-            }
-        }
+        row_transform<L, SG>(v, j0, buf, t_hi, t_lo, emit);
     };
     for (int i = blockIdx.x; i < rows; i += gridDim.x)
     {
@@ -417,6 +444,125 @@ __global__ __launch_bounds__(L / 16, fft_row_waves_per_simd<L>()) void fft_row_k
             row_fft(y, A.scratch + static_cast<size_t>(y) * L, [&](int, int k, float2 val) {
                 out_row[elem(k)] = (flip && (k & 1)) ? make_float2(-val.x, -val.y) : val; // k odd <=> j0 odd
             });
+        }
+    }
+}
+
+// Output idx of row_transform's emit is element j0 + row_out_t<L>(idx) L / 16 of the row: the 16 elements a thread
+// loads, in the order its last stage produces them.
+template <int L> constexpr int row_out_t(int idx)
+{
+    constexpr int kRl = 1 << (ilog2(L) % 4);
+    return kRl == 1 ? idx : idx / kRl + (16 / kRl) * (idx % kRl);
+}
+
+// W-stacked row pass: the image side of the w-stacked master-grid transform, one image row (row y of one
+// correlation plane) per item with every w-layer in it, so that neither the layers' image-domain planes nor a
+// separate w-screen pass go through HBM.  Layer z (w_z = (z_min + z + 0.5) w_step) has the screen
+// exp(SG 2 pi i w_z n) at pixel (X, Y), n = image_n(X, Y): n in FP64 once per item and point, w_z n reduced to a
+// fraction of a turn in FP64, then v_cos / v_sin of that fraction (absolute error about 1e-7).  The transforms carry
+// the row pass's share of the centred signs.
+//   SG = -1, to the image (after the column passes): image[y][k] = sum_z screen_z(k) (-1)^k F_z[k], F_z the
+//     transform of layer z's row y, read at rows_in + z layer_stride + y pitch as columns [cx0, cx1) (stored from
+//     cx0 on; the other columns are zero);
+//   SG = +1, from the image (before the column passes): for each layer z, the transform of
+//     (-1)^(x + y) screen_z(x) image[y][x], with the output signs (-1)^k, goes to columns [cx0, cx1) of row
+//     rows_out + z layer_stride + y pitch.
+struct WsArgs
+{
+    const float2 *image_in;
+    float2 *image_out;
+    const float2 *rows_in;
+    float2 *rows_out;
+    long long layer_stride;
+    int pitch, nr_layers, z_min, cx0, cx1;
+    double image_size, w_step;
+};
+
+template <int L, int SG> __global__ __launch_bounds__(L / 16, (L >= 2048 ? 4 : 2)) void wstack_row_kernel(WsArgs W)
+{
+    constexpr int kQ = L / 16;
+    __shared__ float buf[L + L / 16];
+    __shared__ float2 t_hi[L / 64], t_lo[64];
+    row_twiddle_tables<L, SG>(t_hi, t_lo);
+    __syncthreads();
+    for (int y = blockIdx.x; y < L; y += gridDim.x)
+    {
+        int j0 = threadIdx.x; // opaque per row, as in the row pass
+        asm volatile("" : "+v"(j0));
+        // The thread's points j0 + t kQ (t < 16), in and out: n once per row; (-1)^k = (-1)^j0 (kQ is even).
+        double nn[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+            nn[t] = image_n(j0 + t * kQ, y, L, W.image_size);
+        const bool odd = (j0 & 1) != 0;
+        auto screen = [&](double wz, int t) {
+            // An opaque copy: the screen is formed where it is used, not ahead of the transform (which would hold 16
+            // screens in registers across it).
+            double n = nn[t];
+            asm volatile("" : "+v"(n));
+            double turns = wz * n;
+            turns -= __builtin_rint(turns);
+            const float f = static_cast<float>(turns);
+            return make_float2(__builtin_amdgcn_cosf(f), SG * __builtin_amdgcn_sinf(f));
+        };
+        if constexpr (SG < 0)
+        {
+            float2 acc[16];
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                acc[t] = make_float2(0.F, 0.F);
+            for (int z = 0; z < W.nr_layers; ++z)
+            {
+                const double wz = (W.z_min + z + 0.5) * W.w_step;
+                const float2 *in_row = W.rows_in + z * W.layer_stride + static_cast<long long>(y) * W.pitch;
+                int jz = j0; // opaque per layer: what derives from it is formed per layer, not held across layers
+                asm volatile("" : "+v"(jz));
+                float2 v[16];
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                {
+                    const int col = jz + r * kQ;
+                    v[r] = col >= W.cx0 && col < W.cx1 ? in_row[col - W.cx0] : make_float2(0.F, 0.F);
+                }
+                row_transform<L, SG>(v, jz, buf, t_hi, t_lo, [&](int idx, int, float2 val) {
+                    const int t = row_out_t<L>(idx);
+                    const float2 w = screen(wz, t);
+                    acc[t].x = __builtin_fmaf(val.x, w.x, __builtin_fmaf(-val.y, w.y, acc[t].x));
+                    acc[t].y = __builtin_fmaf(val.x, w.y, __builtin_fmaf(val.y, w.x, acc[t].y));
+                });
+            }
+            float2 *out = W.image_out + static_cast<size_t>(y) * L + j0;
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                out[t * kQ] = odd ? make_float2(-acc[t].x, -acc[t].y) : acc[t];
+        }
+        else
+        {
+            const float sg_in = ((j0 + y) & 1) ? -1.F : 1.F; // (-1)^(x + y), x = j0 + t kQ
+            const float2 *in = W.image_in + static_cast<size_t>(y) * L + j0;
+            float2 img[16];
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+            {
+                const float2 g = in[t * kQ];
+                img[t] = make_float2(sg_in * g.x, sg_in * g.y);
+            }
+            for (int z = 0; z < W.nr_layers; ++z)
+            {
+                const double wz = (W.z_min + z + 0.5) * W.w_step;
+                float2 v[16];
+#pragma unroll
+                for (int t = 0; t < 16; ++t)
+                    v[t] = f2mul(img[t], screen(wz, t));
+                float2 *out_row = W.rows_out + z * W.layer_stride + static_cast<long long>(y) * W.pitch;
+                int jz = j0;
+                asm volatile("" : "+v"(jz));
+                row_transform<L, SG>(v, jz, buf, t_hi, t_lo, [&](int, int k, float2 val) {
+                    if (k >= W.cx0 && k < W.cx1)
+                        out_row[k] = odd ? make_float2(-val.x, -val.y) : val;
+                });
+            }
         }
     }
 }
@@ -468,6 +614,7 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
         int pl, x, q;
         decode(tile, pl, x, q);
         const size_t base = pl * plane_elems;
+        const size_t sbase = pl * static_cast<size_t>(A.splane) - A.sx0;
         float2 v[16];
         // C1 applies the input sign (-1)^(x + y), y = N2 n1 + n2 with N2 even: (-1)^(x + n2) for the whole thread.
         const float sg_in = PASS == kPassCol1 && A.centred && !A.rows_first && ((x + q) & 1) ? -1.F : 1.F;
@@ -525,7 +672,7 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
                     // Only the rows the row pass reads ([ry0, ry0 + nry), all rows when dense).
                     const int row = q + A.N1 * k;
                     if (static_cast<unsigned>(row - A.ry0) < static_cast<unsigned>(A.nry))
-                        A.scratch[base + static_cast<size_t>(row) * G + x] =
+                        A.scratch[sbase + static_cast<size_t>(row) * A.spitch + x] =
                             (A.centred && (row & 1)) ? make_float2(-u[r].x, -u[r].y) : u[r];
                 }
             }
@@ -759,6 +906,36 @@ template <int SG> hipError_t launch_row_pass(const FftArgs &A, hipStream_t st)
     }
 }
 
+template <int L, int SG> hipError_t launch_wstack_rows(const WsArgs &W, hipStream_t st)
+{
+    int blocks = 0;
+    const hipError_t e =
+        resident_blocks(reinterpret_cast<const void *>(&wstack_row_kernel<L, SG>), L / 16, L, blocks);
+    if (e != hipSuccess)
+        return e;
+    wstack_row_kernel<L, SG><<<blocks, L / 16, 0, st>>>(W);
+    return hipGetLastError();
+}
+
+template <int SG> hipError_t launch_wstack_row_pass(const WsArgs &W, int G, hipStream_t st)
+{
+    switch (G)
+    {
+    case 1024:
+        return launch_wstack_rows<1024, SG>(W, st);
+    case 2048:
+        return launch_wstack_rows<2048, SG>(W, st);
+    case 4096:
+        return launch_wstack_rows<4096, SG>(W, st);
+    case 8192:
+        return launch_wstack_rows<8192, SG>(W, st);
+    case 16384:
+        return launch_wstack_rows<16384, SG>(W, st);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
 bool stockham_size(int G)
 {
     return G >= 1024 && G <= 16384 && (G & (G - 1)) == 0;
@@ -864,29 +1041,13 @@ struct FftBox
 {
     int mode = kFullTransform;
     int x0 = 0, x1 = 0, y0 = 0, y1 = 0;
+    bool rows_done = false; // kRowsOut: the planes already hold their row transforms (the w-stacked row pass)
 };
 
-// no_alloc: the scratch is the file-scope array (the caller has checked that a group fits it) instead of a plane
-// allocated on first use.
-template <int SG>
-hipError_t stockham_fft2d(const float2 *src, float2 *dst, int gx, int gy, long long planes, bool centred,
-                          hipStream_t st, const FftBox &box = FftBox(), bool no_alloc = false)
+// Dense three-pass arguments for planes of gy rows of gx points (pointers and plane count left to the caller).
+FftArgs fft_args(int gx, int gy, bool centred)
 {
-    FftArgs A;
-    const long long plane = static_cast<long long>(gx) * gy;
-    const long long plane_bytes = plane * static_cast<long long>(sizeof(float2));
-    const long long group = std::max(1LL, std::min(planes, kScratchBytes / plane_bytes));
-    hipError_t e = hipSuccess;
-    if (no_alloc)
-    {
-        if (group * plane * static_cast<long long>(sizeof(float2)) > kStaticScratchBytes)
-            return hipErrorInvalidValue;
-        e = static_scratch(A.scratch);
-    }
-    else
-        e = fft_scratch(group * plane, A.scratch);
-    if (e != hipSuccess)
-        return e;
+    FftArgs A{};
     A.gx = gx;
     A.gy = gy;
     A.log_gy = ilog2(gy);
@@ -902,6 +1063,34 @@ hipError_t stockham_fft2d(const float2 *src, float2 *dst, int gx, int gy, long l
     A.cx1 = gx;
     A.ry0 = 0;
     A.nry = gy;
+    A.rows_first = 0;
+    A.splane = static_cast<long long>(gx) * gy;
+    A.spitch = gx;
+    A.sx0 = 0;
+    return A;
+}
+
+// no_alloc: the scratch is the file-scope array (the caller has checked that a group fits it) instead of a plane
+// allocated on first use.
+template <int SG>
+hipError_t stockham_fft2d(const float2 *src, float2 *dst, int gx, int gy, long long planes, bool centred,
+                          hipStream_t st, const FftBox &box = FftBox(), bool no_alloc = false)
+{
+    FftArgs A = fft_args(gx, gy, centred);
+    const long long plane = static_cast<long long>(gx) * gy;
+    const long long plane_bytes = plane * static_cast<long long>(sizeof(float2));
+    const long long group = std::max(1LL, std::min(planes, kScratchBytes / plane_bytes));
+    hipError_t e = hipSuccess;
+    if (no_alloc)
+    {
+        if (group * plane * static_cast<long long>(sizeof(float2)) > kStaticScratchBytes)
+            return hipErrorInvalidValue;
+        e = static_scratch(A.scratch);
+    }
+    else
+        e = fft_scratch(group * plane, A.scratch);
+    if (e != hipSuccess)
+        return e;
     if (box.mode == kSparseInput)
     {
         const int t0 = box.x0 / kColTile, t1 = (box.x1 + kColTile - 1) / kColTile;
@@ -924,11 +1113,14 @@ hipError_t stockham_fft2d(const float2 *src, float2 *dst, int gx, int gy, long l
             A.nplanes = static_cast<int>(std::min(group, planes - p));
             float2 *g = dst + static_cast<size_t>(p) * plane;
             // 1. Rows, in place and dense, with the input signs.
-            FftArgs R = A;
-            R.scratch = g;
-            R.grid = g;
-            R.rows_first = 1;
-            e = launch_row_pass<SG>(R, st);
+            if (!box.rows_done)
+            {
+                FftArgs R = A;
+                R.scratch = g;
+                R.grid = g;
+                R.rows_first = 1;
+                e = launch_row_pass<SG>(R, st);
+            }
             // 2. Columns of the box's column tiles; column pass 2 stores the box's rows into the scratch.
             FftArgs C = A;
             C.src = g;
@@ -1193,6 +1385,9 @@ hipError_t radix3_fft2d(const float2 *src, float2 *dst, int N, long long planes,
     A.ry0 = 0;
     A.nry = M;
     A.rows_first = 0;
+    A.splane = plane; // (fft_col3_kernel stores with its own layout)
+    A.spitch = N;
+    A.sx0 = 0;
     float2 *sc = A.scratch;
     for (long long pl = 0; pl < planes && e == hipSuccess; ++pl)
     {
@@ -1299,5 +1494,125 @@ hipError_t launch_wstack(const void *src, void *dst, int G, int z_min, int nr_la
         wstack_from_image_kernel<<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(
             static_cast<const float2 *>(src), static_cast<float2 *>(dst), G, z_min, nr_layers, image_size, w_step);
     return hipGetLastError();
+}
+
+namespace
+{
+// Scratch of the w-stacked transform to the image (column pass 2's output for one correlation's layers, the box's
+// columns only), per device: grown when a call needs more (the device is synchronised and the old one freed first),
+// kept for the life of the process.
+std::mutex g_ws_mutex;
+std::map<int, std::pair<float2 *, long long>> g_ws_scratch;
+
+hipError_t wstack_scratch(long long elements, float2 *&out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    auto &s = g_ws_scratch[dev];
+    if (s.second < elements)
+    {
+        if (s.first)
+        {
+            e = hipDeviceSynchronize();
+            if (e == hipSuccess)
+                e = hipFree(s.first);
+            if (e != hipSuccess)
+                return e;
+            s = {nullptr, 0};
+        }
+        float2 *p = nullptr;
+        e = hipMalloc(&p, static_cast<size_t>(elements) * sizeof(float2));
+        if (e != hipSuccess)
+            return e;
+        s = {p, elements};
+    }
+    out = s.first;
+    return hipSuccess;
+}
+} // namespace
+
+hipError_t launch_grid_wstack(void *grid, void *image, int G, int z_min, int nr_layers, float image_size,
+                              float w_step, bool to_image, const int *box, hipStream_t st)
+{
+    const long long planes = static_cast<long long>(nr_layers) * kCorr;
+    const long long plane = static_cast<long long>(G) * G;
+    if (planes > 0x7fffffffLL || plane > 0x7fffffffLL)
+        return hipErrorInvalidConfiguration;
+    hipError_t e = hipSuccess;
+    if (!stockham_size(G))
+    {
+        if (to_image)
+        {
+            e = launch_grid_fft(grid, G, nr_layers, false, st, box);
+            return e == hipSuccess ? launch_wstack(grid, image, G, z_min, nr_layers, image_size, w_step, true, st) : e;
+        }
+        e = launch_wstack(image, grid, G, z_min, nr_layers, image_size, w_step, false, st);
+        return e == hipSuccess ? launch_grid_fft(grid, G, nr_layers, true, st, box) : e;
+    }
+    // The box's whole column tiles, as the boxed transform takes them.
+    const int t0 = box[0] / kColTile, t1 = std::max(t0, (box[1] + kColTile - 1) / kColTile);
+    WsArgs W{};
+    W.nr_layers = nr_layers;
+    W.z_min = z_min;
+    W.cx0 = t0 * kColTile;
+    W.cx1 = t1 * kColTile;
+    W.image_size = image_size;
+    W.w_step = w_step;
+    float2 *g = static_cast<float2 *>(grid), *img = static_cast<float2 *>(image);
+    if (to_image)
+    {
+        // Per correlation: the column passes of its layers (the grid is zero outside the box's columns) into the
+        // scratch, then the row pass of all its layers into its image plane.
+        const int nc = W.cx1 - W.cx0;
+        float2 *sc = nullptr;
+        if (nc > 0)
+            e = wstack_scratch(static_cast<long long>(nr_layers) * G * nc, sc);
+        W.rows_in = sc;
+        W.layer_stride = static_cast<long long>(G) * nc;
+        W.pitch = nc;
+        for (int p = 0; p < kCorr && e == hipSuccess; ++p)
+        {
+            for (int z = 0; z < nr_layers && nc > 0 && e == hipSuccess; ++z)
+            {
+                FftArgs C = fft_args(G, G, true);
+                C.src = C.grid = g + (static_cast<long long>(z) * kCorr + p) * plane;
+                C.scratch = sc + static_cast<long long>(z) * G * nc;
+                C.splane = static_cast<long long>(G) * nc;
+                C.spitch = nc;
+                C.sx0 = W.cx0;
+                C.nplanes = 1;
+                C.ct0 = t0;
+                C.nct = t1 - t0;
+                e = launch_col_pass<kPassCol1, -1>(C, C.N1, C.nct * C.N2, st);
+                if (e == hipSuccess)
+                    e = launch_col_pass<kPassCol2, -1>(C, C.N2, C.nct * C.N1, st);
+            }
+            W.image_out = img + p * plane;
+            if (e == hipSuccess)
+                e = launch_wstack_row_pass<-1>(W, G, st);
+        }
+        return e;
+    }
+    // From the image: the row pass of every layer of each correlation (the box's columns of each layer row), then
+    // the boxed transform's column passes over all planes.
+    W.layer_stride = kCorr * plane;
+    W.pitch = G;
+    for (int p = 0; p < kCorr && e == hipSuccess; ++p)
+    {
+        W.image_in = img + p * plane;
+        W.rows_out = g + p * plane;
+        e = launch_wstack_row_pass<1>(W, G, st);
+    }
+    FftBox b;
+    b.mode = kRowsOut;
+    b.x0 = box[0];
+    b.x1 = box[1];
+    b.y0 = box[2];
+    b.y1 = box[3];
+    b.rows_done = true;
+    return e == hipSuccess ? stockham_fft2d<1>(g, g, G, G, planes, true, st, b) : e;
 }
 } // namespace idgx

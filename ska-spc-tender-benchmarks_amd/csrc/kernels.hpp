@@ -216,4 +216,13 @@ hipError_t launch_fft2d(const void *in, void *out, int nx, int ny, long long bat
                         hipStream_t st);
 hipError_t launch_wstack(const void *src, void *dst, int G, int z_min, int nr_layers, float image_size,
                          float w_step, bool to_image, hipStream_t st);
+// The boxed master-grid transform and the w-screens of launch_wstack in one chain (box as for launch_grid_fft):
+//   to_image:  the image from BACKWARD-transformed layers (launch_grid_fft, then launch_wstack to the image); the
+//              grid is left undefined;
+//   !to_image: the layers from the image (launch_wstack from the image, then launch_grid_fft FORWARD), in the box's
+//              cells.
+// Three-pass sizes fold the screens and the layer sum into the row pass, so the layers' image-domain planes are
+// never stored; other sizes run the two calls one after the other.
+hipError_t launch_grid_wstack(void *grid, void *image, int G, int z_min, int nr_layers, float image_size,
+                              float w_step, bool to_image, const int *box, hipStream_t st);
 } // namespace idgx

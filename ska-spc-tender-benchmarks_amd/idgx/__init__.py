@@ -170,6 +170,10 @@ def lib():
                                                                                 ctypes.POINTER(ctypes.c_int64), vp]
         L.idgx_wstack_to_image.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.idgx_wstack_from_image.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.idgx_grid_to_image.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int32), vp, vp]
+        L.idgx_image_to_grid.argtypes = [ctypes.POINTER(Plan), vp, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int32), vp, vp]
         L.idgx_split_subgrids.argtypes = [ctypes.POINTER(Plan)] + [vp] * 4
         L.idgx_batch_create.argtypes = [ctypes.POINTER(BatchClass), ctypes.c_int, ctypes.POINTER(vp)]
         L.idgx_batch_destroy.argtypes = [vp]
@@ -424,6 +428,22 @@ def wstack_to_image(plan, d_layers, z_min, nr_layers, d_image, stream=None):
 def wstack_from_image(plan, d_image, z_min, nr_layers, d_layers, stream=None):
     _check(lib().idgx_wstack_from_image(ctypes.byref(plan), _dptr(d_image), int(z_min), int(nr_layers),
                                         _dptr(d_layers), _stream(stream)), "idgx_wstack_from_image")
+
+
+def grid_to_image(plan, d_grid, z_min, nr_layers, box, d_image, stream=None):
+    """idgx_grid_to_image: the boxed transform of the layered grid to the image and the w-screens with the sum over
+    layers in one call (d_grid is left undefined)."""
+    b = (ctypes.c_int32 * 4)(*[int(v) for v in box])
+    _check(lib().idgx_grid_to_image(ctypes.byref(plan), _dptr(d_grid), int(z_min), int(nr_layers), b,
+                                    _dptr(d_image), _stream(stream)), "idgx_grid_to_image")
+
+
+def image_to_grid(plan, d_image, z_min, nr_layers, box, d_grid, stream=None):
+    """idgx_image_to_grid: the w-screens from the image and the boxed transform to the layered grid in one call
+    (the box's cells of every layer; the rest of the grid undefined)."""
+    b = (ctypes.c_int32 * 4)(*[int(v) for v in box])
+    _check(lib().idgx_image_to_grid(ctypes.byref(plan), _dptr(d_image), int(z_min), int(nr_layers), b,
+                                    _dptr(d_grid), _stream(stream)), "idgx_image_to_grid")
 
 
 def plan_segments_device(uvw, frequencies, station1, station2, seg_bl, seg_t0, seg_t1, *, subgrid_size, grid_size,

@@ -6,7 +6,8 @@ import numpy as np
 
 from . import METADATA_DTYPE, CHANNELS_DTYPE, ATERM_ADJOINT, PLAIN_FP32, make_plan, check_plan, grid, degrid, \
     IdgxError, OK, FFT_FORWARD, FFT_BACKWARD, subgrid_fft, adder_plan_build, add_subgrids, split_subgrids, \
-    adder_plan_build_layered, split_subgrids_layered, grid_fft, wstack_to_image, wstack_from_image
+    adder_plan_build_layered, split_subgrids_layered, grid_fft, wstack_to_image, wstack_from_image, grid_to_image, \
+    image_to_grid
 
 
 def _torch():
@@ -141,6 +142,17 @@ class DeviceWorkload:
     def from_image(self, plan, d_image, d_layers, stream=None):
         z_min, nl = self.layers()
         wstack_from_image(plan, d_image, z_min, nl, d_layers, stream)
+
+    def grid_to_image(self, plan, d_layers, d_image, box, stream=None):
+        """The boxed transform of the layers and the w-screens in one call (idgx_grid_to_image); d_layers is left
+        undefined."""
+        z_min, nl = self.layers()
+        grid_to_image(plan, d_layers, z_min, nl, box, d_image, stream)
+
+    def image_to_grid(self, plan, d_image, d_layers, box, stream=None):
+        """The w-screens and the boxed transform to the layers in one call (idgx_image_to_grid): the box's cells."""
+        z_min, nl = self.layers()
+        image_to_grid(plan, d_image, z_min, nl, box, d_layers, stream)
 
 
 def selection_lists(w, flags, chunk_entries=1 << 26):

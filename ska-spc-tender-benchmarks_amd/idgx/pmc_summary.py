@@ -21,7 +21,8 @@ import sys
 KERNELS = ["gridder_plain_queue_tiles", "gridder_plain_queue", "degridder_plain_queue", "gridder_plain",
            "degridder_plain", "gridder_mfma_queue_tiles", "gridder_mfma_queue", "degridder_mfma_queue",
            "degrid_prepare_kernel", "gridder_mfma", "degridder_mfma", "subgrid_fft_kernel", "adder_kernel",
-           "splitter_kernel", "fft_col_kernel", "fft_row_kernel", "copy_box_kernel", "clear_band_kernel",
+           "splitter_kernel", "fft_col_kernel", "fft_row_kernel", "wstack_row_kernel", "copy_box_kernel",
+           "clear_band_kernel",
            "checkerboard_kernel", "wstack_to_image_kernel", "wstack_from_image_kernel", "reset_counters_kernel"]
 
 

@@ -132,8 +132,8 @@ def _vdot64(a, b):
     return acc
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp16x3"])
-def test_full_cycle_adjoint_on_the_16k_grid(cycle, precision):
+@pytest.mark.parametrize("precision,fused", [("fp32", False), ("fp16x3", False), ("fp16x3", True)])
+def test_full_cycle_adjoint_on_the_16k_grid(cycle, precision, fused):
     """The whole full-cycle plan (102,901 subgrids, |w| to ~5300 wavelengths) through the w-stacked imaging cycle on
     six 16384^2 w-layers in HBM, and back: with M = to_image . grid_fft(BACKWARD) . layered add . subgrid FFT .
     gridder and M^H = degridder . inverse subgrid FFT . layered split . grid_fft(FORWARD) . from_image,
@@ -141,39 +141,48 @@ def test_full_cycle_adjoint_on_the_16k_grid(cycle, precision):
     mixed form (FP64 phase reduction); 2e-4 for plain FP32, whose phases round as the reference's do (a large
     phase offset minus k (u l + v m + w n), each rounded to FP32) and so differ from FP64 by up to 1.4e-4 per
     subgrid on this plan's 40 km baselines (test_full_cycle_jones_vs_fp64), in the gridder and degridder
-    differently: the identity cannot hold closer than that."""
+    differently: the identity cannot hold closer than that.  fused: both image-side steps in one call each on the
+    subgrids' box (idgx_grid_to_image, idgx_image_to_grid: screens and layer sum inside the row pass)."""
     import torch
 
-    from idgx import FFT_BACKWARD, FFT_FORWARD
+    from idgx import FFT_BACKWARD, FFT_FORWARD, grid_box
 
     w = cycle
     dw = DeviceWorkload(w)
     plan = dw.plan(precision=precision)
     z_min, nl = dw.layers()
     G = int(w["grid_size"])
-    print(f"full_cycle adjoint {precision}: {w['nr_subgrids']} subgrids, {nl} w-layers from z = {z_min}, "
-          f"G = {G}", flush=True)
+    box = grid_box(plan, w["metadata"])
+    print(f"full_cycle adjoint {precision}{' fused' if fused else ''}: {w['nr_subgrids']} subgrids, {nl} w-layers "
+          f"from z = {z_min}, G = {G}, box {box}", flush=True)
     dw.grid(plan)
     dw.fft(plan, FFT_FORWARD)
     d_layers = dw.layer_buffer(nl)
     dw.add_layered(plan, d_layers)
-    dw.grid_fft(plan, d_layers, nl, FFT_BACKWARD)
     d_image = dw.grid_buffer()
-    dw.to_image(plan, d_layers, d_image)
+    if fused:
+        dw.grid_to_image(plan, d_layers, d_image, box)
+    else:
+        dw.grid_fft(plan, d_layers, nl, FFT_BACKWARD)
+        dw.to_image(plan, d_layers, d_image)
     gen = torch.Generator(device=d_image.device)
     gen.manual_seed(41)
     d_rand = torch.randn(4 * G * G * 2, generator=gen, device=d_image.device, dtype=torch.float32)
     rhs = _vdot64(d_rand, d_image.view(torch.float32))
     del d_image
-    dw.from_image(plan, d_rand, d_layers)
-    dw.grid_fft(plan, d_layers, nl, FFT_FORWARD)
+    if fused:
+        dw.image_to_grid(plan, d_rand, d_layers, box)
+    else:
+        dw.from_image(plan, d_rand, d_layers)
+        dw.grid_fft(plan, d_layers, nl, FFT_FORWARD)
     dw.split_layered(plan, d_layers)
     del d_layers
     dw.fft(plan, FFT_BACKWARD)
     dw.degrid(plan)
     lhs = _vdot64(dw.d_vis_out.view(torch.float32), dw.d_visibilities.view(torch.float32))
     rel = abs(lhs - rhs) / abs(rhs)
-    print(f"full_cycle adjoint {precision}: <M^H I, V> = {lhs:.6e}, <I, M V> = {rhs:.6e}, rel diff {rel:.2e}")
+    print(f"full_cycle adjoint {precision}{' fused' if fused else ''}: <M^H I, V> = {lhs:.6e}, <I, M V> = "
+          f"{rhs:.6e}, rel diff {rel:.2e}")
     assert rel <= (2e-4 if precision == "fp32" else 2e-5)
 
 

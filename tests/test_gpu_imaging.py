@@ -272,39 +272,128 @@ def test_wstack_screens_match_their_formula_and_are_adjoint():
     assert rel_l2(got, ref) < 1e-6 and abs(lhs - rhs) <= 1e-6 * abs(rhs)
 
 
+@pytest.mark.parametrize("G", [1024, 2048, 1028])
+def test_grid_to_image_and_back_match_the_separate_calls(G):
+    """idgx_grid_to_image / idgx_image_to_grid, whose row pass applies the w-screens and sums the layers, against
+    idgx_grid_fft_box followed by idgx_wstack_to_image, and idgx_wstack_from_image followed by idgx_grid_fft_box
+    FORWARD (both pinned to FP64 above).  w_z n reaches about 16 turns; the box is off the 64-column tiles.  One
+    correlation is also checked against the FP64 formula directly (numpy FFTs).  Then an empty box (a zero image),
+    the whole grid, and a smaller box after a larger one (the kept scratch is reused).  G = 1028 runs the two calls
+    (no three-pass kernels there): bit for bit the separate chain."""
+    from idgx import grid_to_image, image_to_grid, wstack_from_image, wstack_to_image
+
+    rng = np.random.default_rng(11)
+    z_min, nl, w_step, im = -2, 5 if G < 2048 else 3, 2600.0, 0.1
+    md = np.zeros(1, METADATA_DTYPE)
+    md["nr_timesteps"] = 1
+    md["station2"] = 1
+    w = _grid_workload(G, md)
+    w.update(w_step=w_step, image_size=np.float32(im))
+    dw = DeviceWorkload(w)
+    plan = dw.plan()
+    shape = (nl, 4, G, G)
+
+    def layers_in(box):
+        x = np.zeros(shape, np.complex64)
+        h, wd = box[3] - box[2], box[1] - box[0]
+        x[..., box[2]:box[3], box[0]:box[1]] = (rng.standard_normal((nl, 4, h, wd)) +
+                                                1j * rng.standard_normal((nl, 4, h, wd)))
+        return x
+
+    def to_image_both(x, box):
+        d_sep, d_img_sep = upload(x), dw.layer_buffer(1)
+        grid_fft_box(plan, d_sep, nl, FFT_BACKWARD, box)
+        wstack_to_image(plan, d_sep, z_min, nl, d_img_sep)
+        d_fused, d_img = upload(x), dw.layer_buffer(1)
+        d_img.fill_(3)  # every image cell must be written
+        grid_to_image(plan, d_fused, z_min, nl, box, d_img)
+        return download(d_img, np.complex64, (4, G, G)), download(d_img_sep, np.complex64, (4, G, G))
+
+    big = (300, 701, 100, 523) if G >= 2048 else (150, 351, 50, 261)
+    x = layers_in(big)
+    img, img_sep = to_image_both(x, big)
+    e_to = rel_l2(img, img_sep)
+    # From the image: the box's cells of every layer.
+    X = (rng.standard_normal((4, G, G)) + 1j * rng.standard_normal((4, G, G))).astype(np.complex64)
+    d_sep = dw.layer_buffer(nl)
+    wstack_from_image(plan, upload(X), z_min, nl, d_sep)
+    grid_fft_box(plan, d_sep, nl, FFT_FORWARD, big)
+    d_fused = dw.layer_buffer(nl)
+    image_to_grid(plan, upload(X), z_min, nl, big, d_fused)
+    rows, cols = slice(big[2], big[3]), slice(big[0], big[1])
+    got = download(d_fused, np.complex64, shape)[..., rows, cols]
+    sep = download(d_sep, np.complex64, shape)[..., rows, cols]
+    e_from = rel_l2(got, sep)
+    print(f"G={G}: grid_to_image vs the separate calls {e_to:.2e}; image_to_grid {e_from:.2e}")
+    if G == 1028:
+        assert e_to == 0.0 and e_from == 0.0
+        return
+    assert e_to < 1e-6 and e_from < 1e-6
+    if G == 1024:
+        lm = (np.arange(G) - G / 2) * float(np.float32(im)) / G
+        r2 = lm[None, :] ** 2 + lm[:, None] ** 2
+        n = r2 / (1 + np.sqrt(1 - r2))
+        wz = (z_min + np.arange(nl) + 0.5) * w_step
+        assert (wz[-1] * n).max() > 10  # phases of many turns
+        ref = sum(np.exp(-2j * np.pi * wz[z] * n) * _centred_np(x[z, 1].astype(np.complex128), -1)
+                  for z in range(nl))
+        e64 = rel_l2(img[1], ref)
+        print(f"G={G}: grid_to_image vs FP64 {e64:.2e}")
+        assert e64 < 1e-6
+    # Empty box: zero layers give a zero image (every cell written).
+    img0, _ = to_image_both(np.zeros(shape, np.complex64), (0, 0, 0, 0))
+    assert not img0.any()
+    # The whole grid, then a box smaller than the first (the scratch grown for the whole grid is reused).
+    for box in ((0, G, 0, G), (64, 129, 7, 90)):
+        x = layers_in(box)
+        img, img_sep = to_image_both(x, box)
+        e = rel_l2(img, img_sep)
+        print(f"G={G} box {box}: grid_to_image vs the separate calls {e:.2e}")
+        assert e < 1e-6
+
+
 @pytest.fixture(scope="module")
 def wstack():
     return observation.make_observation("wstack")
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp16x3"])
-def test_imaging_chain_with_wstacking_is_adjoint(wstack, precision):
-    """<degrid(fft_b(split(fft_f(from_image(X))))), V> = <X, to_image(fft_b(add(fft_f(grid(V)))))>."""
+@pytest.mark.parametrize("precision,fused", [("fp32", False), ("fp16x3", False), ("fp16x3", True)])
+def test_imaging_chain_with_wstacking_is_adjoint(wstack, precision, fused):
+    """<degrid(fft_b(split(fft_f(from_image(X))))), V> = <X, to_image(fft_b(add(fft_f(grid(V)))))>; fused: the
+    transforms and screens of each side in one call on the subgrids' box (idgx_grid_to_image / idgx_image_to_grid)."""
     w = wstack
     G = int(w["grid_size"])
     dw = DeviceWorkload(w)
     plan = dw.plan(precision=precision)
     z_min, nl = dw.layers()
     assert nl >= 5
+    box = grid_box(plan, w["metadata"])
     dw.grid(plan)
     dw.fft(plan, FFT_FORWARD)
     d_layers = dw.layer_buffer(nl)
     dw.add_layered(plan, d_layers)
-    dw.grid_fft(plan, d_layers, nl, FFT_BACKWARD)
     d_img = dw.layer_buffer(1)
-    dw.to_image(plan, d_layers, d_img)
+    if fused:
+        dw.grid_to_image(plan, d_layers, d_img, box)
+    else:
+        dw.grid_fft(plan, d_layers, nl, FFT_BACKWARD)
+        dw.to_image(plan, d_layers, d_img)
     img = download(d_img, np.complex64, (4, G, G)).astype(np.complex128)
     rng = np.random.default_rng(17)
     X = (rng.standard_normal((4, G, G)) + 1j * rng.standard_normal((4, G, G))).astype(np.complex64)
-    dw.from_image(plan, upload(X), d_layers)
-    dw.grid_fft(plan, d_layers, nl, FFT_FORWARD)
+    if fused:
+        dw.image_to_grid(plan, upload(X), d_layers, box)
+    else:
+        dw.from_image(plan, upload(X), d_layers)
+        dw.grid_fft(plan, d_layers, nl, FFT_FORWARD)
     dw.split_layered(plan, d_layers)
     dw.fft(plan, FFT_BACKWARD)
     dw.degrid(plan)
     dX = dw.visibilities_out().astype(np.complex128)
     lhs = np.vdot(dX, w["visibilities"].astype(np.complex128))
     rhs = np.vdot(X.astype(np.complex128), img)
-    print(f"w-stacked chain adjoint {precision} ({nl} layers): rel diff {abs(lhs - rhs) / abs(rhs):.2e}")
+    print(f"w-stacked chain adjoint {precision}{' fused' if fused else ''} ({nl} layers, G = {G}): rel diff "
+          f"{abs(lhs - rhs) / abs(rhs):.2e}")
     assert abs(lhs - rhs) <= 2e-5 * abs(rhs)
 
 
@@ -322,25 +411,28 @@ def _dirty_image(w, vis, pixels):
     return np.array(out)
 
 
-def _image(dw, plan, w):
+def _image(dw, plan, w, fused=False):
     G = int(w["grid_size"])
     z_min, nl = dw.layers()
     dw.grid(plan)
     dw.fft(plan, FFT_FORWARD)
     d_layers = dw.layer_buffer(nl)
     dw.add_layered(plan, d_layers)
-    dw.grid_fft(plan, d_layers, nl, FFT_BACKWARD)
     d_img = dw.layer_buffer(1)
-    dw.to_image(plan, d_layers, d_img)
+    if fused:
+        dw.grid_to_image(plan, d_layers, d_img, grid_box(plan, w["metadata"]))
+    else:
+        dw.grid_fft(plan, d_layers, nl, FFT_BACKWARD)
+        dw.to_image(plan, d_layers, d_img)
     return download(d_img, np.complex64, (4, G, G))
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp16x3"])
-def test_point_source_through_the_wstacked_chain_matches_the_direct_dft(wstack, precision):
+@pytest.mark.parametrize("precision,fused", [("fp32", False), ("fp16x3", False), ("fp16x3", True)])
+def test_point_source_through_the_wstacked_chain_matches_the_direct_dft(wstack, precision, fused):
     """A point source at (l0, m0) with the full w-term, imaged with w-stacking, divided by the chain's response to
     a visibility at the uv origin (the taper correction), equals the direct FP64 DFT dirty image to 5e-5 of its
     peak over the central half of the field (IDG is exact up to the taper's aliasing); the peak sits at the source
-    pixel."""
+    pixel.  fused: idgx_grid_to_image on the subgrids' box."""
     w = dict(wstack)
     G, im = int(w["grid_size"]), float(w["image_size"])
     X0, Y0 = G // 2 + 301, G // 2 - 197
@@ -352,7 +444,7 @@ def test_point_source_through_the_wstacked_chain_matches_the_direct_dft(wstack, 
     pol = np.array([1.0, 0.0, 0.0, 1.0])
     w["visibilities"] = (np.exp(1j * ph)[..., None] * pol).astype(np.complex64)
     dw = DeviceWorkload(w)
-    img = _image(dw, dw.plan(precision=precision), w)[0]
+    img = _image(dw, dw.plan(precision=precision), w, fused)[0]
     # Response to one visibility at the uv origin, in a subgrid centred on the grid (layer 0).
     md = np.zeros(1, METADATA_DTYPE)
     md["nr_timesteps"] = 1
@@ -364,7 +456,7 @@ def test_point_source_through_the_wstacked_chain_matches_the_direct_dft(wstack, 
     d = dict(w, metadata=md, channels=ch, uvw=np.zeros((1, 3), np.float32), nr_subgrids=1, max_nr_timesteps=1,
              visibilities=(np.ones((1, w["wavenumbers"].shape[0], 1)) * pol).astype(np.complex64))
     dd = DeviceWorkload(d)
-    resp = _image(dd, dd.plan(precision=precision), d)[0]
+    resp = _image(dd, dd.plan(precision=precision), d, fused)[0]
     pix = [(X, Y) for X in range(G // 4, 3 * G // 4, 61) for Y in range(G // 4, 3 * G // 4, 59)] + \
           [(X0 + dx, Y0 + dy) for dx in (-1, 0, 1) for dy in (-1, 0, 1)]
     ref = _dirty_image(w, w["visibilities"].astype(np.complex128), pix)[:, 0]
@@ -374,6 +466,7 @@ def test_point_source_through_the_wstacked_chain_matches_the_direct_dft(wstack, 
     c0, c1 = G // 4, 3 * G // 4
     Yp, Xp = np.unravel_index(np.argmax(np.abs(img / resp)[c0:c1, c0:c1]), (c1 - c0, c1 - c0))
     Yp, Xp = Yp + c0, Xp + c0
-    print(f"{precision}: point source through the w-stacked chain vs direct DFT: max error {err:.2e} of the peak; "
+    print(f"{precision}{' fused' if fused else ''}: point source through the w-stacked chain vs direct DFT: max "
+          f"error {err:.2e} of the peak; "
           f"peak at ({Xp}, {Yp}), source at ({X0}, {Y0})")
     assert err < 5e-5 and (Xp, Yp) == (X0, Y0)

@@ -142,6 +142,29 @@ def test_boxed_grid_calls_reject_bad_boxes_without_touching_the_device():
     assert L.idgx_grid_fft_box(ctypes.byref(p), fake, 1, idgx.FFT_FORWARD, None, None) == idgx.ERR_INVALID_ARGUMENT
 
 
+def test_grid_to_image_calls_reject_bad_arguments_without_touching_the_device():
+    """idgx_grid_to_image / idgx_image_to_grid check the box, the layer count and the buffers on the host: aliased,
+    missing or bad ones fail before any device work (this runs without a GPU)."""
+    w = observation.make_observation("small", visibilities=False)
+    p = _plan(w)
+    G = w["grid_size"]
+    L = idgx.lib()
+    grid, image = ctypes.c_void_p(1), ctypes.c_void_p(2)  # never dereferenced
+    good = (ctypes.c_int32 * 4)(0, 10, 0, 10)
+    for fn in (L.idgx_grid_to_image, L.idgx_image_to_grid):
+        for box in [(-1, 10, 0, 10), (10, 5, 0, 10), (0, G + 1, 0, 10), (0, 10, 20, 10), (0, 10, 0, G + 1)]:
+            b = (ctypes.c_int32 * 4)(*box)
+            assert fn(ctypes.byref(p), grid, -1, 3, b, image, None) == idgx.ERR_INVALID_ARGUMENT
+        assert fn(ctypes.byref(p), grid, -1, 0, good, image, None) == idgx.ERR_INVALID_ARGUMENT
+        assert fn(ctypes.byref(p), grid, -1, 3, None, image, None) == idgx.ERR_INVALID_ARGUMENT
+    assert L.idgx_grid_to_image(ctypes.byref(p), None, 0, 1, good, image, None) == idgx.ERR_INVALID_ARGUMENT
+    assert L.idgx_grid_to_image(ctypes.byref(p), grid, 0, 1, good, None, None) == idgx.ERR_INVALID_ARGUMENT
+    assert L.idgx_grid_to_image(ctypes.byref(p), grid, 0, 1, good, grid, None) == idgx.ERR_INVALID_ARGUMENT
+    assert L.idgx_image_to_grid(ctypes.byref(p), None, 0, 1, good, grid, None) == idgx.ERR_INVALID_ARGUMENT
+    assert L.idgx_image_to_grid(ctypes.byref(p), image, 0, 1, good, None, None) == idgx.ERR_INVALID_ARGUMENT
+    assert L.idgx_image_to_grid(ctypes.byref(p), image, 0, 1, good, image, None) == idgx.ERR_INVALID_ARGUMENT
+
+
 def test_planner_covers_observation_and_keeps_subgrids_on_grid():
     w = observation.make_observation("ska_low", visibilities=False)
     md, ch = w["metadata"], w["channels"]
