@@ -248,7 +248,8 @@ int idgx_grid_box(const idgx_plan *plan, const idgx_metadata *h_metadata, int32_
 /* idgx_grid_fft restricted to a box of cells {x0, x1, y0, y1} of every plane (0 <= x0 <= x1 <= G, same for y), for
  * grids whose uv cells outside the box carry nothing (a plan's subgrids cover the box of idgx_grid_box):
  *   BACKWARD (uv -> image): the grid must be zero outside the box.  The column transforms run only over the
- *     64-column tiles that meet [x0, x1), and the row transforms read only those columns.  The image is complete
+ *     64-column tiles that meet [x0, x1), their first pass reads only rows [y0, y1), and the row transforms read
+ *     only those columns.  The image is complete
  *     and equals idgx_grid_fft's bit for bit.
  *   FORWARD (image -> uv): only the box's cells of the uv grid are produced (with the rest of the 64-column tiles
  *     around it), equal to idgx_grid_fft's to FP32 rounding: the row transforms run first, dense and in place,
@@ -261,8 +262,8 @@ int idgx_grid_box(const idgx_plan *plan, const idgx_metadata *h_metadata, int32_
 int idgx_grid_fft_box(const idgx_plan *plan, void *d_grid, int nr_layers, int direction, const int32_t box[4],
                       void *stream);
 
-/* Zeroes every cell that idgx_grid_fft_box(BACKWARD) reads for this box (the 64-column tiles that meet [x0, x1),
- * all rows; the whole grid for sizes outside the three-pass kernels' range), with a fill kernel: enough to start
+/* Zeroes every cell that idgx_grid_fft_box(BACKWARD) reads for this box (rows [y0, y1) of the 64-column tiles that
+ * meet [x0, x1); the whole grid for sizes outside the three-pass kernels' range), with a fill kernel: enough to start
  * an imaging cycle's grid when the adder writes only inside the box, even if a FORWARD transform left the rest of
  * the grid undefined. */
 int idgx_grid_clear_box(const idgx_plan *plan, void *d_grid, int nr_layers, const int32_t box[4], void *stream);

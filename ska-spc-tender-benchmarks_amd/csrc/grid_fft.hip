@@ -224,10 +224,11 @@ struct FftArgs
     int cx0, cx1;      // row pass: scratch columns outside [cx0, cx1) are read as zero (0, gx: dense)
     int ry0, nry;      // row pass: rows [ry0, ry0 + nry) of each plane (0, gy: dense)
     int rows_first;    // 1: the row pass runs first, in place (scratch == grid), and takes the input signs
+    int iy0, iy1;      // column pass 1: input rows outside [iy0, iy1) are zero and not read (0, gy: dense)
     // Column pass 2 stores row y, column x of plane pl at scratch[pl splane + y spitch + x - sx0]: (gx gy, gx, 0)
     // for whole planes, (gy nc, nc, first column) for a box of nc columns stored compactly.
-    long long splane;
     int spitch, sx0;
+    long long splane;
 };
 
 constexpr int ilog2(int n)
@@ -571,7 +572,8 @@ template <int L, int SG> __global__ __launch_bounds__(L / 16, (L >= 2048 ? 4 : 2
 // rows N2 k1 + n2 for one k1), tile index = x tile + (G / kColTile) x (n2 or k1), taken blockIdx.x, blockIdx.x +
 // gridDim.x, ...  The next tile is loaded into registers while the current one is transformed.  L = 16 kRl:
 // a radix-16 stage from global memory, one LDS exchange, a radix-kRl stage to global memory.
-template <int PASS, int L, int SG>
+// ROWS (column pass 1 of a box): input rows outside [iy0, iy1) are zero and not read.
+template <int PASS, int L, int SG, bool ROWS = false>
 __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, int nr_tiles)
 {
     constexpr int kLog = ilog2(L);
@@ -597,6 +599,12 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
         const int y = PASS == kPassCol1 ? A.N2 * i + q : A.N2 * q + i;
         return pl * plane_elems + static_cast<size_t>(y) * G + x;
     };
+    auto load = [&](int tile, int i) {
+        const int q = (tile / A.nct) & ((1 << log_q) - 1);
+        return static_cast<unsigned>(A.N2 * i + q - A.iy0) < static_cast<unsigned>(A.iy1 - A.iy0)
+                   ? in[element(tile, i)]
+                   : make_float2(0.F, 0.F);
+    };
 
     float2 nxt[16];
     int tile = blockIdx.x;
@@ -605,7 +613,7 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
         const int j = threadIdx.x / kColTile;
 #pragma unroll
         for (int r = 0; r < 16; ++r)
-            nxt[r] = in[element(tile, j + r * kQ)];
+            nxt[r] = ROWS ? load(tile, j + r * kQ) : in[element(tile, j + r * kQ)];
     }
     for (; tile < nr_tiles; tile += gridDim.x)
     {
@@ -626,7 +634,7 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
         {
 #pragma unroll
             for (int r = 0; r < 16; ++r)
-                nxt[r] = in[element(tn, j0 + r * kQ)];
+                nxt[r] = ROWS ? load(tn, j0 + r * kQ) : in[element(tn, j0 + r * kQ)];
         }
         dft<16, SG>(v);
         // Output r of butterfly j0 is element 16 j0 + r; the radix-kRl stage takes elements j0 + b kQ + r (L / kRl).
@@ -849,16 +857,27 @@ hipError_t resident_blocks(const void *fn, int threads, int work, int &blocks)
     return e;
 }
 
-template <int PASS, int L, int SG> hipError_t launch_cols(const FftArgs &A, int nr_tiles, hipStream_t st)
+template <int PASS, int L, int SG, bool ROWS>
+hipError_t launch_cols_kernel(const FftArgs &A, int nr_tiles, hipStream_t st)
 {
     constexpr int threads = kColTile * L / 16;
     int blocks = 0;
-    const hipError_t e = resident_blocks(reinterpret_cast<const void *>(&fft_col_kernel<PASS, L, SG>), threads,
+    const hipError_t e = resident_blocks(reinterpret_cast<const void *>(&fft_col_kernel<PASS, L, SG, ROWS>), threads,
                                          nr_tiles, blocks);
     if (e != hipSuccess)
         return e;
-    fft_col_kernel<PASS, L, SG><<<blocks, threads, 0, st>>>(A, nr_tiles);
+    fft_col_kernel<PASS, L, SG, ROWS><<<blocks, threads, 0, st>>>(A, nr_tiles);
     return hipGetLastError();
+}
+
+template <int PASS, int L, int SG> hipError_t launch_cols(const FftArgs &A, int nr_tiles, hipStream_t st)
+{
+    if constexpr (PASS == kPassCol1)
+    {
+        if (A.iy0 > 0 || A.iy1 < A.gy)
+            return launch_cols_kernel<PASS, L, SG, true>(A, nr_tiles, st);
+    }
+    return launch_cols_kernel<PASS, L, SG, false>(A, nr_tiles, st);
 }
 
 template <int PASS, int SG> hipError_t launch_col_pass(const FftArgs &A, int L, int nr_tiles, hipStream_t st)
@@ -1067,6 +1086,8 @@ FftArgs fft_args(int gx, int gy, bool centred)
     A.splane = static_cast<long long>(gx) * gy;
     A.spitch = gx;
     A.sx0 = 0;
+    A.iy0 = 0;
+    A.iy1 = gy;
     return A;
 }
 
@@ -1098,6 +1119,8 @@ hipError_t stockham_fft2d(const float2 *src, float2 *dst, int gx, int gy, long l
         A.nct = std::max(0, t1 - t0);
         A.cx0 = t0 * kColTile;
         A.cx1 = std::max(A.cx0, t1 * kColTile);
+        A.iy0 = box.y0;
+        A.iy1 = std::max(box.y0, box.y1);
     }
     A.rows_first = 0;
     if (box.mode == kRowsOut)
@@ -1209,35 +1232,41 @@ bool grid_fft_supported(int G)
     return G >= 4 && G % 4 == 0;
 }
 
-// Zeroes columns [c0, c0 + width) of every row of every plane (width even: two cells per 16-byte store).
-__global__ __launch_bounds__(kThreads) void clear_band_kernel(float4 *grid, long long rows, int G, int c0, int width)
+// Zeroes columns [c0, c0 + width) of rows [y0, y0 + ny) of every plane; rows = planes x ny (width even: two
+// cells per 16-byte store).
+__global__ __launch_bounds__(kThreads) void clear_band_kernel(float4 *grid, long long rows, int G, int y0, int ny,
+                                                              int c0, int width)
 {
     const long long half = width / 2;
     const long long i = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
     if (i >= rows * half)
         return;
     const long long r = i / half, c = c0 / 2 + (i - r * half);
-    grid[r * (G / 2) + c] = make_float4(0.F, 0.F, 0.F, 0.F);
+    const long long plane = r / ny, y = y0 + (r - plane * ny);
+    grid[(plane * G + y) * (G / 2) + c] = make_float4(0.F, 0.F, 0.F, 0.F);
 }
 
 hipError_t launch_grid_clear_box(void *grid, int G, long long nr_layers, const int *box, hipStream_t st)
 {
     const long long planes = nr_layers * kCorr;
-    int c0 = 0, c1 = G;
+    int c0 = 0, c1 = G, y0 = 0, y1 = G;
     if (stockham_size(G))
     {
-        // The columns the boxed BACKWARD transform reads: whole column tiles around [x0, x1).
+        // The cells the boxed BACKWARD transform reads: the box's rows of the column tiles around [x0, x1).
         c0 = box[0] / kColTile * kColTile;
         c1 = std::max(c0, (box[1] + kColTile - 1) / kColTile * kColTile);
+        y0 = box[2];
+        y1 = std::max(y0, box[3]);
     }
-    const long long items = planes * G * ((c1 - c0) / 2);
+    const long long ny = y1 - y0;
+    const long long items = planes * ny * ((c1 - c0) / 2);
     if (items == 0)
         return hipSuccess;
     const long long blocks = (items + kThreads - 1) / kThreads;
     if (blocks > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
-    clear_band_kernel<<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(static_cast<float4 *>(grid), planes * G, G,
-                                                                          c0, c1 - c0);
+    clear_band_kernel<<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(
+        static_cast<float4 *>(grid), planes * ny, G, y0, static_cast<int>(ny), c0, c1 - c0);
     return hipGetLastError();
 }
 
@@ -1388,6 +1417,8 @@ hipError_t radix3_fft2d(const float2 *src, float2 *dst, int N, long long planes,
     A.splane = plane; // (fft_col3_kernel stores with its own layout)
     A.spitch = N;
     A.sx0 = 0;
+    A.iy0 = 0;
+    A.iy1 = M;
     float2 *sc = A.scratch;
     for (long long pl = 0; pl < planes && e == hipSuccess; ++pl)
     {
@@ -1586,6 +1617,8 @@ hipError_t launch_grid_wstack(void *grid, void *image, int G, int z_min, int nr_
                 C.nplanes = 1;
                 C.ct0 = t0;
                 C.nct = t1 - t0;
+                C.iy0 = box[2];
+                C.iy1 = std::max(box[2], box[3]);
                 e = launch_col_pass<kPassCol1, -1>(C, C.N1, C.nct * C.N2, st);
                 if (e == hipSuccess)
                     e = launch_col_pass<kPassCol2, -1>(C, C.N2, C.nct * C.N1, st);

@@ -191,7 +191,7 @@ bool grid_fft_supported(int G);
 // box, FORWARD produces only the box's cells.  Sizes outside the three-pass kernels' range ignore it.
 hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward, hipStream_t st,
                            const int *box = nullptr);
-// Zeroes the cells a boxed BACKWARD transform reads (whole 64-column tiles around [x0, x1), every row; the whole
+// Zeroes the cells a boxed BACKWARD transform reads (rows [y0, y1) of the 64-column tiles around [x0, x1); the whole
 // grid for sizes outside the three-pass kernels' range).
 hipError_t launch_grid_clear_box(void *grid, int G, long long nr_layers, const int *box, hipStream_t st);
 // The same transform (no box) with the scratch in a file-scope array instead of a plane allocated on first use:
