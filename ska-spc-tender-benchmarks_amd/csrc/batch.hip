@@ -70,7 +70,7 @@ struct idgx_batch
     // Degridder pixels prepared once per subgrid and pass (per class): split words, FP16 scales, slot n table.
     uint4 *d_prepared[idgx::kMaxClasses] = {};
     int *d_prepared_scale[idgx::kMaxClasses] = {};
-    float *d_slot_n[idgx::kMaxClasses] = {};
+    float4 *d_slot_coord[idgx::kMaxClasses] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
 };
@@ -367,7 +367,7 @@ int make_queue_args(const idgx_batch *b, const idgx_batch_data *d, bool gridder,
         {
             a.prepared = b->d_prepared[k];
             a.prepared_scale = b->d_prepared_scale[k];
-            a.slot_n = b->d_slot_n[k];
+            a.slot_coord = b->d_slot_coord[k];
         }
     }
     return IDGX_OK;
@@ -493,7 +493,7 @@ extern "C" int idgx_batch_create(const idgx_batch_class *classes, int nr_classes
         if (e == hipSuccess)
             e = hipMalloc(&b->d_prepared_scale[k], n * sizeof(int));
         if (e == hipSuccess)
-            e = hipMalloc(&b->d_slot_n[k], static_cast<size_t>(S) * idgx::degrid_row_stride(S) * sizeof(float));
+            e = hipMalloc(&b->d_slot_coord[k], static_cast<size_t>(S) * idgx::degrid_row_stride(S) * sizeof(float4));
     }
     int rc = to_status(e);
     for (int k = 0; k < nr_classes && rc == IDGX_OK; ++k)
@@ -851,8 +851,8 @@ extern "C" void idgx_batch_destroy(idgx_batch *b)
             (void)hipFree(b->d_prepared[k]);
         if (b->d_prepared_scale[k])
             (void)hipFree(b->d_prepared_scale[k]);
-        if (b->d_slot_n[k])
-            (void)hipFree(b->d_slot_n[k]);
+        if (b->d_slot_coord[k])
+            (void)hipFree(b->d_slot_coord[k]);
     }
     delete b;
 }

@@ -29,10 +29,11 @@ struct KernelArgs
     const long long *vis_offset;
     // Mixed degridder pixels prepared once per subgrid (null: every work item computes them).  Per pixel slot
     // 8 words: for each correlation the packed high and low (re, im) parts of taper x A-term-applied pixel,
-    // FP16 forms scaled by 2^-prepared_scale[s]; slot_n holds n of every slot (shared by the class).
+    // FP16 forms scaled by 2^-prepared_scale[s]; slot_coord holds (x', y', n, 0) of every slot (shared by the
+    // class; n = 0 in the padding slots of a row).
     const uint4 *prepared;
     const int *prepared_scale;
-    const float *slot_n;
+    const float4 *slot_coord;
 };
 
 // Persistent-kernel work over up to kMaxClasses subgrid classes (one subgrid size each): the items are
@@ -167,7 +168,7 @@ int degrid_visibilities_per_item(int shape);             // visibilities one deg
 hipError_t launch_grid_mixed_queue(const QueueArgs &q, int precision, int shape, int workgroups, hipStream_t st);
 hipError_t launch_degrid_mixed_queue(const QueueArgs &q, int precision, int shape, int workgroups, hipStream_t st);
 hipError_t launch_reset_counters(int *counters, int n, hipStream_t st); // counters[0 .. n) = 0
-// Degridder pixel preparation for a.prepared / a.prepared_scale / a.slot_n (writes through those pointers).
+// Degridder pixel preparation for a.prepared / a.prepared_scale / a.slot_coord (writes through those pointers).
 size_t degrid_prepared_bytes(int S, long long nr_subgrids); // bytes of a.prepared
 hipError_t launch_degrid_prepare(const KernelArgs &a, int precision, hipStream_t st);
 int queue_workgroups(int precision, bool gridder, int shape); // resident workgroups for a persistent launch

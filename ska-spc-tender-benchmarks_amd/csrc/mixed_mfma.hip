@@ -612,7 +612,7 @@ __device__ __forceinline__ void gridder_item(const KernelArgs &a, int s, int pbl
 // through LDS PB at a time (taper and A-term applied, split into B fragments); the accumulators stay in
 // registers across batches, so each visibility is written once.
 // ---------------------------------------------------------------------------------------------------------------
-// PREP: the pixels come prepared (a.prepared, a.prepared_scale, a.slot_n: the persistent kernels, whose batch
+// PREP: the pixels come prepared (a.prepared, a.prepared_scale, a.slot_coord: the persistent kernels, whose batch
 // prepares them before every pass); otherwise the item applies taper and A-terms as it stages them (the one-shot
 // kernel).  A compile-time choice, so the persistent kernel carries no registers for the staging it never runs.
 template <int DT, bool SPLIT, int VT, int PB, bool PM, bool PREP>
@@ -714,10 +714,10 @@ __device__ __forceinline__ void degridder_item(const KernelArgs &a, int s, int v
                     const uint4 w0 = src[2 * q], w1 = src[2 * q + 1];
                     hi[0] = w0.x, lo[0] = w0.y, hi[1] = w0.z, lo[1] = w0.w;
                     hi[2] = w1.x, lo[2] = w1.y, hi[3] = w1.z, lo[3] = w1.w;
-                    const int sy = (pb + q) / Sp;
-                    cx = pixel_offset(pb + q - sy * Sp, S);
-                    cy = pixel_offset(sy, S);
-                    cn = a.slot_n[pb + q];
+                    const float4 c = a.slot_coord[pb + q];
+                    cx = c.x;
+                    cy = c.y;
+                    cn = c.z;
                 }
                 const int blk = q >> 4, qq = q & 15;
                 write_b_words(s_bw + blk * 256, qq, hi, lo);
@@ -992,7 +992,8 @@ __global__ void reset_counters_kernel(int *counters, int n)
 
 // One workgroup per subgrid: the taper x A-term-applied pixels of every slot, split into the packed words the
 // degridder stages (FP16: scaled by 2^-scale so that the largest value is about 2^14).  The workgroup of subgrid 0
-// also writes the slot n table.  Subgrids outside the arrays are skipped (the degridder skips them too).  Each XCD
+// also writes the slot table (x', y', n), so that the degridder's staging reads a slot's coordinates instead of
+// dividing by the row stride.  Subgrids outside the arrays are skipped (the degridder skips them too).  Each XCD
 // takes a contiguous run of subgrids (plan order: baseline by baseline), so the stations' A-terms it reads stay in
 // its own L2 instead of every XCD streaming the whole table.
 template <int DT, bool SPLIT, int CACHED>
@@ -1001,14 +1002,15 @@ __global__ __launch_bounds__(kThreads) void degrid_prepare_kernel(KernelArgs a)
     __shared__ unsigned maxbits;
     const int s = xcd_contiguous(blockIdx.x, gridDim.x), tid = threadIdx.x;
     const int S = a.p.S, P = S * S, Sp = degrid_row_stride(S), nslots = S * Sp;
-    float *slot_n = const_cast<float *>(a.slot_n);
+    float4 *slot_coord = const_cast<float4 *>(a.slot_coord);
     if (s == 0)
     {
         for (int q = tid; q < nslots; q += kThreads)
         {
             const int sy = q / Sp, sx = q - sy * Sp;
-            slot_n[q] = sx < S ? pixel_n_f(pixel_lm_f(sx, S, a.p.image_size), pixel_lm_f(sy, S, a.p.image_size))
-                               : 0.F;
+            const float n = sx < S ? pixel_n_f(pixel_lm_f(sx, S, a.p.image_size), pixel_lm_f(sy, S, a.p.image_size))
+                                   : 0.F;
+            slot_coord[q] = make_float4(pixel_offset(sx, S), pixel_offset(sy, S), n, 0.F);
         }
     }
     const idgx_metadata m = a.metadata[s];
@@ -1269,7 +1271,7 @@ hipError_t launch_queue(const QueueArgs &q, int precision, bool gridder, int sha
         return hipErrorInvalidValue;
     // The persistent degridder reads prepared pixels only (launch_degrid_prepare fills them first).
     for (int k = 0; k < kMaxClasses && !gridder; ++k)
-        if (q.cls[k].p.nr_subgrids > 0 && (!q.cls[k].prepared || !q.cls[k].prepared_scale || !q.cls[k].slot_n))
+        if (q.cls[k].p.nr_subgrids > 0 && (!q.cls[k].prepared || !q.cls[k].prepared_scale || !q.cls[k].slot_coord))
             return hipErrorInvalidValue;
     QueueArgs arg = q;
     void *args[] = {&arg};
@@ -1379,7 +1381,7 @@ hipError_t launch_degrid_prepare(const KernelArgs &a, int precision, hipStream_t
 {
     if (a.p.nr_subgrids <= 0)
         return hipSuccess;
-    if (!a.prepared || !a.prepared_scale || !a.slot_n)
+    if (!a.prepared || !a.prepared_scale || !a.slot_coord)
         return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>(a.p.nr_subgrids));
     // FP16 needs the subgrid's maximum before the split: hold up to 4, 9 or 16 slots per thread, so that subgrids
