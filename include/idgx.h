@@ -77,7 +77,8 @@ typedef enum
 typedef enum
 {
     /* Plain FP32 path: evaluate sin/cos with the ROCm device library's sincosf (full-range reduction, ~1 ulp)
-     * instead of the default FP32 Cody-Waite reduction to [-pi, pi] followed by v_sin_f32/v_cos_f32. */
+     * instead of the default FP32 reduction to a fraction of a turn (two FMAs with a two-part 1/(2 pi)) followed by
+     * v_sin_f32/v_cos_f32. */
     IDGX_FLAG_ACCURATE_SINCOS = 1,
     /* Mixed forms: evaluate the per-pair phase qu x' + qv y' + dw n as an FP32 MFMA (v_mfma_f32_16x16x4_f32)
      * instead of three VALU fused multiply-adds.  Same accuracy; a tuning choice. */

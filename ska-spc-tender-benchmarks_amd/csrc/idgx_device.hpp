@@ -88,14 +88,12 @@ __device__ __forceinline__ Offsets degridder_offsets(const idgx_metadata &m, int
 enum SincosMode
 {
     kSincosOcml = 0, // ROCm device library sincosf: full range reduction, ~1 ulp
-    kSincosHw = 1    // two-constant Cody-Waite reduction to [-pi, pi] in FP32 FMAs, then v_sin/v_cos_f32
+    kSincosHw = 1    // reduction to a fraction of a turn in FP32 FMAs (two-constant 1/(2 pi)), then v_sin/v_cos_f32
 };
 
-// 2*pi split for the reduction: kTwoPiHi = float(2*pi), kTwoPiLo = 2*pi - kTwoPiHi.  With fused multiply-adds the
-// reduced argument is exact to ~2 ulp of pi for |k| < 2^20 (the product k*kTwoPiHi is exact inside the FMA).
-constexpr float kTwoPiHi = 6.28318548202514648f;
-constexpr float kTwoPiLo = -1.74845553146951e-07f;
+// 1/(2 pi) split for the reduction: kInvTwoPi = float(1/(2 pi)), kInvTwoPiLo = 1/(2 pi) - kInvTwoPi.
 constexpr float kInvTwoPi = 0.159154943091895336f;
+constexpr float kInvTwoPiLo = 6.42063816e-09f;
 
 template <int MODE> __device__ __forceinline__ void sincos_rad(float x, float &s, float &c)
 {
@@ -105,10 +103,11 @@ template <int MODE> __device__ __forceinline__ void sincos_rad(float x, float &s
     }
     else
     {
+        // x / (2 pi) - k in revolutions: x kInvTwoPi - k is formed exactly inside the first FMA and rounded once
+        // (|result| <= ~0.5), the low part of 1/(2 pi) added by the second; absolute error below 6e-8 revolutions
+        // for |x| < 5e5, against 7e-8 for a Cody-Waite reduction in radians followed by a scaling to revolutions.
         const float k = __builtin_rintf(x * kInvTwoPi);
-        float r = __builtin_fmaf(-k, kTwoPiHi, x);
-        r = __builtin_fmaf(-k, kTwoPiLo, r);
-        const float rev = r * kInvTwoPi; // |rev| <= ~0.5 revolutions
+        const float rev = __builtin_fmaf(x, kInvTwoPiLo, __builtin_fmaf(x, kInvTwoPi, -k));
         s = __builtin_amdgcn_sinf(rev);
         c = __builtin_amdgcn_cosf(rev);
     }

@@ -196,8 +196,9 @@ template <int NCH, int SC>
 __global__ __launch_bounds__(kThreads, (degrid_plain_min_blocks<NCH, SC>())) void degridder_plain(KernelArgs a,
                                                                                                   int per)
 {
-    __shared__ float4 s_pix[2 * kDegridPixBatch];
-    __shared__ float4 s_geo[kDegridPixBatch]; // l, m, n, phase offset
+    // Per staged pixel: (l, m, n, phase offset), then the four correlations, side by side (one address register
+    // serves the pixel loop's three reads).
+    __shared__ float4 s_px[3 * kDegridPixBatch];
 
     // Workgroup b = vblock * n + s: a subgrid's workgroups are spread over the XCDs (balances uneven subgrids).
     const int vblock = blockIdx.x / a.p.nr_subgrids;
@@ -274,21 +275,21 @@ __global__ __launch_bounds__(kThreads, (degrid_plain_min_blocks<NCH, SC>())) voi
                 apply_a1h_p_a2(px, a1, a2);
             else
                 apply_a1_p_a2h(px, a1, a2);
-            s_pix[2 * q] = make_float4(px[0].re, px[0].im, px[1].re, px[1].im);
-            s_pix[2 * q + 1] = make_float4(px[2].re, px[2].im, px[3].re, px[3].im);
+            s_px[3 * q + 1] = make_float4(px[0].re, px[0].im, px[1].re, px[1].im);
+            s_px[3 * q + 2] = make_float4(px[2].re, px[2].im, px[3].re, px[3].im);
             const float l = pixel_lm(x, S, a.p.image_size);
             const float mm = pixel_lm(y, S, a.p.image_size);
             const float n = pixel_n(l, mm);
-            s_geo[q] = make_float4(l, mm, n, (off.u * l + off.v * mm) + off.w * n);
+            s_px[3 * q] = make_float4(l, mm, n, (off.u * l + off.v * mm) + off.w * n);
         }
         __syncthreads();
         if (active)
         {
             for (int q = 0; q < np; ++q)
             {
-                const float4 g = s_geo[q];
-                const float4 pa = s_pix[2 * q];
-                const float4 pbv = s_pix[2 * q + 1];
+                const float4 g = s_px[3 * q];
+                const float4 pa = s_px[3 * q + 1];
+                const float4 pbv = s_px[3 * q + 2];
                 const float pidx = (u * g.x + v * g.y) + w * g.z;
 #pragma unroll
                 for (int j = 0; j < NCH; ++j)
@@ -330,17 +331,19 @@ constexpr int kQueueVis = 256; // gridder entries per LDS stage (one per thread)
 
 struct PlainGridShared
 {
-    float4 vis[2 * kQueueVis];
-    float4 run_uvw[kQueueVis];  // uvw of each run of entries that share a timestep
+    // Per staged entry: its four correlations (two float4), then (k, 0, 0, 0), side by side, so that the entry loop
+    // reads all three from one address register with constant offsets.
+    float4 ent[3 * kQueueVis];
+    float4 run_uvw[kQueueVis]; // uvw of each run of entries that share a timestep
     int run_start[kQueueVis + 1];
-    float k[kQueueVis];
     int wave_runs[kThreads / 64];
 };
 
+// Per staged pixel: (l, m, n, phase offset), then the four correlations (two float4), side by side, so that the
+// pixel loop reads all three from one address register with constant offsets.
 struct PlainDegridShared
 {
-    float4 pix[2 * kDegridPixBatch];
-    float4 geo[kDegridPixBatch]; // l, m, n, phase offset
+    float4 px[3 * kDegridPixBatch];
 };
 
 // Selected visibilities of subgrid s: the list entries (null = dense, every (t, c) of the window) and their count.
@@ -430,9 +433,9 @@ __device__ __forceinline__ void gridder_plain_item(const KernelArgs &a, int s, i
                 k = a.wavenumbers[c0 + c];
             }
             t = min(t, T - 1); // a skipped entry still names a row inside the subgrid
-            sm.vis[2 * tid] = va;
-            sm.vis[2 * tid + 1] = vb;
-            sm.k[tid] = k;
+            sm.ent[3 * tid] = va;
+            sm.ent[3 * tid + 1] = vb;
+            sm.ent[3 * tid + 2] = make_float4(k, 0.F, 0.F, 0.F);
             start = tid == 0;
             if (!start)
             {
@@ -477,9 +480,9 @@ __device__ __forceinline__ void gridder_plain_item(const KernelArgs &a, int s, i
                 pidx[i] = (uv.x * l[i] + uv.y * mm[i]) + uv.z * n[i];
             for (int e = e0; e < e1; ++e)
             {
-                const float k = sm.k[e];
-                const float4 va = sm.vis[2 * e];
-                const float4 vb = sm.vis[2 * e + 1];
+                const float4 va = sm.ent[3 * e];
+                const float4 vb = sm.ent[3 * e + 1];
+                const float k = sm.ent[3 * e + 2].x;
 #pragma unroll
                 for (int i = 0; i < NPIX; ++i)
                 {
@@ -614,12 +617,12 @@ __device__ __forceinline__ void degridder_plain_item(const KernelArgs &a, int s,
                 apply_a1h_p_a2(px, a1, a2);
             else
                 apply_a1_p_a2h(px, a1, a2);
-            sm.pix[2 * q] = make_float4(px[0].re, px[0].im, px[1].re, px[1].im);
-            sm.pix[2 * q + 1] = make_float4(px[2].re, px[2].im, px[3].re, px[3].im);
+            sm.px[3 * q + 1] = make_float4(px[0].re, px[0].im, px[1].re, px[1].im);
+            sm.px[3 * q + 2] = make_float4(px[2].re, px[2].im, px[3].re, px[3].im);
             const float l = pixel_lm(x, S, a.p.image_size);
             const float mm = pixel_lm(y, S, a.p.image_size);
             const float n = pixel_n(l, mm);
-            sm.geo[q] = make_float4(l, mm, n, (off.u * l + off.v * mm) + off.w * n);
+            sm.px[3 * q] = make_float4(l, mm, n, (off.u * l + off.v * mm) + off.w * n);
         }
         __syncthreads();
         if (!wave_live)
@@ -628,9 +631,9 @@ __device__ __forceinline__ void degridder_plain_item(const KernelArgs &a, int s,
         {
             for (int q = 0; q < np; ++q)
             {
-                const float4 g = sm.geo[q];
-                const float4 pa = sm.pix[2 * q];
-                const float4 pbv = sm.pix[2 * q + 1];
+                const float4 g = sm.px[3 * q];
+                const float4 pa = sm.px[3 * q + 1];
+                const float4 pbv = sm.px[3 * q + 2];
                 const float pidx = (us * g.x + vs * g.y) + ws * g.z;
 #pragma unroll
                 for (int j = 0; j < NCH; ++j)
@@ -646,9 +649,9 @@ __device__ __forceinline__ void degridder_plain_item(const KernelArgs &a, int s,
         {
             for (int q = 0; q < np; ++q)
             {
-                const float4 g = sm.geo[q];
-                const float4 pa = sm.pix[2 * q];
-                const float4 pbv = sm.pix[2 * q + 1];
+                const float4 g = sm.px[3 * q];
+                const float4 pa = sm.px[3 * q + 1];
+                const float4 pbv = sm.px[3 * q + 2];
 #pragma unroll
                 for (int j = 0; j < NCH; ++j)
                 {
