@@ -321,7 +321,9 @@ int idgx_wstack_from_image(const idgx_plan *plan, const void *d_image, int z_min
  * planes nor a separate w-screen pass touch HBM.  The screens there take v_sin / v_cos of the FP64-reduced fraction
  * of a turn (about 1e-7 absolute), not FP64 sincospi.  idgx_grid_to_image keeps a scratch of
  * nr_layers x G x (box width rounded out to 64-column tiles) cells per device, allocated on first use and grown
- * (after a device synchronisation) when a call needs more.  Other sizes run the two calls one after the other.
+ * (after a device synchronisation, which also makes a growing call unsuitable for hipGraph capture) when a call
+ * needs more; calls that overlap on one device share it, so issue them from one host thread on one stream.  Other
+ * sizes run the two calls one after the other.
  * No counterpart in the reference (its benchmark stops at the subgrids).  d_image must not alias d_grid. */
 int idgx_grid_to_image(const idgx_plan *plan, void *d_grid, int z_min, int nr_layers, const int32_t box[4],
                        void *d_image, void *stream);
