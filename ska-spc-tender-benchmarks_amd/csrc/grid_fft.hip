@@ -568,7 +568,7 @@ template <int L, int SG> __global__ __launch_bounds__(L / 16, (L >= 2048 ? 4 : 2
     }
 }
 
-// Column passes, persistent: tiles of kColTile columns x one transform set (C1: rows N2 n1 + n2 for one n2; C2:
+// Column passes, persistent: tiles of kColTile columns x one transform set (pass 1: rows N2 n1 + n2 for one n2; pass 2:
 // rows N2 k1 + n2 for one k1), tile index = x tile + (G / kColTile) x (n2 or k1), taken blockIdx.x, blockIdx.x +
 // gridDim.x, ...  The next tile is loaded into registers while the current one is transformed.  L = 16 kRl:
 // a radix-16 stage from global memory, one LDS exchange, a radix-kRl stage to global memory.
@@ -583,7 +583,7 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
     __shared__ float buf[L * kColTile];
     const int c = threadIdx.x & (kColTile - 1);
     const int G = A.gx; // row stride
-    // Tile index = (plane x Q + q) x nct + (column tile - ct0), q = n2 (C1) or k1 (C2), Q = N2 or N1.
+    // Tile index = (plane x Q + q) x nct + (column tile - ct0), q = n2 (pass 1) or k1 (pass 2), Q = N2 or N1.
     const int log_q = PASS == kPassCol1 ? A.log_n2 : A.log_n1;
     const size_t plane_elems = static_cast<size_t>(G) * A.gy;
     const float2 *in = PASS == kPassCol1 ? A.src : A.grid;
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col_kernel(FftArgs A, i
         const size_t base = pl * plane_elems;
         const size_t sbase = pl * static_cast<size_t>(A.splane) - A.sx0;
         float2 v[16];
-        // C1 applies the input sign (-1)^(x + y), y = N2 n1 + n2 with N2 even: (-1)^(x + n2) for the whole thread.
+        // Pass 1 applies the input sign (-1)^(x + y), y = N2 n1 + n2 with N2 even: (-1)^(x + n2) for the whole thread.
         const float sg_in = PASS == kPassCol1 && A.centred && !A.rows_first && ((x + q) & 1) ? -1.F : 1.F;
 #pragma unroll
         for (int r = 0; r < 16; ++r)
@@ -707,11 +707,11 @@ template <int SG> __device__ __forceinline__ void dft3(float2 a, float2 b, float
 //   X[3 k + r][.] = sum_{y < M} W_M^(k y) t_r[y][.],   t_r[y] = W_N^(r y) sum_{q < 3} W_3^(r q) X_x[y + M q][.].
 // The x-axis combination acts on whole columns and the y-axis steps on each column alone, so they commute: column
 // pass 1 (in place) forms t_0, t_1, t_2 from rows y, y + M, y + 2 M of the row pass's output and runs the first
-// four-step stage of their M-point transforms (as fft_col_kernel's C1), storing t_r's rows in row block r (rows
+// four-step stage of their M-point transforms (as fft_col_kernel's pass 1), storing t_r's rows in row block r (rows
 // [r M, r M + M)): a tile writes the rows it read.  Column pass 2 reads columns k, k + M, k + 2 M of a row block's
-// rows, forms the x-axis combination, runs the second stage (C2) on each of its three results q and stores them at
-// columns [q M, q M + M) of output row 3 (k1 + N1 k2) + r.  Every pass reads and writes whole cache lines.  No
-// prefetch of the next tile: a thread already holds three sequences.
+// rows, forms the x-axis combination, runs the second stage (as pass 2) on each of its three results q and stores
+// them at columns [q M, q M + M) of output row 3 (k1 + N1 k2) + r.  Every pass reads and writes whole cache lines.
+// No prefetch of the next tile: a thread already holds three sequences.
 template <int PASS, int L, int SG>
 __global__ __launch_bounds__(kColTile * L / 16) void fft_col3_kernel(FftArgs A, int nr_tiles)
 {
@@ -733,14 +733,14 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col3_kernel(FftArgs A, 
     {
         int j0 = threadIdx.x / kColTile; // opaque per tile: keeps twiddles and slots out of registers across tiles
         asm volatile("" : "+v"(j0));
-        // Tile index = (blk x Q + q) x nct + column tile: C1 blk = plane, q = n2, nct = N / kColTile; C2 blk =
+        // Tile index = (blk x Q + q) x nct + column tile: pass 1 blk = plane, q = n2, nct = N / kColTile; pass 2 blk =
         // 3 plane + row block, q = k1, nct = M / kColTile.
         const int rest = tile / A.nct;
         const int x0 = (tile - rest * A.nct) * kColTile; // the tile's first column
         const int x = x0 + c;
         const int q = rest & ((1 << log_q) - 1);
         const int blk = rest >> log_q;
-        // Element r of sequence qq: C1 row N2 (j0 + r kQ) + q + M qq, column x; C2 row N2 q + j0 + r kQ, column
+        // Element r of sequence qq: pass 1 row N2 (j0 + r kQ) + q + M qq, column x; pass 2 row N2 q + j0 + r kQ, column
         // x + M qq.  Addresses: a pointer that is uniform over the workgroup (tile, r, qq) plus the thread's 32-bit
         // offset (j0 rows of the pass's row step, c columns).
         const float2 *in = PASS == kPassCol1 ? A.src + 3 * static_cast<size_t>(blk) * block_elems
@@ -759,8 +759,8 @@ __global__ __launch_bounds__(kColTile * L / 16) void fft_col3_kernel(FftArgs A, 
                                                    : static_cast<size_t>(r * kQ) * N + static_cast<size_t>(M) * qq;
                 t[qq][r] = (in + u)[toff];
             }
-        // The radix-3 steps: C1 twiddles the butterfly's outputs by W_N^(r y), C2 its inputs by W_N^(b k) (y, k < M,
-        // so 2 y, 2 k < N).
+        // The radix-3 steps: pass 1 twiddles the butterfly's outputs by W_N^(r y), pass 2 its inputs by W_N^(b k)
+        // (y, k < M, so 2 y, 2 k < N).
 #pragma unroll
         for (int r = 0; r < 16; ++r)
         {

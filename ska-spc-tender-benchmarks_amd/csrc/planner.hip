@@ -191,8 +191,8 @@ __global__ __launch_bounds__(kThreads) void plan_write_kernel(PlannerArgs A)
 
 size_t plan_workspace_bytes(long long nr_segments)
 {
-    return static_cast<size_t>(nr_segments) * sizeof(int32_t) + static_cast<size_t>(nr_segments + 1) * sizeof(long long) +
-           64;
+    return static_cast<size_t>(nr_segments) * sizeof(int32_t) +
+           static_cast<size_t>(nr_segments + 1) * sizeof(long long) + 64;
 }
 
 hipError_t launch_plan(const idgx_planner &pl, const float *uvw, const double *frequencies, const uint32_t *station1,

@@ -773,16 +773,16 @@ template <int N1, int N2, int SG, class T>
 hipError_t four_step_columns(const void *src, void *dst, void *scratch, int nx, long long planes, hipStream_t st)
 {
     using V = typename Vec<T>::type;
-    constexpr int C1 = column_tile<N1, T>(), C2 = column_tile<N2, T>();
-    if (nx % C1 != 0 || nx % C2 != 0)
+    constexpr int CT1 = column_tile<N1, T>(), CT2 = column_tile<N2, T>();
+    if (nx % CT1 != 0 || nx % CT2 != 0)
         return hipErrorInvalidValue;
-    const long long ta = planes * N2 * (nx / C1), tb = planes * N1 * (nx / C2);
-    fft_columns_a_kernel<N1, N2, SG, T><<<grid_for(ta), N1 * C1 / kPointsPerThread, 0, st>>>(
+    const long long ta = planes * N2 * (nx / CT1), tb = planes * N1 * (nx / CT2);
+    fft_columns_a_kernel<N1, N2, SG, T><<<grid_for(ta), N1 * CT1 / kPointsPerThread, 0, st>>>(
         static_cast<const V *>(src), static_cast<V *>(dst), nx, ta);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
-    fft_columns_b_kernel<N1, N2, SG, T><<<grid_for(tb), N2 * C2 / kPointsPerThread, 0, st>>>(
+    fft_columns_b_kernel<N1, N2, SG, T><<<grid_for(tb), N2 * CT2 / kPointsPerThread, 0, st>>>(
         static_cast<const V *>(dst), static_cast<V *>(scratch), nx, tb);
     return hipGetLastError();
 }
