@@ -149,9 +149,12 @@ def main():
              f"{f_fetch:.3f}, WRITE_SIZE factor {f_write:.3f} (true bytes / counter bytes).", ""]
     if a.command:
         lines += ["Commands:", "", "```", a.command, "```", ""]
-    lines += ["| kernel | launches | HBM read GB/launch | HBM write GB/launch | VALU instr | TRANS_F32 | FMA_F32 | "
-              "MFMA MOPs F16 | MFMA MOPs BF16 |",
-              "|---|---|---|---|---|---|---|---|---|"]
+    if sq:
+        lines += ["| kernel | launches | HBM read GB/launch | HBM write GB/launch | VALU instr | TRANS_F32 | FMA_F32 | "
+                  "MFMA MOPs F16 | MFMA MOPs BF16 |",
+                  "|---|---|---|---|---|---|---|---|---|"]
+    else:  # traffic only
+        lines += ["| kernel | launches | HBM read GB/launch | HBM write GB/launch |", "|---|---|---|---|"]
     for k in sorted(set(fetch) | set(write)):
         if k == "copy":
             continue
@@ -162,6 +165,9 @@ def main():
                              **{c: mean(v) for c, v in s.items()}}
         n = len(fetch[k]["FETCH_SIZE"]) if k in fetch else len(write[k]["WRITE_SIZE"])
         res["kernels"][k]["launches"] = n
+        if not sq:
+            lines.append(f"| `{k}` | {n} | {rd / 1e9:.2f} | {wr / 1e9:.2f} |")
+            continue
         lines.append(f"| `{k}` | {n} | {rd / 1e9:.2f} | {wr / 1e9:.2f} | {mean(s.get('SQ_INSTS_VALU', [])):.3e} | "
                      f"{mean(s.get('SQ_INSTS_VALU_TRANS_F32', [])):.3e} | {mean(s.get('SQ_INSTS_VALU_FMA_F32', [])):.3e} | "
                      f"{mean(s.get('SQ_INSTS_VALU_MFMA_MOPS_F16', [])):.3e} | "
