@@ -356,9 +356,13 @@ def test_grid_to_image_and_back_match_the_separate_calls(G):
         e64 = rel_l2(img[1], ref)
         print(f"G={G}: grid_to_image vs FP64 {e64:.2e}")
         assert e64 < 1e-6
-    # Empty box: zero layers give a zero image (every cell written).
+    # Empty box: zero layers give a zero image (every cell written); from the image, the grid is not touched.
     img0, _ = to_image_both(np.zeros(shape, np.complex64), (0, 0, 0, 0))
     assert not img0.any()
+    d_g = dw.layer_buffer(nl)
+    d_g.fill_(7)
+    image_to_grid(plan, upload(X), z_min, nl, (0, 0, 0, 0), d_g)
+    assert bool((d_g == 7).all())
     # The whole grid, then a box smaller than the first (the scratch grown for the whole grid is reused).
     for box in ((0, G, 0, G), (64, 129, 7, 90)):
         x = layers_in(box)
