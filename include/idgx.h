@@ -458,8 +458,10 @@ int idgx_batch_get_single_launch(const idgx_batch *batch, int *gridder_shape, in
 
 /* Times every shape `repeats` times per class and direction on the given data (overwriting its outputs), keeps
  * each class's fastest and reports the sum over classes of their average times (ms).  A direction in
- * single-launch mode then also times every shape of its merged launch, keeps the fastest and reports that
- * launch's time. */
+ * single-launch mode then times, on the whole direction, its tuned per-class launches run together and every shape
+ * of its merged launch (gridder: also IDGX_SHAPE_TILES_PER_CLASS), keeps the fastest (per-class launches switch
+ * the direction to per-class mode, as idgx_batch_set_single_launch with -1 would) and reports that time.  A
+ * direction in per-class mode stays in it. */
 int idgx_batch_autotune(idgx_batch *batch, const idgx_batch_data *data, int repeats, void *stream,
                         float *best_gridder_ms, float *best_degridder_ms);
 

@@ -663,40 +663,42 @@ extern "C" int idgx_batch_autotune(idgx_batch *b, const idgx_batch_data *data, i
         return IDGX_ERR_DEVICE;
     }
     int rc = IDGX_OK;
+    // Average time (ms) of `repeats` runs of direction `gridder` after one warm-up (`only`: one class, or -1 for the
+    // direction as it is set up); each trial ends synchronised on its event.
+    const auto timed = [&](bool gridder, int only, float &ms) {
+        ms = 0.F;
+        int e = run(b, data, gridder, st, only); // warm-up
+        if (e == IDGX_OK && hipEventRecord(e0, st) != hipSuccess)
+            e = IDGX_ERR_DEVICE;
+        for (int r = 0; r < repeats && e == IDGX_OK; ++r)
+            e = run(b, data, gridder, st, only);
+        if (e == IDGX_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                             hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            e = IDGX_ERR_DEVICE;
+        ms /= repeats;
+        return e;
+    };
     float total[2] = {0.F, 0.F};
     // Each class is tuned on its own launch: the launches of a direction run one after another.
     for (int dir = 0; dir < 2 && rc == IDGX_OK; ++dir)
     {
         const bool gridder = dir == 0;
+        const bool offer_single = single_mode(b, dir);
         for (int k = 0; k < b->nr_classes && rc == IDGX_OK; ++k)
         {
             float best = -1.F;
             int best_shape = b->shape[dir][k];
             for (int sh = 0; sh < shape_count(b) && rc == IDGX_OK; ++sh)
             {
-                // One roctx range per trial (each ends synchronised on its event), for rocprofv3 --marker-trace.
+                // One roctx range per trial, for rocprofv3 --marker-trace.
                 char name[64];
                 std::snprintf(name, sizeof(name), "idgx autotune %s class %d shape %d",
                               gridder ? "gridder" : "degridder", k, sh);
                 roctxRangePushA(name);
                 rc = upload_queue(b, gridder, k, sh);
-                if (rc != IDGX_OK)
-                {
-                    roctxRangePop();
-                    break;
-                }
-                rc = run(b, data, gridder, st, k); // warm-up
-                if (rc != IDGX_OK)
-                    break;
-                if (hipEventRecord(e0, st) != hipSuccess)
-                    rc = IDGX_ERR_DEVICE;
-                for (int r = 0; r < repeats && rc == IDGX_OK; ++r)
-                    rc = run(b, data, gridder, st, k);
                 float ms = 0.F;
-                if (rc == IDGX_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                                      hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                    rc = IDGX_ERR_DEVICE;
-                ms /= repeats;
+                if (rc == IDGX_OK)
+                    rc = timed(gridder, k, ms);
                 roctxRangePop();
                 if (rc == IDGX_OK && (best < 0.F || ms < best))
                 {
@@ -708,35 +710,32 @@ extern "C" int idgx_batch_autotune(idgx_batch *b, const idgx_batch_data *data, i
                 rc = upload_queue(b, gridder, k, best_shape);
             total[dir] += std::max(best, 0.F);
         }
-        if (rc != IDGX_OK || !single_mode(b, dir))
+        if (rc != IDGX_OK || !offer_single)
             continue;
-        // Single-launch mode: the one shape of the merged queue (for the gridder also each class's tuned tiles in
-        // the one launch), timed on the whole direction.
+        // Single-launch mode: the tuned per-class launches run together (-1), then every shape of the merged queue
+        // (for the gridder also each class's tuned tiles in the one launch), all timed on the whole direction; the
+        // fastest of them is kept, which may leave the direction in per-class mode.
         float best = -1.F;
-        int best_shape = b->single_shape[dir];
-        for (int sh = gridder ? idgx::kShapeTilesPerClass : 0; sh < shape_count(b) && rc == IDGX_OK; ++sh)
+        int best_shape = -1;
+        static_assert(idgx::kShapeTilesPerClass == -2, "the trials below run -2 (tiles per class), -1, 0, 1, ...");
+        for (int sh = gridder ? idgx::kShapeTilesPerClass : -1; sh < shape_count(b) && rc == IDGX_OK; ++sh)
         {
             if (sh == -1)
-                continue;
-            rc = upload_single_queue(b, gridder, sh);
-            if (rc == IDGX_OK)
-                rc = run(b, data, gridder, st); // warm-up
-            if (rc == IDGX_OK && hipEventRecord(e0, st) != hipSuccess)
-                rc = IDGX_ERR_DEVICE;
-            for (int r = 0; r < repeats && rc == IDGX_OK; ++r)
-                rc = run(b, data, gridder, st);
+                b->single_shape[dir] = -1;
+            else
+                rc = upload_single_queue(b, gridder, sh);
             float ms = 0.F;
-            if (rc == IDGX_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                                  hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                rc = IDGX_ERR_DEVICE;
-            ms /= repeats;
+            if (rc == IDGX_OK)
+                rc = timed(gridder, -1, ms);
             if (rc == IDGX_OK && (best < 0.F || ms < best))
             {
                 best = ms;
                 best_shape = sh;
             }
         }
-        if (rc == IDGX_OK)
+        if (rc == IDGX_OK && best_shape == -1)
+            b->single_shape[dir] = -1;
+        else if (rc == IDGX_OK)
             rc = upload_single_queue(b, gridder, best_shape);
         total[dir] = std::max(best, 0.F);
     }
