@@ -228,9 +228,10 @@ int idgx_split_subgrids_layered(const idgx_plan *plan, const idgx_metadata *d_me
  * unnormalised (BACKWARD(FORWARD(I)) = G^2 I), the signs of idgx_subgrid_fft.  G must be a multiple of 4.  For
  * G = 1024 .. 16384 (powers of two) the transform is this library's three-pass kernel set; the first call for a G on
  * a device allocates a scratch of up to 64 MB, or one plane (G^2 x 8 B) when a plane is larger, kept for the life
- * of the process.  Other sizes use rocFFT
- * between two sign-flip passes (plans cached per device, G and batch; the first call for a size allocates them).
- * Calls that overlap on one device share the scratch, so issue them on one stream. */
+ * of the process.  Other sizes run idgx_fft2d's transform (in place, batched over layers and correlations) between
+ * two sign-flip passes: this library's kernels for G = 16 .. 512 and 3 x 2^k from 3072 (a scratch of one plane),
+ * rocFFT otherwise (plans cached per device, G and batch; the first call for a size allocates them).  Calls that
+ * overlap on one device share the scratch, so issue them on one stream. */
 int idgx_grid_fft(const idgx_plan *plan, void *d_grid, int nr_layers, int direction, void *stream);
 
 /* idgx_grid_fft that allocates nothing (no hipMalloc / hipFree, no library plan), for callers that cannot allocate:

@@ -23,8 +23,10 @@
 // writes whole cache lines: 3 x (read + write) of the grid, against 2 checkerboard passes plus the library
 // transform otherwise.  All three kernels are persistent (as many workgroups as are resident); the column kernels
 // load their next tile into registers while they transform the current one (their LDS barriers do not wait for
-// those loads), the row kernel holds only its row, which keeps two workgroups on a CU.  Other sizes use rocFFT
-// (hipFFT C2C, batched over correlations and layers) between two checkerboard passes.
+// those loads), the row kernel holds only its row, which keeps two workgroups on a CU.  Other sizes run the plain
+// batched 2-D transform (launch_fft2d, over correlations and layers: LDS or register kernels for sides 16 .. 512,
+// four-step columns for the other powers of two, the radix-3 three-pass set for 3 x 2^k from 3072, rocFFT for the
+// rest) between two checkerboard passes.
 //
 // W-stacking.  A subgrid with w-layer z is gridded relative to w_z = (z + 0.5) w_step (in wavelengths), so layer
 // z of the dirty image carries exp(+2 pi i w_z n(l, m)) relative to the true one, n = 1 - sqrt(1 - l^2 - m^2)
@@ -1304,19 +1306,14 @@ hipError_t launch_grid_fft(void *grid, int G, long long nr_layers, bool forward,
         return forward ? stockham_fft2d<1>(g, g, G, G, planes, true, st, b)
                        : stockham_fft2d<-1>(g, g, G, G, planes, true, st, b);
     }
-    hipfftHandle plan;
-    hipError_t e = fft_plan(G, G, planes, HIPFFT_C2C, plan);
+    // Other sizes: the plain 2-D transform in place between the two sign flips (this library's kernels for sides
+    // 16 .. 512, the other powers of two and 3 x 2^k from 3072; rocFFT for the rest).  FORWARD has the + sign.
+    hipError_t e = checkerboard(grid, planes, G, st);
     if (e != hipSuccess)
         return e;
-    e = checkerboard(grid, planes, G, st);
+    e = launch_fft2d(grid, grid, G, G, planes, forward ? 1 : -1, false, st);
     if (e != hipSuccess)
         return e;
-    if (hipfftSetStream(plan, st) != HIPFFT_SUCCESS)
-        return hipErrorUnknown;
-    // FORWARD has the + sign: hipFFT's BACKWARD (exp(+2 pi i ...)); BACKWARD the - sign: hipFFT's FORWARD.
-    hipfftComplex *d = static_cast<hipfftComplex *>(grid);
-    if (hipfftExecC2C(plan, d, d, forward ? HIPFFT_BACKWARD : HIPFFT_FORWARD) != HIPFFT_SUCCESS)
-        return hipErrorUnknown;
     return checkerboard(grid, planes, G, st);
 }
 

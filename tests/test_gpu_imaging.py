@@ -188,6 +188,33 @@ def test_three_pass_grid_fft_matches_numpy_fp64(G):
     assert e_f < 1e-6 and e_b < 1e-6 and e_rt < 1e-6
 
 
+@pytest.mark.parametrize("G", [256, 512, 3072, 1028])
+def test_other_size_grid_fft_matches_numpy_fp64(G):
+    """Sizes outside 1024 .. 16384 run the plain 2-D transform between the two sign flips: the LDS kernels (256: one
+    row pass and one column pass; 512 likewise), the radix-3 three-pass set (3072 = 3 x 1024) and rocFFT (1028).
+    Every correlation plane transformed, planes 0 and 3 compared with numpy's FP64 FFT both ways, and the round
+    trip."""
+    from idgx import FFT_FP32, fft2d_own_kernels
+
+    assert fft2d_own_kernels(G, G, FFT_FP32) == (G != 1028)
+    rng = np.random.default_rng(G)
+    x = (rng.standard_normal((4, G, G), np.float32) + 1j * rng.standard_normal((4, G, G), np.float32)).astype(
+        np.complex64)
+    plan = _fft_plan(G)
+    d = upload(x)
+    grid_fft(plan, d, 1, FFT_FORWARD)
+    fwd = download(d, np.complex64, x.shape)
+    e_f = max(rel_l2(fwd[p], _centred_np(x[p].astype(np.complex128), +1)) for p in (0, 3))
+    grid_fft(plan, d, 1, FFT_BACKWARD)
+    e_rt = rel_l2(download(d, np.complex64, x.shape) / np.float32(G * G), x)
+    d2 = upload(x)
+    grid_fft(plan, d2, 1, FFT_BACKWARD)
+    bwd = download(d2, np.complex64, x.shape)
+    e_b = max(rel_l2(bwd[p], _centred_np(x[p].astype(np.complex128), -1)) for p in (0, 3))
+    print(f"G={G} grid FFT vs numpy FP64: forward {e_f:.2e}, backward {e_b:.2e}, round trip {e_rt:.2e}")
+    assert e_f < 1e-6 and e_b < 1e-6 and e_rt < 1e-6
+
+
 def test_three_pass_grid_fft_at_16384():
     """The 16384^2 master grid (8.6 GB, the full-cycle configuration): four output rows of plane 0 against the
     FP64 centred DFT (column sums by matrix-vector product, then numpy's row FFT), Parseval on every plane, and the
