@@ -608,6 +608,10 @@ class Batch:
         return Batch.plain_shapes() if self.precision == PLAIN_FP32 else Batch.shapes()
 
     def autotune(self, data, repeats=2, stream=None):
+        """Times every shape per class and direction on `data` (overwriting its outputs) and keeps each class's
+        fastest.  A direction in single-launch mode then also times its per-class launches together and every shape
+        of the merged launch, and keeps the fastest mode (possibly per-class, as single_launch() then reports with
+        -1).  Returns (gridder ms, degridder ms) of what was kept."""
         g, d = ctypes.c_float(), ctypes.c_float()
         _check(lib().idgx_batch_autotune(self.handle, ctypes.byref(data), repeats, _stream(stream), ctypes.byref(g),
                                          ctypes.byref(d)), "idgx_batch_autotune")
